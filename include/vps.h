@@ -1,0 +1,291 @@
+/*
+ * vps.h -- C ABI of the MI355X virtual-particle SPH pressure projection.
+ *
+ * The library replaces the dual-particle ("virtual particle") projection of
+ * the reference solver:
+ *
+ *   DualParticleSphLiquid<Dim>::initialize / advance / applyPressureForce
+ *       (Cores/Physics/DualParticleSphLiquid.cpp:78-126)
+ *   VirtualParticle<Dim>::generateParticles / setAlpha / setKappa / project
+ *       (Cores/Structures/VirtualParticle.h:76-87, VirtualParticle.cpp:45-201)
+ *   SmoothedParticles<Dim>::resetNearbySearcher / computeDensities /
+ *   computeVolumes  (Cores/Structures/SmoothedParticles.h:53-55,110;
+ *                    SmoothedParticles.cpp:25-41)
+ *   HashGridSearcher<Dim>::reset / forEach
+ *       (Cores/Structures/ParticlesNearbySearcher.cpp:10-35)
+ *   IterativeSolver::solve  (Cores/Solvers/IterativeSolver.h:19-40)
+ *
+ * The reference exposes that path only as C++ template virtuals; a caller
+ * binds this ABI instead (plain pointers, sizes and status codes, no torch or
+ * Eigen types).  All host arrays use the reference memory layout: FP64,
+ * array-of-structures with stride Dim (an Eigen Vector<Dim,double> is Dim
+ * contiguous doubles).  The context owns every device buffer; calls on one
+ * context must be serialised by the caller, separate contexts are
+ * independent.  Errors are returned as status codes (the reference calls
+ * std::exit, IterativeSolver.h:28-38); vps_last_error() gives the message.
+ */
+#ifndef VPS_H
+#define VPS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VPS_ABI_VERSION 1
+
+/* Status codes returned by every entry point. */
+enum vps_status {
+    VPS_OK = 0,
+    VPS_ERR_INVALID_ARG = 1,   /* bad size, null pointer, empty set (the
+                                  reference divides by zero there,
+                                  ParticlesNearbySearcher.h:71 with .cpp:13) */
+    VPS_ERR_NO_CONVERGENCE = 2,/* CG hit max_iterations (reference: exit(-1)) */
+    VPS_ERR_NONFINITE = 3,     /* NaN/Inf in the right-hand side or solution */
+    VPS_ERR_OOM = 4,           /* device allocation failed */
+    VPS_ERR_HIP = 5,           /* any other HIP runtime error */
+    VPS_ERR_STATE = 6          /* call order violated (e.g. project before
+                                  initialize) */
+};
+
+/* Linear operator handed to CG.  Eigen's ConjugateGradient with
+ * UpLo = Lower|Upper on a column-major SparseMatrix multiplies by the
+ * transpose of the assembled matrix (IterativeSolver.h:12, Types.h:21); the
+ * assembled rows (VirtualParticle.cpp:149-169) are not symmetric because row
+ * I carries the volume V_J of its neighbour, so the two choices differ. */
+enum vps_operator {
+    VPS_OP_TRANSPOSE = 0,      /* M = A^T: what the reference actually runs */
+    VPS_OP_ROW = 1             /* M = A: the rows as written */
+};
+
+/* CG initial guess. */
+enum vps_warm_start {
+    VPS_WARM_INDEX_CARRY = 0,  /* x0[k] = p_prev[k] by position in the virtual
+                                  list, zero past its end: what
+                                  ParticlesBasedData::resize does to the
+                                  pressures (ParticlesBasedData.h:28 via
+                                  VirtualParticle.cpp:95) */
+    VPS_WARM_ZERO = 1,         /* x0 = 0 */
+    VPS_WARM_NODE_KEYED = 2,   /* x0 = previous pressure at the same lattice
+                                  node, zero for nodes that are new */
+    VPS_WARM_LEARNED = 3       /* x0 from the per-node MLP (no reference
+                                  counterpart); falls back to node keyed
+                                  until the network has been trained */
+};
+
+/* Where rho0 and kappa come from. */
+enum vps_rest_mode {
+    VPS_FROM_INIT = 0,         /* reference: rho0 = max real density at
+                                  initialize (DualParticleSphLiquid.cpp:91),
+                                  kappa = 1/packed kernel sum (:98) */
+    VPS_FROM_PARAMS = 1        /* use vps_params.rest_density / .kappa */
+};
+
+typedef struct vps_params {
+    int32_t dim;               /* 2 or 3 */
+    int32_t op_mode;           /* enum vps_operator */
+    int32_t warm_start;        /* enum vps_warm_start */
+    int32_t rest_density_mode; /* enum vps_rest_mode */
+    int32_t kappa_mode;        /* enum vps_rest_mode */
+    int32_t reserved0;
+    double particle_radius;    /* real and boundary kernel radius h = 4 r
+                                  (SmoothedParticles.h:45, kernel_rate 4) */
+    double lattice_spacing;    /* virtual kernel radius H = 4 s
+                                  (DualParticleSphLiquid.h:28) */
+    double lattice_origin[3];  /* node (0,0,0) position (Grid.h:51) */
+    int32_t lattice_dims[3];   /* node counts per axis; dims[2] = 1 in 2D */
+    int32_t reserved1;
+    double mass;               /* real particle mass; boundary mass = mass
+                                  (DualParticleSphLiquid.cpp:84) */
+    double rest_density;       /* used when rest_density_mode = FROM_PARAMS */
+    double kappa;              /* used when kappa_mode = FROM_PARAMS */
+    double tolerance;          /* CG relative tolerance, reference 1e-6 */
+    int64_t max_iterations;    /* < 0: 2 * N_V, Eigen's default */
+    double domain_lo[3];       /* axis-aligned box that real and boundary
+                                  particles stay in; particles outside are
+                                  clamped to the border cells (still correct,
+                                  only slower) */
+    double domain_hi[3];
+} vps_params;
+
+/* Per-call statistics.  Times are device times (HIP events on the library
+ * stream) in milliseconds. */
+typedef struct vps_stats {
+    int64_t n_real;
+    int64_t n_boundary;
+    int64_t n_virtual;
+    int64_t n_active_bricks;
+    int64_t cg_iterations;     /* Eigen convention: the converging iteration is
+                                  not counted (ConjugateGradient.h) */
+    double cg_rel_residual;    /* sqrt(r.r / b.b) of the recursive residual */
+    double cg_true_rel_residual; /* ||b - M x|| / ||b||, FP64 on the device,
+                                  computed only when requested (< 0 if not) */
+    double rhs_norm;           /* ||b|| = ||D|| */
+    double alpha0;
+    double kappa;
+    double rest_density;
+    double ms_sort;            /* real counting sort + density */
+    double ms_generate;        /* virtual marking + fused virtual sums */
+    double ms_diag;            /* Laplacian diagonal / alpha0 */
+    double ms_cg;              /* warm start + CG */
+    double ms_gradient;        /* pressure gradient + velocity update */
+    double ms_total;
+    double ms_spmv_avg;        /* average duration of one CG SpMV launch */
+    double ms_update_avg;      /* average duration of one CG update launch */
+    int64_t spmv_launches;
+    int64_t reserved[4];
+} vps_stats;
+
+typedef struct vps_context vps_context;
+
+/* ---- lifetime --------------------------------------------------------- */
+
+/* Size checks for foreign-language bindings. */
+int32_t vps_abi_version(void);
+int64_t vps_sizeof_params(void);
+int64_t vps_sizeof_stats(void);
+
+/* Creates a context on the current HIP device.  Replaces the constructors
+ * DualParticleSphLiquid(grid, particleRadius) and
+ * VirtualParticle(grid, grid.spacing()) (DualParticleSphLiquid.h:27-29). */
+int vps_create(const vps_params* params, vps_context** out);
+void vps_destroy(vps_context* ctx);
+
+/* Message of the last failed call on ctx (or of the last failed vps_create
+ * when ctx is NULL).  Never NULL. */
+const char* vps_last_error(const vps_context* ctx);
+
+/* ---- state ------------------------------------------------------------ */
+
+/* Boundary (solid wall) particles: positions, inward unit normals and
+ * velocities, each n*dim doubles.  Replaces BoundaryParticles::addSurface
+ * and the boundary velocity field (DualParticleSphLiquid.cpp:79-84): the
+ * caller samples the walls.  May be called again to move the walls (their
+ * densities are recomputed at the next vps_initialize or
+ * vps_refresh_boundary). */
+int vps_set_boundary(vps_context* ctx, int64_t n, const double* x,
+                     const double* normals, const double* v);
+
+/* Recomputes boundary neighbour lists, densities and volumes without a full
+ * re-initialisation (moving boundaries). */
+int vps_refresh_boundary(vps_context* ctx);
+
+/* Real (fluid) particle positions and velocities, n*dim doubles each. */
+int vps_set_particles(vps_context* ctx, int64_t n, const double* x,
+                      const double* v);
+
+/* Copies the real particle state back, in the caller's original order. */
+int vps_get_particles(vps_context* ctx, double* x, double* v);
+
+/* ---- the path --------------------------------------------------------- */
+
+/* DualParticleSphLiquid::initialize minus the boundary sampling
+ * (DualParticleSphLiquid.cpp:78-110): real rho and frozen V, rho0, boundary
+ * rho and V, virtual generation, alpha0 = max diagonal, kappa. */
+int vps_initialize(vps_context* ctx, vps_stats* stats);
+
+/* The two lines of SmthParticleHydrodLiquid::moveParticles that precede the
+ * path (SmthParticleHydrodLiquid.cpp:80-81): neighbour structure rebuild and
+ * real densities. */
+int vps_refresh_real(vps_context* ctx);
+
+/* VirtualParticle::generateParticles (VirtualParticle.cpp:45-62). */
+int vps_generate_particles(vps_context* ctx);
+
+/* VirtualParticle::project (VirtualParticle.cpp:76-86): right-hand side,
+ * Laplacian, CG, pressure gradient and velocity update, in place on the
+ * resident velocities.  stats may be NULL. */
+int vps_project(vps_context* ctx, vps_stats* stats);
+
+/* One timed projection step as the reference orders it:
+ * vps_refresh_real + vps_generate_particles + vps_project. */
+int vps_projection_step(vps_context* ctx, vps_stats* stats);
+
+/* x += v dt, then v.y -= g dt (SmthParticleHydrodLiquid.cpp:73, 84-88),
+ * on the resident state.  Not part of the projection. */
+int vps_advect(vps_context* ctx, double dt, double gravity);
+
+/* Requests that the next vps_project also computes the FP64 true residual
+ * ||b - M x||/||b|| into stats.cg_true_rel_residual. */
+int vps_set_check_true_residual(vps_context* ctx, int32_t enable);
+
+/* Records HIP events around every CG SpMV and update launch of the next
+ * solves, reported as stats.ms_spmv_avg / ms_update_avg. */
+int vps_set_kernel_timing(vps_context* ctx, int32_t enable);
+
+/* Blocks until all work on the context's stream is done. */
+int vps_synchronize(vps_context* ctx);
+
+/* ---- read-back (tests, output) --------------------------------------- */
+
+int vps_get_virtual_count(vps_context* ctx, int64_t* n_virtual);
+
+/* Virtual particle fields in the reference's order (lattice-linear, x
+ * fastest; VirtualParticle.cpp:55-57).  Any pointer may be NULL.
+ * node: linear node index; rho: rho_I; vol: V_I; div: D_I (CG right-hand
+ * side); diag: Laplacian diagonal; pressure: CG solution p_I. */
+int vps_get_virtual(vps_context* ctx, int64_t* node, double* rho, double* vol,
+                    double* div, double* diag, double* pressure);
+
+/* Real particle densities (current) and frozen volumes, caller order. */
+int vps_get_real_fields(vps_context* ctx, double* rho, double* vol);
+
+/* Boundary densities and volumes in the order given to vps_set_boundary. */
+int vps_get_boundary_fields(vps_context* ctx, double* rho, double* vol);
+
+/* Scalars fixed at initialize. */
+int vps_get_scalars(vps_context* ctx, double* rest_density, double* kappa,
+                    double* alpha0);
+
+/* Neighbour query against the resident real (which = 0), boundary (1) or
+ * virtual (2) set, with the reference's membership rule
+ * (x_j - q).squaredNorm() < radius^2 in FP64 (ParticlesNearbySearcher.h:30-34,
+ * .cpp:32), radius = h for sets 0/1 and H for set 2.  Writes a CSR:
+ * offsets[nq+1] and, per query, the caller-order indices (reference virtual
+ * order for set 2) sorted ascending.  Returns VPS_ERR_INVALID_ARG with
+ * *needed set when capacity is too small. */
+int vps_query_neighbors(vps_context* ctx, int32_t which, int64_t nq,
+                        const double* q, int64_t* offsets, int64_t* indices,
+                        int64_t capacity, int64_t* needed);
+
+/* Applies the CG operator M of the current system (VPS_OP_TRANSPOSE: A^T,
+ * VPS_OP_ROW: A, with the assembled diagonal) to p, both in the reference
+ * virtual order, with the same FP32 stencil CG uses.  Replaces the
+ * SparseMatrix * vector product inside Eigen's CG (IterativeSolver.h:35). */
+int vps_debug_apply_operator(vps_context* ctx, const double* p, double* out);
+
+/* ---- learned warm start (no reference counterpart) ------------------- */
+
+/* Online training of the warm-start MLP on (features, p) pairs produced by
+ * the plain solver: after each vps_project with training enabled, one Adam
+ * step (batch = min(batch, N_V)) runs on the GPU. */
+int vps_mlp_set_training(vps_context* ctx, int32_t enable, int32_t batch,
+                         double learning_rate, uint64_t seed);
+
+/* Parameter I/O for checks and checkpoints: 4,801 FP32 values in the order
+ * W1[64x8], b1[64], W2[64x64], b2[64], W3[1x64], b3[1] (row-major). */
+int64_t vps_mlp_parameter_count(void);
+int vps_mlp_get_parameters(vps_context* ctx, float* out);
+int vps_mlp_set_parameters(vps_context* ctx, const float* in);
+
+/* One forward + backward pass over given features (n x 8, FP32) and targets
+ * (n, FP32): writes the loss (mean squared error) and the gradient of every
+ * parameter (same order as above).  Used to check the hand-written MFMA
+ * kernels against the FP64 oracle network. */
+int vps_mlp_loss_and_grad(vps_context* ctx, int64_t n, const float* features,
+                          const float* targets, double* loss, float* grad);
+
+/* Inference over given features: out[n]. */
+int vps_mlp_forward(vps_context* ctx, int64_t n, const float* features,
+                    float* out);
+
+/* Per-virtual-particle features of the current step in reference order
+ * (n_virtual x 8 FP32). */
+int vps_mlp_get_features(vps_context* ctx, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VPS_H */

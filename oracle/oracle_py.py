@@ -1,0 +1,290 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(ROOT, "sand-water-coupling_amd"))
+from vps import _abi  # noqa: E402
+
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+SEARCH_HASH = 0
+SEARCH_BRUTE = 1
+
+
+class OrStats(C.Structure):
+    _fields_ = [
+        ("n_virtual", C.c_int64),
+        ("cg_iterations", C.c_int64),
+        ("cg_error", C.c_double),
+        ("rhs_norm", C.c_double),
+        ("cg_status", C.c_int32),
+        ("pad", C.c_int32),
+        ("s_refresh", C.c_double),
+        ("s_generate", C.c_double),
+        ("s_build", C.c_double),
+        ("s_solve", C.c_double),
+        ("s_gradient", C.c_double),
+        ("s_total", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad"}
+
+
+_lib = None
+
+
+def build():
+    """Compiles liboracle.so from its C source (make)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    D = C.POINTER(C.c_double)
+    I64 = C.POINTER(C.c_int64)
+    I32 = C.POINTER(C.c_int32)
+
+    def pr(name, res, args):
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+    pr("or_create", P, [C.POINTER(_abi.Params), C.c_int32, C.c_int32])
+    pr("or_destroy", None, [P])
+    pr("or_set_boundary", C.c_int, [P, C.c_int64, D, D, D])
+    pr("or_set_particles", C.c_int, [P, C.c_int64, D, D])
+    pr("or_get_particles", C.c_int, [P, D, D])
+    pr("or_initialize", C.c_int, [P])
+    pr("or_refresh_real", C.c_int, [P])
+    pr("or_generate", C.c_int, [P])
+    pr("or_project", C.c_int, [P, C.POINTER(OrStats)])
+    pr("or_projection_step", C.c_int, [P, C.POINTER(OrStats)])
+    pr("or_advect", C.c_int, [P, C.c_double, C.c_double])
+    pr("or_virtual_count", C.c_int64, [P])
+    pr("or_get_virtual", C.c_int, [P, I64, D, D, D, D, D, D])
+    pr("or_set_pressure", C.c_int, [P, D])
+    pr("or_pressure_gradient", C.c_int, [P])
+    pr("or_get_real_fields", C.c_int, [P, D, D])
+    pr("or_get_boundary_fields", C.c_int, [P, D, D])
+    pr("or_get_scalars", C.c_int, [P, D, D, D])
+    pr("or_get_node_weights", C.c_int, [P, D])
+    pr("or_query", C.c_int, [P, C.c_int32, C.c_int64, D, I64, I64, C.c_int64, I64])
+    pr("or_query_ordered", C.c_int,
+       [P, C.c_int32, C.c_int64, D, I64, I64, C.c_int64, I64])
+    pr("or_matrix_nnz", C.c_int64, [P])
+    pr("or_get_matrix", C.c_int, [P, I64, I32, D])
+    pr("or_apply_operator", C.c_int, [P, D, D])
+    pr("or_cg_csr", C.c_int, [C.c_int64, I64, I32, D, D, D, C.c_double,
+                              C.c_int64, I64, D])
+    pr("or_kernel", C.c_double, [C.c_int32, C.c_double, C.c_double])
+    pr("or_kernel_d1", C.c_double, [C.c_int32, C.c_double, C.c_double])
+    pr("or_packed_kernel_sum", C.c_double, [C.c_int32, C.c_double])
+    pr("or_hash_key", C.c_int64, [C.c_int32, I32, C.c_int64])
+    _lib = L
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def _i32(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise OracleError("%s failed: %s" % (what, _abi.STATUS_NAMES.get(rc, rc)))
+
+
+class Oracle:
+    """FP64 restatement of DualParticleSphLiquid + VirtualParticle."""
+
+    def __init__(self, scene, search=SEARCH_HASH, threads=0, **overrides):
+        self.L = lib()
+        self.scene = scene
+        self.dim = scene["dim"]
+        self.params = _abi.make_params(scene, **overrides)
+        self.h = self.L.or_create(C.byref(self.params), search, threads)
+        if not self.h:
+            raise OracleError("or_create failed")
+        bx = np.ascontiguousarray(scene["bx"], dtype=np.float64)
+        bn = np.ascontiguousarray(scene["bn"], dtype=np.float64)
+        bv = np.ascontiguousarray(scene["bv"], dtype=np.float64)
+        _check(self.L.or_set_boundary(self.h, len(bx), _d(bx), _d(bn), _d(bv)),
+               "or_set_boundary")
+        x = np.ascontiguousarray(scene["x"], dtype=np.float64)
+        v = np.ascontiguousarray(scene["v"], dtype=np.float64)
+        _check(self.L.or_set_particles(self.h, len(x), _d(x), _d(v)),
+               "or_set_particles")
+        self.n = len(x)
+        self.nb = len(bx)
+
+    def close(self):
+        if self.h:
+            self.L.or_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def initialize(self):
+        _check(self.L.or_initialize(self.h), "or_initialize")
+
+    def refresh_real(self):
+        _check(self.L.or_refresh_real(self.h), "or_refresh_real")
+
+    def generate(self):
+        _check(self.L.or_generate(self.h), "or_generate")
+
+    def project(self, allow_fail=False):
+        st = OrStats()
+        rc = self.L.or_project(self.h, C.byref(st))
+        if not allow_fail:
+            _check(rc, "or_project")
+        return st.as_dict()
+
+    def projection_step(self):
+        st = OrStats()
+        _check(self.L.or_projection_step(self.h, C.byref(st)), "or_projection_step")
+        return st.as_dict()
+
+    def advect(self, dt, gravity):
+        _check(self.L.or_advect(self.h, dt, gravity), "or_advect")
+
+    def particles(self):
+        x = np.zeros((self.n, self.dim))
+        v = np.zeros((self.n, self.dim))
+        self.L.or_get_particles(self.h, _d(x), _d(v))
+        return x, v
+
+    def virtual(self):
+        n = self.L.or_virtual_count(self.h)
+        out = {k: np.zeros(n) for k in ("rho", "vol", "div", "diag", "p")}
+        out["node"] = np.zeros(n, dtype=np.int64)
+        out["vbar"] = np.zeros((n, self.dim))
+        self.L.or_get_virtual(self.h, _i64(out["node"]), _d(out["rho"]),
+                              _d(out["vol"]), _d(out["vbar"]), _d(out["div"]),
+                              _d(out["diag"]), _d(out["p"]))
+        return out
+
+    def set_pressure(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        self.L.or_set_pressure(self.h, _d(p))
+
+    def pressure_gradient(self):
+        _check(self.L.or_pressure_gradient(self.h), "or_pressure_gradient")
+
+    def real_fields(self):
+        rho = np.zeros(self.n)
+        vol = np.zeros(self.n)
+        self.L.or_get_real_fields(self.h, _d(rho), _d(vol))
+        return rho, vol
+
+    def boundary_fields(self):
+        rho = np.zeros(self.nb)
+        vol = np.zeros(self.nb)
+        self.L.or_get_boundary_fields(self.h, _d(rho), _d(vol))
+        return rho, vol
+
+    def scalars(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        self.L.or_get_scalars(self.h, C.byref(a), C.byref(b), C.byref(c))
+        return {"rest_density": a.value, "kappa": b.value, "alpha0": c.value}
+
+    def node_weights(self):
+        dims = self.scene["lattice_dims"]
+        nn = int(dims[0] * dims[1] * (dims[2] if self.dim == 3 else 1))
+        w = np.zeros(nn)
+        _check(self.L.or_get_node_weights(self.h, _d(w)), "or_get_node_weights")
+        return w
+
+    def query(self, which, q, ordered=False):
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, self.dim)
+        nq = len(q)
+        offs = np.zeros(nq + 1, dtype=np.int64)
+        need = C.c_int64(0)
+        fn = self.L.or_query_ordered if ordered else self.L.or_query
+        fn(self.h, which, nq, _d(q), _i64(offs), None, 0, C.byref(need))
+        idx = np.zeros(max(need.value, 1), dtype=np.int64)
+        _check(fn(self.h, which, nq, _d(q), _i64(offs), _i64(idx), len(idx),
+                  C.byref(need)), "or_query")
+        return offs, idx[: need.value]
+
+    def matrix(self):
+        n = self.L.or_virtual_count(self.h)
+        rp = np.zeros(n + 1, dtype=np.int64)
+        nnz = self.L.or_matrix_nnz(self.h)
+        col = np.zeros(max(nnz, 1), dtype=np.int32)
+        val = np.zeros(max(nnz, 1))
+        _check(self.L.or_get_matrix(self.h, _i64(rp), _i32(col), _d(val)),
+               "or_get_matrix")
+        return rp, col[:nnz], val[:nnz]
+
+    def apply_operator(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        out = np.zeros_like(p)
+        _check(self.L.or_apply_operator(self.h, _d(p), _d(out)), "or_apply_operator")
+        return out
+
+
+def cg_csr(rowptr, col, val, b, x0=None, tol=1e-6, max_iter=-1):
+    """Eigen-faithful CG on a CSR matrix; returns (x, iters, err, status)."""
+    L = lib()
+    n = len(b)
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    it = C.c_int64(0)
+    err = C.c_double(0)
+    rc = L.or_cg_csr(n, _i64(rowptr), _i32(col), _d(val), _d(b), _d(x), tol,
+                     max_iter, C.byref(it), C.byref(err))
+    return x, it.value, err.value, rc
+
+
+def kernel(dim, radius, dist):
+    return lib().or_kernel(dim, radius, dist)
+
+
+def kernel_d1(dim, radius, dist):
+    return lib().or_kernel_d1(dim, radius, dist)
+
+
+def packed_kernel_sum(dim, radius):
+    return lib().or_packed_kernel_sum(dim, radius)
+
+
+def hash_key(dim, coord, nbuckets):
+    c = np.zeros(3, dtype=np.int32)
+    c[: len(coord)] = coord
+    return lib().or_hash_key(dim, _i32(c), nbuckets)

@@ -1,0 +1,1125 @@
+/*
+ * vps_oracle.c -- FP64 CPU restatement of the reference's virtual-particle
+ * ("dual-particle") SPH projection.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
+ * the cpu_baseline leg of bench.py as the checker and the CPU baseline; the
+ * product library never calls it.
+ *
+ * Pinning (what checks each function; the reference ships no tests, golden
+ * vectors or fixtures for this path):
+ *   kernels / packed sum     closed forms, quadrature, finite differences
+ *   hash searcher            brute force (same sets), hand-computed keys
+ *   densities / sums         brute-force O(N^2) sums, lattice invariants
+ *   virtual generation       brute force over all nodes, threshold cases
+ *   Laplacian assembly       interior row sums, A vs A^T on a tiny system
+ *   CG                       dense solves, 1D Poisson, stopping rule
+ *   pressure gradient        constant pressure, hand-built boundary cases
+ * The last-bit FP64 values of the reference depend on Eigen's SIMD reduction
+ * order inside CG (dot/squaredNorm over long vectors); that part is "parity
+ * unpinned" -- only compile of the reference against Eigen would fix it and
+ * Eigen is not available here.
+ *
+ * Arithmetic follows the reference expression by expression, in FP64, with
+ * FP contraction off (build flag -ffp-contract=off), so that a 3-vector
+ * squared norm is ((dx*dx + dy*dy) + dz*dz) as Eigen's vectorised redux
+ * evaluates it for a Vector3d with SSE2 packets.
+ */
+#include "vps_oracle.h"
+
+#include <math.h>
+#include <float.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* std::numbers::inv_pi, sqrt2, sqrt3 as doubles */
+#define OR_INV_PI 0.31830988618379067154
+#define OR_SQRT2 1.41421356237309504880
+#define OR_SQRT3 1.73205080756887729353
+
+static double now_s(void) {
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+/* ------------------------------------------------------------------------
+ * Cubic spline kernel.  SmoothedParticles constructor
+ * (Cores/Structures/SmoothedParticles.cpp:13-23) and kernel /
+ * firstDerivativeKernel (SmoothedParticles.h:60-72).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int dim;
+    double radius, kr, kr2, inv_kr, inv_kr2, c0, c1;
+} or_kern;
+
+static void kern_init(or_kern* k, int dim, double radius) {
+    const double kernel_rate = 4; /* SmoothedParticles.h:45 default */
+    k->dim = dim;
+    k->radius = radius;
+    k->kr = radius * kernel_rate;
+    k->kr2 = k->kr * k->kr;
+    k->inv_kr = 1 / k->kr;
+    k->inv_kr2 = 1 / k->kr2;
+    k->c0 = OR_INV_PI * k->inv_kr2 * (dim == 2 ? (double)40 / 7 : 8 * k->inv_kr);
+    k->c1 = 6 * k->inv_kr * k->c0;
+}
+
+static inline double kern_W(const or_kern* k, double distance) {
+    const double x = distance * k->inv_kr;
+    if (x < .5) return k->c0 * (6 * x * x * (x - 1) + 1);
+    else if (x < 1) return k->c0 * 2 * (1 - x) * (1 - x) * (1 - x);
+    else return 0;
+}
+
+static inline double kern_dW(const or_kern* k, double distance) {
+    const double x = distance * k->inv_kr;
+    if (x < .5) return k->c1 * x * (3 * x - 2);
+    else if (x < 1) return k->c1 * (1 - x) * (x - 1);
+    else return 0;
+}
+
+/* squaredNorm of a Dim-vector as Eigen evaluates it (see file header) */
+static inline double sqnorm(int dim, const double* d) {
+    if (dim == 2) return d[0] * d[0] + d[1] * d[1];
+    return (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+}
+static inline double dot3(int dim, const double* a, const double* b) {
+    if (dim == 2) return a[0] * b[0] + a[1] * b[1];
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+/* kernel(deltaPos) = kernel(deltaPos.norm()) (SmoothedParticles.h:95) */
+static inline double kern_Wv(const or_kern* k, const double* d) {
+    return kern_W(k, sqrt(sqnorm(k->dim, d)));
+}
+
+/* gradientKernel(d) = -firstDerivativeKernel(|d|) * d.normalized()
+ * (SmoothedParticles.h:96-98); Eigen normalized() divides by sqrt(z) when
+ * z > 0 and returns d unchanged otherwise. */
+static inline void kern_grad(const or_kern* k, const double* d, double* g) {
+    const double z = sqnorm(k->dim, d);
+    const double s = -kern_dW(k, sqrt(z));
+    g[0] = g[1] = g[2] = 0;
+    for (int a = 0; a < k->dim; a++) {
+        const double nd = z > 0 ? d[a] / sqrt(z) : d[a];
+        g[a] = s * nd;
+    }
+}
+
+double or_kernel(int32_t dim, double radius, double dist) {
+    or_kern k;
+    kern_init(&k, dim, radius);
+    return kern_W(&k, dist);
+}
+double or_kernel_d1(int32_t dim, double radius, double dist) {
+    or_kern k;
+    kern_init(&k, dim, radius);
+    return kern_dW(&k, dist);
+}
+
+/* getPackedKernelSum (SmoothedParticles.cpp:74-80) */
+static double packed_sum(const or_kern* k) {
+    if (k->dim == 2)
+        return kern_W(k, 0) + kern_W(k, k->radius * 2) * 6 +
+               kern_W(k, k->radius * 2 * OR_SQRT3) * 6;
+    return kern_W(k, 0) + kern_W(k, k->radius * 2) * 12 +
+           kern_W(k, k->radius * 2 * OR_SQRT2) * 6 +
+           kern_W(k, k->radius * 2 * OR_SQRT3) * 24;
+}
+double or_packed_kernel_sum(int32_t dim, double radius) {
+    or_kern k;
+    kern_init(&k, dim, radius);
+    return packed_sum(&k);
+}
+
+/* ------------------------------------------------------------------------
+ * Neighbour search.  HashGridSearcher (ParticlesNearbySearcher.h:37-74,
+ * .cpp:10-35) on a Grid with spacing = kernel radius and origin 0
+ * (ParticlesNearbySearcher.h:57), getQuadraticLower (Grid.h:70) and
+ * quadraticNearbyDataPoints (Grid.cpp:26-45).  Buckets are stored as CSR
+ * rather than vector<vector<int>>: same contents, same insertion order.
+ * The brute-force mode is ParticlesNearbySearcher::forEach (.h:30-34).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int dim;
+    double inv_spacing;
+    int64_t nbuckets;
+    int64_t* start;   /* nbuckets + 1 */
+    int32_t* items;
+} or_hash;
+
+int64_t or_hash_key(int32_t dim, const int32_t* c, int64_t nbuckets) {
+    /* (_kPrime0 * ullong(c0) | _kPrime1 * ullong(c1) [| _kPrime2 * ullong(c2)])
+     * % buckets, ParticlesNearbySearcher.h:67-73 */
+    uint64_t k = (uint64_t)73856093 * (uint64_t)(int64_t)c[0] |
+                 (uint64_t)19349663 * (uint64_t)(int64_t)c[1];
+    if (dim == 3) k |= (uint64_t)83492791 * (uint64_t)(int64_t)c[2];
+    return (int64_t)(int)(k % (uint64_t)nbuckets);
+}
+
+static inline void quadratic_lower(const or_hash* h, const double* pos,
+                                   int32_t* lo) {
+    for (int a = 0; a < h->dim; a++)
+        lo[a] = (int32_t)floor((pos[a] - 0.0) * h->inv_spacing - 1.0 / 2);
+}
+
+static void hash_free(or_hash* h) {
+    free(h->start);
+    free(h->items);
+    h->start = NULL;
+    h->items = NULL;
+    h->nbuckets = 0;
+}
+
+/* HashGridSearcher::reset (ParticlesNearbySearcher.cpp:10-18) */
+static void hash_reset(or_hash* h, int dim, double kr, int64_t n,
+                       const double* x) {
+    hash_free(h);
+    h->dim = dim;
+    h->inv_spacing = 1 / kr;
+    h->nbuckets = n * 2;
+    if (n == 0) return;
+    h->start = (int64_t*)calloc((size_t)h->nbuckets + 1, sizeof(int64_t));
+    h->items = (int32_t*)malloc((size_t)n * sizeof(int32_t));
+    int64_t* key = (int64_t*)malloc((size_t)n * sizeof(int64_t));
+    for (int64_t i = 0; i < n; i++) {
+        int32_t c[3];
+        quadratic_lower(h, x + 3 * i, c);
+        for (int a = 0; a < dim; a++) c[a] += 1;
+        key[i] = or_hash_key(dim, c, h->nbuckets);
+        h->start[key[i] + 1]++;
+    }
+    for (int64_t b = 0; b < h->nbuckets; b++) h->start[b + 1] += h->start[b];
+    int64_t* fill = (int64_t*)malloc((size_t)h->nbuckets * sizeof(int64_t));
+    memcpy(fill, h->start, (size_t)h->nbuckets * sizeof(int64_t));
+    for (int64_t i = 0; i < n; i++) h->items[fill[key[i]]++] = (int32_t)i;
+    free(fill);
+    free(key);
+}
+
+static int cmp_i64(const void* a, const void* b) {
+    const int64_t x = *(const int64_t*)a, y = *(const int64_t*)b;
+    return x < y ? -1 : x > y;
+}
+
+typedef void (*or_cb)(void* ctx, int32_t j, const double* xj);
+
+typedef struct {
+    or_kern k;
+    int64_t n;
+    double* x;      /* n*3, z = 0 in 2D */
+    double* rho;
+    double* vol;
+    double mass;
+    int search_mode;
+    or_hash hash;
+} or_set;
+
+/* SmoothedParticles::forEachNearby -> HashGridSearcher::forEach
+ * (SmoothedParticles.h:111-113; ParticlesNearbySearcher.cpp:20-35) */
+static void set_for_each(const or_set* s, const double* q, or_cb cb,
+                         void* ctx) {
+    const int dim = s->k.dim;
+    if (s->n == 0) return;
+    if (s->search_mode == OR_SEARCH_BRUTE) {
+        for (int64_t j = 0; j < s->n; j++) {
+            double d[3] = {s->x[3 * j] - q[0], s->x[3 * j + 1] - q[1],
+                           s->x[3 * j + 2] - q[2]};
+            if (sqnorm(dim, d) < s->k.kr2) cb(ctx, (int32_t)j, s->x + 3 * j);
+        }
+        return;
+    }
+    int64_t keys[27];
+    int cnt = 0;
+    int32_t lo[3];
+    quadratic_lower(&s->hash, q, lo);
+    if (dim == 2) {
+        for (int j = 0; j < 3; j++)
+            for (int i = 0; i < 3; i++) {
+                int32_t c[3] = {lo[0] + i, lo[1] + j, 0};
+                keys[cnt++] = or_hash_key(2, c, s->hash.nbuckets);
+            }
+    } else {
+        for (int kk = 0; kk < 3; kk++)
+            for (int j = 0; j < 3; j++)
+                for (int i = 0; i < 3; i++) {
+                    int32_t c[3] = {lo[0] + i, lo[1] + j, lo[2] + kk};
+                    keys[cnt++] = or_hash_key(3, c, s->hash.nbuckets);
+                }
+    }
+    qsort(keys, (size_t)cnt, sizeof(int64_t), cmp_i64);
+    for (int i = 0; i < cnt; i++) {
+        if (i && keys[i] == keys[i - 1]) continue;
+        for (int64_t t = s->hash.start[keys[i]]; t < s->hash.start[keys[i] + 1];
+             t++) {
+            const int32_t j = s->hash.items[t];
+            double d[3] = {s->x[3 * j] - q[0], s->x[3 * j + 1] - q[1],
+                           s->x[3 * j + 2] - q[2]};
+            if (sqnorm(dim, d) < s->k.kr2) cb(ctx, j, s->x + 3 * j);
+        }
+    }
+}
+
+static void set_reset(or_set* s) {
+    if (s->search_mode == OR_SEARCH_HASH)
+        hash_reset(&s->hash, s->k.dim, s->k.kr, s->n, s->x);
+}
+
+/* ---- sums over neighbours (SmoothedParticles.cpp:25-72,
+ *      ParticlesBasedVectorField.cpp:5-21) ------------------------------- */
+typedef struct {
+    const or_set* s;
+    const double* q;
+    const or_kern* k; /* kernel used in the sum (may differ from s->k) */
+    double acc;
+    double vec[3];
+    const double* field; /* vector field on s (3 per particle) */
+    const double* other;
+} sum_ctx;
+
+static void cb_weight(void* c, int32_t j, const double* xj) {
+    sum_ctx* t = (sum_ctx*)c;
+    double d[3] = {t->q[0] - xj[0], t->q[1] - xj[1], t->q[2] - xj[2]};
+    (void)j;
+    t->acc += kern_Wv(t->k, d);
+}
+static void cb_bias_weight(void* c, int32_t j, const double* xj) {
+    sum_ctx* t = (sum_ctx*)c;
+    double d[3] = {t->q[0] - xj[0], t->q[1] - xj[1], t->q[2] - xj[2]};
+    t->acc += t->s->vol[j] * kern_Wv(t->k, d);
+}
+static void cb_bias_grad(void* c, int32_t j, const double* xj) {
+    sum_ctx* t = (sum_ctx*)c;
+    double d[3] = {t->q[0] - xj[0], t->q[1] - xj[1], t->q[2] - xj[2]};
+    double g[3];
+    kern_grad(t->k, d, g);
+    for (int a = 0; a < t->k->dim; a++) t->vec[a] += t->s->vol[j] * g[a];
+}
+/* ParticlesBasedVectorField::operator() (ParticlesBasedVectorField.cpp:5-12) */
+static void cb_field_value(void* c, int32_t j, const double* xj) {
+    sum_ctx* t = (sum_ctx*)c;
+    double d[3] = {t->q[0] - xj[0], t->q[1] - xj[1], t->q[2] - xj[2]};
+    const double w = kern_Wv(t->k, d);
+    for (int a = 0; a < t->k->dim; a++)
+        t->vec[a] += t->field[3 * j + a] * w / t->s->rho[j];
+}
+/* ParticlesBasedVectorField::divergence (ParticlesBasedVectorField.cpp:14-21) */
+static void cb_field_div(void* c, int32_t j, const double* xj) {
+    sum_ctx* t = (sum_ctx*)c;
+    double d[3] = {t->q[0] - xj[0], t->q[1] - xj[1], t->q[2] - xj[2]};
+    double g[3];
+    kern_grad(t->k, d, g);
+    t->acc += dot3(t->k->dim, t->field + 3 * j, g) * t->s->vol[j];
+}
+
+static double neighbor_weight(const or_set* s, const double* q) {
+    sum_ctx c = {s, q, &s->k, 0, {0, 0, 0}, NULL, NULL};
+    set_for_each(s, q, cb_weight, &c);
+    return c.acc;
+}
+static double bias_weight(const or_set* s, const double* q) {
+    sum_ctx c = {s, q, &s->k, 0, {0, 0, 0}, NULL, NULL};
+    set_for_each(s, q, cb_bias_weight, &c);
+    return c.acc;
+}
+static void bias_gradient(const or_set* s, const double* q, double* out) {
+    sum_ctx c = {s, q, &s->k, 0, {0, 0, 0}, NULL, NULL};
+    set_for_each(s, q, cb_bias_grad, &c);
+    out[0] = c.vec[0];
+    out[1] = c.vec[1];
+    out[2] = c.vec[2];
+}
+static void field_value(const or_set* s, const double* field, const double* q,
+                        double* out) {
+    sum_ctx c = {s, q, &s->k, 0, {0, 0, 0}, field, NULL};
+    set_for_each(s, q, cb_field_value, &c);
+    for (int a = 0; a < 3; a++) out[a] = c.vec[a] * s->mass;
+}
+static double field_divergence(const or_set* s, const double* field,
+                               const double* q) {
+    sum_ctx c = {s, q, &s->k, 0, {0, 0, 0}, field, NULL};
+    set_for_each(s, q, cb_field_div, &c);
+    return c.acc;
+}
+
+/* SmoothedParticles::computeDensities / computeVolumes
+ * (SmoothedParticles.cpp:25-36) */
+static void set_compute_densities(or_set* s) {
+    int64_t i;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < s->n; i++)
+        s->rho[i] = s->mass * neighbor_weight(s, s->x + 3 * i);
+}
+static void set_compute_volumes(or_set* s) {
+    int64_t i;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < s->n; i++) s->vol[i] = s->mass / s->rho[i];
+}
+
+static int set_resize(or_set* s, int64_t n) {
+    s->n = n;
+    s->x = (double*)realloc(s->x, (size_t)(n ? n : 1) * 3 * sizeof(double));
+    s->rho = (double*)realloc(s->rho, (size_t)(n ? n : 1) * sizeof(double));
+    s->vol = (double*)realloc(s->vol, (size_t)(n ? n : 1) * sizeof(double));
+    return s->x && s->rho && s->vol ? 0 : VPS_ERR_OOM;
+}
+static void set_free(or_set* s) {
+    free(s->x);
+    free(s->rho);
+    free(s->vol);
+    hash_free(&s->hash);
+}
+
+/* ------------------------------------------------------------------------
+ * Simulation state: DualParticleSphLiquid + VirtualParticle.
+ * ---------------------------------------------------------------------- */
+struct or_sim {
+    vps_params p;
+    int dim;
+    or_set real, bnd, virt;
+    double* v;        /* real velocities n*3 */
+    double* bv;       /* boundary velocities */
+    double* bn;       /* boundary normals */
+    double rho0, kappa, alpha0;
+    int initialized;
+    /* virtual state */
+    int64_t nnodes;
+    double* w;        /* node weights */
+    int64_t* node;    /* virtual -> linear node index */
+    double* vbar;     /* n_virtual*3 */
+    double* div;
+    double* pres;
+    int64_t npres;    /* current length of the pressure array */
+    double* prev_node_p; /* node-keyed previous pressure (nnodes) or NULL */
+    unsigned char* prev_node_valid;
+    /* assembled operator M (CSR) */
+    int64_t* rowptr;
+    int32_t* col;
+    double* val;
+    double* diag;
+};
+
+static void lattice_pos(const or_sim* s, int64_t lin, double* out) {
+    /* Grid::dataPosition: origin + coord * spacing (Grid.h:51); linear index
+     * x fastest (Grid.h:54-58) */
+    const int32_t nx = s->p.lattice_dims[0], ny = s->p.lattice_dims[1];
+    const int64_t cx = lin % nx, cy = (lin / nx) % ny, cz = lin / nx / ny;
+    const double sp = s->p.lattice_spacing;
+    out[0] = s->p.lattice_origin[0] + (double)cx * sp;
+    out[1] = s->p.lattice_origin[1] + (double)cy * sp;
+    out[2] = s->dim == 3 ? s->p.lattice_origin[2] + (double)cz * sp : 0.0;
+}
+
+or_sim* or_create(const vps_params* p, int32_t search_mode, int32_t threads) {
+    if (!p || (p->dim != 2 && p->dim != 3)) return NULL;
+    or_sim* s = (or_sim*)calloc(1, sizeof(or_sim));
+    s->p = *p;
+    s->dim = p->dim;
+    if (s->dim == 2) s->p.lattice_dims[2] = 1;
+    kern_init(&s->real.k, s->dim, p->particle_radius);
+    kern_init(&s->bnd.k, s->dim, p->particle_radius);
+    kern_init(&s->virt.k, s->dim, p->lattice_spacing);
+    s->real.mass = p->mass;
+    s->bnd.mass = p->mass; /* DualParticleSphLiquid.cpp:84 */
+    s->virt.mass = 1;      /* SmoothedParticles default; unused */
+    s->real.search_mode = s->bnd.search_mode = s->virt.search_mode =
+        search_mode;
+    s->nnodes = (int64_t)s->p.lattice_dims[0] * s->p.lattice_dims[1] *
+                s->p.lattice_dims[2];
+    s->alpha0 = -1e6; /* VirtualParticle ctor (VirtualParticle.cpp:16) */
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    return s;
+}
+
+static void free_matrix(or_sim* s) {
+    free(s->rowptr);
+    free(s->col);
+    free(s->val);
+    free(s->diag);
+    s->rowptr = NULL;
+    s->col = NULL;
+    s->val = NULL;
+    s->diag = NULL;
+}
+
+void or_destroy(or_sim* s) {
+    if (!s) return;
+    set_free(&s->real);
+    set_free(&s->bnd);
+    set_free(&s->virt);
+    free(s->v);
+    free(s->bv);
+    free(s->bn);
+    free(s->w);
+    free(s->node);
+    free(s->vbar);
+    free(s->div);
+    free(s->pres);
+    free(s->prev_node_p);
+    free(s->prev_node_valid);
+    free_matrix(s);
+    free(s);
+}
+
+static void copy_in(int dim, int64_t n, const double* src, double* dst) {
+    for (int64_t i = 0; i < n; i++) {
+        dst[3 * i] = src[dim * i];
+        dst[3 * i + 1] = src[dim * i + 1];
+        dst[3 * i + 2] = dim == 3 ? src[dim * i + 2] : 0.0;
+    }
+}
+static void copy_out(int dim, int64_t n, const double* src, double* dst) {
+    for (int64_t i = 0; i < n; i++)
+        for (int a = 0; a < dim; a++) dst[dim * i + a] = src[3 * i + a];
+}
+
+int or_set_boundary(or_sim* s, int64_t n, const double* x, const double* nrm,
+                    const double* v) {
+    if (n < 0 || (n > 0 && (!x || !nrm))) return VPS_ERR_INVALID_ARG;
+    if (set_resize(&s->bnd, n)) return VPS_ERR_OOM;
+    s->bv = (double*)realloc(s->bv, (size_t)(n ? n : 1) * 3 * sizeof(double));
+    s->bn = (double*)realloc(s->bn, (size_t)(n ? n : 1) * 3 * sizeof(double));
+    copy_in(s->dim, n, x, s->bnd.x);
+    copy_in(s->dim, n, nrm, s->bn);
+    if (v) copy_in(s->dim, n, v, s->bv);
+    else memset(s->bv, 0, (size_t)(n ? n : 1) * 3 * sizeof(double));
+    return 0;
+}
+
+int or_set_particles(or_sim* s, int64_t n, const double* x, const double* v) {
+    if (n <= 0 || !x) return VPS_ERR_INVALID_ARG;
+    if (set_resize(&s->real, n)) return VPS_ERR_OOM;
+    s->v = (double*)realloc(s->v, (size_t)n * 3 * sizeof(double));
+    copy_in(s->dim, n, x, s->real.x);
+    if (v) copy_in(s->dim, n, v, s->v);
+    else memset(s->v, 0, (size_t)n * 3 * sizeof(double));
+    return 0;
+}
+
+int or_get_particles(or_sim* s, double* x, double* v) {
+    if (x) copy_out(s->dim, s->real.n, s->real.x, x);
+    if (v) copy_out(s->dim, s->real.n, s->v, v);
+    return 0;
+}
+
+
+/* ---- VirtualParticle::generateParticles (VirtualParticle.cpp:45-62) ---- */
+static void cb_vol_sq(void* c, int32_t j, const double* xj) {
+    /* volumes[I] += V_j * V_j * real.kernel(p_I - p_j) (VirtualParticle.cpp:34) */
+    sum_ctx* t = (sum_ctx*)c;
+    double d[3] = {t->q[0] - xj[0], t->q[1] - xj[1], t->q[2] - xj[2]};
+    t->acc += t->s->vol[j] * t->s->vol[j] * kern_Wv(t->k, d);
+}
+
+int or_generate(or_sim* s) {
+    const int64_t nn = s->nnodes;
+    if (s->real.n == 0) return VPS_ERR_STATE;
+    if (!s->w) s->w = (double*)malloc((size_t)nn * sizeof(double));
+    if (!s->w) return VPS_ERR_OOM;
+    /* _weight.parallelForEach: w_I = sum_j real.kernel(p_I - p_j) over real
+     * neighbours within h; OpenMP over z slices (y rows in 2D),
+     * Grid.cpp:246-266 */
+    const int64_t nx = s->p.lattice_dims[0], ny = s->p.lattice_dims[1],
+                  nz = s->p.lattice_dims[2];
+    const int64_t outer = s->dim == 3 ? nz : ny;
+    const int64_t inner = s->dim == 3 ? nx * ny : nx;
+    int64_t o;
+#pragma omp parallel for schedule(static)
+    for (o = 0; o < outer; o++) {
+        for (int64_t t = 0; t < inner; t++) {
+            const int64_t lin = o * inner + t;
+            double q[3];
+            lattice_pos(s, lin, q);
+            sum_ctx c = {&s->real, q, &s->real.k, 0, {0, 0, 0}, NULL, NULL};
+            set_for_each(&s->real, q, cb_weight, &c);
+            s->w[lin] = c.acc;
+        }
+    }
+    /* _weight.forEach: add(position) when weight > 1e-6, linear order */
+    int64_t cnt = 0;
+    for (int64_t lin = 0; lin < nn; lin++)
+        if (s->w[lin] > 1e-6) cnt++;
+    if (set_resize(&s->virt, cnt)) return VPS_ERR_OOM;
+    s->node = (int64_t*)realloc(s->node, (size_t)(cnt ? cnt : 1) * sizeof(int64_t));
+    cnt = 0;
+    for (int64_t lin = 0; lin < nn; lin++)
+        if (s->w[lin] > 1e-6) {
+            lattice_pos(s, lin, s->virt.x + 3 * cnt);
+            s->node[cnt++] = lin;
+        }
+    (void)nx; (void)nz;
+    /* resetNearbySearcher(); computeInfo(realParticles) */
+    set_reset(&s->virt);
+    const or_set* r = &s->real;
+    int64_t I;
+    /* VirtualParticle::computeDensities (VirtualParticle.cpp:18-27): the
+     * virtual kernel (H) over real neighbours found at h, times real mass */
+#pragma omp parallel for schedule(static)
+    for (I = 0; I < cnt; I++) {
+        sum_ctx c = {r, s->virt.x + 3 * I, &s->virt.k, 0, {0, 0, 0}, NULL, NULL};
+        set_for_each(r, s->virt.x + 3 * I, cb_weight, &c);
+        s->virt.rho[I] = c.acc * r->mass;
+    }
+    /* VirtualParticle::computeVolumes (VirtualParticle.cpp:29-38) */
+#pragma omp parallel for schedule(static)
+    for (I = 0; I < cnt; I++) {
+        sum_ctx c = {r, s->virt.x + 3 * I, &r->k, 0, {0, 0, 0}, NULL, NULL};
+        set_for_each(r, s->virt.x + 3 * I, cb_vol_sq, &c);
+        s->virt.vol[I] = c.acc;
+    }
+    return 0;
+}
+
+/* ---- VirtualParticle::calculateCoef (VirtualParticle.cpp:149-169) ------ *
+ * Rows assembled serially in the searcher's visit order, diagonal triplet
+ * appended; setFromTriplets sums the duplicate (I,I) pair (-0 + diag).  The
+ * CG operator M is the transpose (Eigen's TransposeInput for a column-major
+ * matrix with Lower|Upper) or A itself, stored as CSR with ascending
+ * columns: for M = A^T that is column I of A with ascending rows, the order
+ * in which Eigen's row-major sparse*dense product accumulates. */
+typedef struct {
+    int64_t cap, n;
+    int32_t* col;
+    double* val;
+} row_buf;
+
+typedef struct {
+    const or_sim* s;
+    const double* q;
+    double sum;
+    row_buf* rb;
+} coef_ctx;
+
+static void cb_coef(void* c, int32_t J, const double* pJ) {
+    coef_ctx* t = (coef_ctx*)c;
+    double d[3] = {t->q[0] - pJ[0], t->q[1] - pJ[1], t->q[2] - pJ[2]};
+    const double r_ij = sqrt(sqnorm(t->s->dim, d));
+    const double alpha_ij =
+        2. * t->s->virt.vol[J] * kern_dW(&t->s->virt.k, r_ij) / (r_ij + 1e-6);
+    t->sum -= alpha_ij;
+    row_buf* rb = t->rb;
+    if (rb->n == rb->cap) {
+        rb->cap = rb->cap ? 2 * rb->cap : 1024;
+        rb->col = (int32_t*)realloc(rb->col, (size_t)rb->cap * sizeof(int32_t));
+        rb->val = (double*)realloc(rb->val, (size_t)rb->cap * sizeof(double));
+    }
+    rb->col[rb->n] = J;
+    rb->val[rb->n] = alpha_ij;
+    rb->n++;
+}
+
+static int calculate_coef(or_sim* s) {
+    const int64_t n = s->virt.n;
+    free_matrix(s);
+    /* A as triplet rows (row-major CSR in visit order incl. the diagonal) */
+    int64_t* arow = (int64_t*)malloc((size_t)(n + 1) * sizeof(int64_t));
+    row_buf rb = {0, 0, NULL, NULL};
+    arow[0] = 0;
+    for (int64_t I = 0; I < n; I++) {
+        coef_ctx c = {s, s->virt.x + 3 * I, 0, &rb};
+        set_for_each(&s->virt, s->virt.x + 3 * I, cb_coef, &c);
+        const double d = c.sum < s->alpha0 ? s->alpha0 : c.sum; /* std::max */
+        if (rb.n == rb.cap) {
+            rb.cap = rb.cap ? 2 * rb.cap : 1024;
+            rb.col = (int32_t*)realloc(rb.col, (size_t)rb.cap * sizeof(int32_t));
+            rb.val = (double*)realloc(rb.val, (size_t)rb.cap * sizeof(double));
+        }
+        rb.col[rb.n] = (int32_t)I;
+        rb.val[rb.n] = d;
+        rb.n++;
+        arow[I + 1] = rb.n;
+    }
+    const int64_t nnz = rb.n;
+    /* M rows with duplicates still present */
+    int64_t* mrow = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+    int32_t* mcol = (int32_t*)malloc((size_t)(nnz ? nnz : 1) * sizeof(int32_t));
+    double* mval = (double*)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
+    if (s->p.op_mode == VPS_OP_TRANSPOSE) {
+        for (int64_t t = 0; t < nnz; t++) mrow[rb.col[t] + 1]++;
+        for (int64_t I = 0; I < n; I++) mrow[I + 1] += mrow[I];
+        int64_t* fill = (int64_t*)malloc((size_t)(n ? n : 1) * sizeof(int64_t));
+        memcpy(fill, mrow, (size_t)n * sizeof(int64_t));
+        for (int64_t I = 0; I < n; I++)
+            for (int64_t t = arow[I]; t < arow[I + 1]; t++) {
+                const int64_t dst = fill[rb.col[t]]++;
+                mcol[dst] = (int32_t)I;
+                mval[dst] = rb.val[t];
+            }
+        free(fill);
+    } else {
+        memcpy(mrow, arow, (size_t)(n + 1) * sizeof(int64_t));
+        for (int64_t I = 0; I < n; I++) {
+            /* stable insertion sort of row I by column */
+            for (int64_t t = arow[I] + 1; t < arow[I + 1]; t++) {
+                const int32_t c = rb.col[t];
+                const double v = rb.val[t];
+                int64_t u = t - 1;
+                while (u >= arow[I] && rb.col[u] > c) {
+                    rb.col[u + 1] = rb.col[u];
+                    rb.val[u + 1] = rb.val[u];
+                    u--;
+                }
+                rb.col[u + 1] = c;
+                rb.val[u + 1] = v;
+            }
+        }
+        memcpy(mcol, rb.col, (size_t)nnz * sizeof(int32_t));
+        memcpy(mval, rb.val, (size_t)nnz * sizeof(double));
+    }
+    free(rb.col);
+    free(rb.val);
+    free(arow);
+    /* collapse duplicates (first += later, SparseMatrix::collapseDuplicates) */
+    s->rowptr = (int64_t*)malloc((size_t)(n + 1) * sizeof(int64_t));
+    s->col = (int32_t*)malloc((size_t)(nnz ? nnz : 1) * sizeof(int32_t));
+    s->val = (double*)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
+    s->diag = (double*)malloc((size_t)(n ? n : 1) * sizeof(double));
+    int64_t w = 0;
+    s->rowptr[0] = 0;
+    for (int64_t I = 0; I < n; I++) {
+        const int64_t row_begin = w;
+        for (int64_t t = mrow[I]; t < mrow[I + 1]; t++) {
+            if (w > row_begin && s->col[w - 1] == mcol[t]) {
+                s->val[w - 1] += mval[t];
+            } else {
+                s->col[w] = mcol[t];
+                s->val[w] = mval[t];
+                w++;
+            }
+        }
+        s->rowptr[I + 1] = w;
+        s->diag[I] = 0;
+        for (int64_t t = row_begin; t < w; t++)
+            if (s->col[t] == I) s->diag[I] = s->val[t];
+    }
+    free(mrow);
+    free(mcol);
+    free(mval);
+    return 0;
+}
+
+/* ---- Eigen ConjugateGradient (IterativeSolver.h:19-40 and Eigen's
+ * internal::conjugate_gradient, identity preconditioner) ----------------- */
+static void csr_mv(int64_t n, const int64_t* rp, const int32_t* col,
+                   const double* val, const double* x, double* y) {
+    int64_t i;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < n; i++) {
+        double tmp = 0;
+        for (int64_t t = rp[i]; t < rp[i + 1]; t++) tmp += val[t] * x[col[t]];
+        y[i] = tmp;
+    }
+}
+static double vdot(int64_t n, const double* a, const double* b) {
+    double s = 0;
+    for (int64_t i = 0; i < n; i++) s += a[i] * b[i];
+    return s;
+}
+
+int or_cg_csr(int64_t n, const int64_t* rp, const int32_t* col,
+              const double* val, const double* b, double* x, double tol_in,
+              int64_t max_iter, int64_t* iters, double* err) {
+    /* solver.setTolerance(max(tolerance, eps / b.norm())) */
+    const double bnorm = sqrt(vdot(n, b, b));
+    const double tol = tol_in > DBL_EPSILON / bnorm ? tol_in : DBL_EPSILON / bnorm;
+    const int64_t max_iters = max_iter < 0 ? 2 * n : max_iter;
+    double* r = (double*)malloc((size_t)(n ? n : 1) * sizeof(double));
+    double* p = (double*)malloc((size_t)(n ? n : 1) * sizeof(double));
+    double* t = (double*)malloc((size_t)(n ? n : 1) * sizeof(double));
+    int status = 0;
+    csr_mv(n, rp, col, val, x, t);
+    for (int64_t i = 0; i < n; i++) r[i] = b[i] - t[i];
+    const double rhs2 = vdot(n, b, b);
+    if (rhs2 == 0) {
+        for (int64_t i = 0; i < n; i++) x[i] = 0;
+        *iters = 0;
+        *err = 0;
+        goto done;
+    }
+    {
+        double threshold = tol * tol * rhs2;
+        if (threshold < DBL_MIN) threshold = DBL_MIN;
+        double r2 = vdot(n, r, r);
+        if (r2 < threshold) {
+            *iters = 0;
+            *err = sqrt(r2 / rhs2);
+            goto done;
+        }
+        memcpy(p, r, (size_t)n * sizeof(double));
+        double abs_new = vdot(n, r, p);
+        int64_t it = 0;
+        while (it < max_iters) {
+            csr_mv(n, rp, col, val, p, t);
+            const double alpha = abs_new / vdot(n, p, t);
+            for (int64_t i = 0; i < n; i++) x[i] += alpha * p[i];
+            for (int64_t i = 0; i < n; i++) r[i] -= alpha * t[i];
+            r2 = vdot(n, r, r);
+            if (r2 < threshold) break;
+            const double abs_old = abs_new;
+            abs_new = vdot(n, r, r);
+            const double beta = abs_new / abs_old;
+            for (int64_t i = 0; i < n; i++) p[i] = r[i] + beta * p[i];
+            it++;
+        }
+        *err = sqrt(r2 / rhs2);
+        *iters = it;
+        status = *err <= tol ? 0 : VPS_ERR_NO_CONVERGENCE;
+    }
+done:
+    free(r);
+    free(p);
+    free(t);
+    return status;
+}
+
+/* ---- DualParticleSphLiquid::initialize (DualParticleSphLiquid.cpp:78-110),
+ * boundary sampling left to the caller --------------------------------- */
+int or_initialize(or_sim* s) {
+    if (s->real.n == 0) return VPS_ERR_STATE;
+    if (s->bnd.n == 0) return VPS_ERR_INVALID_ARG; /* empty set: % 0 in ref */
+    set_reset(&s->real);
+    set_compute_densities(&s->real);
+    set_compute_volumes(&s->real);
+    if (s->p.rest_density_mode == VPS_FROM_INIT) {
+        double m = s->real.rho[0]; /* ParticlesAttribute::max */
+        for (int64_t i = 1; i < s->real.n; i++)
+            if (m < s->real.rho[i]) m = s->real.rho[i];
+        s->rho0 = m;
+    } else {
+        s->rho0 = s->p.rest_density;
+    }
+    set_reset(&s->bnd);
+    set_compute_densities(&s->bnd);
+    set_compute_volumes(&s->bnd);
+    /* VirtualParticle::setAlpha (VirtualParticle.cpp:64-73) */
+    s->alpha0 = -1e6;
+    int rc = or_generate(s);
+    if (rc) return rc;
+    rc = calculate_coef(s);
+    if (rc) return rc;
+    double a = s->virt.n ? s->diag[0] : 0;
+    for (int64_t I = 1; I < s->virt.n; I++)
+        if (a < s->diag[I]) a = s->diag[I];
+    s->alpha0 = a;
+    /* setKappa(1 / packed kernel sum of the real particles) (:98) */
+    s->kappa = s->p.kappa_mode == VPS_FROM_INIT ? 1. / packed_sum(&s->real.k)
+                                                : s->p.kappa;
+    /* pressures start empty (zero-filled on the first resize) */
+    s->npres = 0;
+    free(s->prev_node_p);
+    free(s->prev_node_valid);
+    s->prev_node_p = NULL;
+    s->prev_node_valid = NULL;
+    s->initialized = 1;
+    return 0;
+}
+
+/* SmthParticleHydrodLiquid::moveParticles lines 80-81 */
+int or_refresh_real(or_sim* s) {
+    if (!s->initialized) return VPS_ERR_STATE;
+    set_reset(&s->real);
+    set_compute_densities(&s->real);
+    return 0;
+}
+
+/* force += volumes[J] * _pressures[J] * gradientKernel(p_i - p_J)
+ * (VirtualParticle.cpp:118-121), virtual kernel H */
+typedef struct {
+    const or_sim* s;
+    const double* q;
+    double f[3];
+} grad_virtual_ctx;
+static void cb_grad_virtual(void* c, int32_t J, const double* pJ) {
+    grad_virtual_ctx* t = (grad_virtual_ctx*)c;
+    const or_sim* s = t->s;
+    double d[3] = {t->q[0] - pJ[0], t->q[1] - pJ[1], t->q[2] - pJ[2]};
+    double g[3];
+    kern_grad(&s->virt.k, d, g);
+    const double sc = s->virt.vol[J] * s->pres[J];
+    for (int a = 0; a < s->dim; a++) t->f[a] += sc * g[a];
+}
+/* force -= V_b * min(n_b.(v_i - v_b), 0) * n_b * kernel_b(p_i - p_b)
+ * (VirtualParticle.cpp:124-128); v_i is the pre-update velocity */
+typedef struct {
+    const or_sim* s;
+    const double* q;
+    const double* vi;
+    double f[3];
+} grad_boundary_ctx;
+static void cb_grad_boundary(void* c, int32_t b, const double* pb) {
+    grad_boundary_ctx* t = (grad_boundary_ctx*)c;
+    const or_sim* s = t->s;
+    double d[3] = {t->q[0] - pb[0], t->q[1] - pb[1], t->q[2] - pb[2]};
+    double dv[3] = {t->vi[0] - s->bv[3 * b], t->vi[1] - s->bv[3 * b + 1],
+                    t->vi[2] - s->bv[3 * b + 2]};
+    const double dn = dot3(s->dim, s->bn + 3 * b, dv);
+    const double m = 0. < dn ? 0. : dn; /* std::min(dn, 0.) */
+    const double sc = s->bnd.vol[b] * m;
+    const double w = kern_Wv(&s->bnd.k, d);
+    for (int a = 0; a < s->dim; a++) t->f[a] -= sc * s->bn[3 * b + a] * w;
+}
+
+/* applyPressureGradient (VirtualParticle.cpp:110-134) */
+static void pressure_gradient(or_sim* s) {
+    const int dim = s->dim;
+    const or_set* vs = &s->virt;
+    const or_set* bs = &s->bnd;
+    int64_t i;
+    double* vnew = (double*)malloc((size_t)s->real.n * 3 * sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < s->real.n; i++) {
+        const double* pi = s->real.x + 3 * i;
+        double force[3] = {0, 0, 0};
+        grad_virtual_ctx cv = {s, pi, {0, 0, 0}};
+        set_for_each(vs, pi, cb_grad_virtual, &cv);
+        grad_boundary_ctx cb = {s, pi, s->v + 3 * i, {cv.f[0], cv.f[1], cv.f[2]}};
+        set_for_each(bs, pi, cb_grad_boundary, &cb);
+        force[0] = cb.f[0];
+        force[1] = cb.f[1];
+        force[2] = cb.f[2];
+        for (int a = 0; a < 3; a++) vnew[3 * i + a] = s->v[3 * i + a] + force[a];
+        if (dim == 2) vnew[3 * i + 2] = 0;
+    }
+    memcpy(s->v, vnew, (size_t)s->real.n * 3 * sizeof(double));
+    free(vnew);
+}
+
+int or_pressure_gradient(or_sim* s) {
+    if (!s->initialized || !s->pres || s->npres != s->virt.n) return VPS_ERR_STATE;
+    pressure_gradient(s);
+    return 0;
+}
+
+/* ---- VirtualParticle::project (VirtualParticle.cpp:75-134) ------------- */
+int or_project(or_sim* s, or_stats* st) {
+    if (!s->initialized) return VPS_ERR_STATE;
+    const int dim = s->dim;
+    const int64_t n = s->virt.n;
+    double t0 = now_s();
+    /* _pressures.resize(this): keep the first min(old,new), zero the rest
+     * (ParticlesBasedData.h:25-29); other warm starts are options */
+    double* p_new = (double*)calloc((size_t)(n ? n : 1), sizeof(double));
+    if (s->p.warm_start == VPS_WARM_INDEX_CARRY) {
+        const int64_t keep = s->npres < n ? s->npres : n;
+        if (s->pres) memcpy(p_new, s->pres, (size_t)keep * sizeof(double));
+    } else if (s->p.warm_start == VPS_WARM_NODE_KEYED ||
+               s->p.warm_start == VPS_WARM_LEARNED) {
+        if (s->prev_node_p)
+            for (int64_t I = 0; I < n; I++)
+                if (s->prev_node_valid[s->node[I]])
+                    p_new[I] = s->prev_node_p[s->node[I]];
+    }
+    free(s->pres);
+    s->pres = p_new;
+    s->npres = n;
+    s->div = (double*)realloc(s->div, (size_t)(n ? n : 1) * sizeof(double));
+    s->vbar = (double*)realloc(s->vbar, (size_t)(n ? n : 1) * 3 * sizeof(double));
+    int64_t I;
+    /* calculateVelocity (VirtualParticle.cpp:136-147) */
+#pragma omp parallel for schedule(static)
+    for (I = 0; I < n; I++) {
+        const double* q = s->virt.x + 3 * I;
+        double vr[3], vb[3];
+        field_value(&s->real, s->v, q, vr);
+        field_value(&s->bnd, s->bv, q, vb);
+        const double den = bias_weight(&s->real, q) + bias_weight(&s->bnd, q) + 1e-6;
+        for (int a = 0; a < 3; a++) s->vbar[3 * I + a] = (vr[a] + vb[a]) / den;
+    }
+    /* calculateDiv (VirtualParticle.cpp:171-201) */
+#pragma omp parallel for schedule(static)
+    for (I = 0; I < n; I++) {
+        const double* q = s->virt.x + 3 * I;
+        const double* vb = s->vbar + 3 * I;
+        double gr[3], gb[3];
+        bias_gradient(&s->real, q, gr);
+        bias_gradient(&s->bnd, q, gb);
+        const double drho = s->virt.rho[I] - s->rho0;
+        s->div[I] = field_divergence(&s->real, s->v, q) - dot3(dim, vb, gr) +
+                    field_divergence(&s->bnd, s->bv, q) - dot3(dim, vb, gb) -
+                    s->kappa * (drho < 0. ? 0. : drho) / s->rho0;
+    }
+    int rc = calculate_coef(s);
+    if (rc) return rc;
+    double t1 = now_s();
+    /* solveLinearSystem -> IterativeSolver::solve */
+    int64_t iters = 0;
+    double err = 0;
+    rc = or_cg_csr(n, s->rowptr, s->col, s->val, s->div, s->pres,
+                   s->p.tolerance, s->p.max_iterations, &iters, &err);
+    double t2 = now_s();
+    /* node-keyed record of this step's pressures */
+    if (!s->prev_node_p) {
+        s->prev_node_p = (double*)calloc((size_t)s->nnodes, sizeof(double));
+        s->prev_node_valid = (unsigned char*)calloc((size_t)s->nnodes, 1);
+    } else {
+        memset(s->prev_node_valid, 0, (size_t)s->nnodes);
+    }
+    for (I = 0; I < n; I++) {
+        s->prev_node_p[s->node[I]] = s->pres[I];
+        s->prev_node_valid[s->node[I]] = 1;
+    }
+    pressure_gradient(s);
+    double t3 = now_s();
+    if (st) {
+        st->n_virtual = n;
+        st->cg_iterations = iters;
+        st->cg_error = err;
+        st->rhs_norm = sqrt(vdot(n, s->div, s->div));
+        st->cg_status = rc;
+        st->s_build = t1 - t0;
+        st->s_solve = t2 - t1;
+        st->s_gradient = t3 - t2;
+    }
+    return rc;
+}
+
+int or_projection_step(or_sim* s, or_stats* st) {
+    double t0 = now_s();
+    int rc = or_refresh_real(s);
+    if (rc) return rc;
+    double t1 = now_s();
+    rc = or_generate(s);
+    if (rc) return rc;
+    double t2 = now_s();
+    rc = or_project(s, st);
+    double t3 = now_s();
+    if (st) {
+        st->s_refresh = t1 - t0;
+        st->s_generate = t2 - t1;
+        st->s_total = t3 - t0;
+    }
+    return rc;
+}
+
+/* x += v dt; v.y -= g dt (SmthParticleHydrodLiquid.cpp:73, 84-88) */
+int or_advect(or_sim* s, double dt, double gravity) {
+    int64_t i;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < s->real.n; i++) {
+        for (int a = 0; a < s->dim; a++) s->real.x[3 * i + a] += s->v[3 * i + a] * dt;
+        s->v[3 * i + 1] -= gravity * dt;
+    }
+    return 0;
+}
+
+/* ---- read-back ---------------------------------------------------------- */
+int64_t or_virtual_count(or_sim* s) { return s->virt.n; }
+
+int or_get_virtual(or_sim* s, int64_t* node, double* rho, double* vol,
+                   double* vbar, double* div, double* diag, double* p) {
+    const int64_t n = s->virt.n;
+    for (int64_t I = 0; I < n; I++) {
+        if (node) node[I] = s->node[I];
+        if (rho) rho[I] = s->virt.rho[I];
+        if (vol) vol[I] = s->virt.vol[I];
+        if (vbar && s->vbar)
+            for (int a = 0; a < s->dim; a++) vbar[s->dim * I + a] = s->vbar[3 * I + a];
+        if (div && s->div) div[I] = s->div[I];
+        if (diag && s->diag) diag[I] = s->diag[I];
+        if (p && s->pres && I < s->npres) p[I] = s->pres[I];
+    }
+    return 0;
+}
+
+int or_set_pressure(or_sim* s, const double* p) {
+    const int64_t n = s->virt.n;
+    double* np = (double*)malloc((size_t)(n ? n : 1) * sizeof(double));
+    memcpy(np, p, (size_t)n * sizeof(double));
+    free(s->pres);
+    s->pres = np;
+    s->npres = n;
+    return 0;
+}
+
+int or_get_real_fields(or_sim* s, double* rho, double* vol) {
+    for (int64_t i = 0; i < s->real.n; i++) {
+        if (rho) rho[i] = s->real.rho[i];
+        if (vol) vol[i] = s->real.vol[i];
+    }
+    return 0;
+}
+int or_get_boundary_fields(or_sim* s, double* rho, double* vol) {
+    for (int64_t i = 0; i < s->bnd.n; i++) {
+        if (rho) rho[i] = s->bnd.rho[i];
+        if (vol) vol[i] = s->bnd.vol[i];
+    }
+    return 0;
+}
+int or_get_scalars(or_sim* s, double* rho0, double* kappa, double* alpha0) {
+    if (rho0) *rho0 = s->rho0;
+    if (kappa) *kappa = s->kappa;
+    if (alpha0) *alpha0 = s->alpha0;
+    return 0;
+}
+int or_get_node_weights(or_sim* s, double* w) {
+    if (!s->w) return VPS_ERR_STATE;
+    memcpy(w, s->w, (size_t)s->nnodes * sizeof(double));
+    return 0;
+}
+
+typedef struct {
+    int64_t* buf;
+    int64_t n, cap;
+} list_ctx;
+static void cb_list(void* c, int32_t j, const double* xj) {
+    list_ctx* t = (list_ctx*)c;
+    (void)xj;
+    if (t->n < t->cap) t->buf[t->n] = j;
+    t->n++;
+}
+
+static int query_impl(or_sim* s, int32_t which, int64_t nq, const double* q,
+                      int64_t* offsets, int64_t* idx, int64_t cap,
+                      int64_t* needed, int sorted) {
+    const or_set* set = which == 0 ? &s->real : which == 1 ? &s->bnd : &s->virt;
+    int64_t total = 0;
+    offsets[0] = 0;
+    for (int64_t k = 0; k < nq; k++) {
+        double qq[3] = {q[s->dim * k], q[s->dim * k + 1],
+                        s->dim == 3 ? q[s->dim * k + 2] : 0.0};
+        list_ctx c = {idx ? idx + total : NULL, 0, idx ? cap - total : 0};
+        if (c.cap < 0) c.cap = 0;
+        set_for_each(set, qq, cb_list, &c);
+        if (sorted && c.n <= c.cap && idx)
+            qsort(idx + total, (size_t)c.n, sizeof(int64_t), cmp_i64);
+        total += c.n;
+        offsets[k + 1] = total;
+    }
+    if (needed) *needed = total;
+    return total > cap ? VPS_ERR_INVALID_ARG : 0;
+}
+int or_query(or_sim* s, int32_t which, int64_t nq, const double* q,
+             int64_t* offsets, int64_t* idx, int64_t cap, int64_t* needed) {
+    return query_impl(s, which, nq, q, offsets, idx, cap, needed, 1);
+}
+int or_query_ordered(or_sim* s, int32_t which, int64_t nq, const double* q,
+                     int64_t* offsets, int64_t* idx, int64_t cap,
+                     int64_t* needed) {
+    return query_impl(s, which, nq, q, offsets, idx, cap, needed, 0);
+}
+
+int64_t or_matrix_nnz(or_sim* s) {
+    return s->rowptr ? s->rowptr[s->virt.n] : 0;
+}
+int or_get_matrix(or_sim* s, int64_t* rowptr, int32_t* col, double* val) {
+    if (!s->rowptr) return VPS_ERR_STATE;
+    const int64_t n = s->virt.n;
+    memcpy(rowptr, s->rowptr, (size_t)(n + 1) * sizeof(int64_t));
+    if (col) memcpy(col, s->col, (size_t)s->rowptr[n] * sizeof(int32_t));
+    if (val) memcpy(val, s->val, (size_t)s->rowptr[n] * sizeof(double));
+    return 0;
+}
+int or_apply_operator(or_sim* s, const double* p, double* out) {
+    if (!s->rowptr) return VPS_ERR_STATE;
+    csr_mv(s->virt.n, s->rowptr, s->col, s->val, p, out);
+    return 0;
+}

@@ -1,0 +1,234 @@
+// vps_cells.hip -- uniform-grid counting sort and real/boundary densities.
+//
+// Replaces HashGridSearcher::reset (ParticlesNearbySearcher.cpp:10-18) and
+// SmoothedParticles::computeDensities / computeVolumes
+// (SmoothedParticles.cpp:25-36).  The reference hashes cells into 2N
+// buckets with OR-combined primes and walks 27 sorted bucket keys per query;
+// here particles are counting-sorted by a dense x-fastest cell id (cell size
+// h (1 + 1e-6)), so the 3 cells of an x-row are one contiguous range and a
+// query reads 9 ranges (3 in 2D).  Membership is the reference's exact FP64
+// test (x_j - q).squaredNorm() < h^2 (.cpp:32), so the neighbour sets are
+// identical; only the visiting order differs.  Within a cell particles are
+// ordered by caller index, so every sum is deterministic.
+#include "vps_internal.h"
+
+namespace vps {
+
+namespace {
+
+template <int D>
+__global__ void __launch_bounds__(256) k_cell_count(const double4* pos,
+                                                    int64_t n, KConst k,
+                                                    int32_t* count,
+                                                    int32_t* cell_of,
+                                                    int32_t* slot_of) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double4 p = pos[i];
+    const int cx = cell_coord(p.x, k.cell_lo[0], k.cell_inv, k.cdims[0]);
+    const int cy = cell_coord(p.y, k.cell_lo[1], k.cell_inv, k.cdims[1]);
+    const int cz = D == 3 ? cell_coord(p.z, k.cell_lo[2], k.cell_inv, k.cdims[2]) : 0;
+    const int c = (cz * k.cdims[1] + cy) * k.cdims[0] + cx;
+    cell_of[i] = c;
+    slot_of[i] = atomicAdd(&count[c], 1);
+}
+
+__global__ void __launch_bounds__(256) k_cell_scatter(const int32_t* cell_of,
+                                                      const int32_t* slot_of,
+                                                      const int32_t* cstart,
+                                                      int64_t n, int32_t* perm) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    perm[cstart[cell_of[i]] + slot_of[i]] = (int32_t)i;
+}
+
+// Orders each cell's entries by caller id (atomics gave an arbitrary order).
+__global__ void __launch_bounds__(256) k_cell_order(const int32_t* cstart,
+                                                    int64_t ncells,
+                                                    int32_t* perm,
+                                                    const int32_t* id) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= ncells) return;
+    const int b = cstart[c], e = cstart[c + 1];
+    for (int t = b + 1; t < e; t++) {
+        const int v = perm[t];
+        const int key = id[v];
+        int u = t - 1;
+        while (u >= b && id[perm[u]] > key) {
+            perm[u + 1] = perm[u];
+            u--;
+        }
+        perm[u + 1] = v;
+    }
+}
+
+template <typename A, typename B>
+__global__ void __launch_bounds__(256) k_gather3(const int32_t* perm, int64_t n,
+                                                 const double4* p_in,
+                                                 double4* p_out, const A* a_in,
+                                                 A* a_out, const B* b_in,
+                                                 B* b_out, const int32_t* id_in,
+                                                 int32_t* id_out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int s = perm[i];
+    p_out[i] = p_in[s];
+    a_out[i] = a_in[s];
+    if (b_in) b_out[i] = b_in[s];
+    id_out[i] = id_in[s];
+}
+
+// rho_i = m * sum_j W_h(|x_i - x_j|) over j with FP64 |x_j - x_i|^2 < h^2
+// (SmoothedParticles.cpp:25-31 via getNeighborWeight, .cpp:48-54).
+template <int D>
+__global__ void __launch_bounds__(256) k_density(const double4* pos,
+                                                 const int32_t* cstart,
+                                                 int64_t n, KConst k,
+                                                 float* rho, float4* vel,
+                                                 int set_volume) {
+    const int nb = (int)((n + 255) / 256);
+    const int64_t i = (int64_t)xcd_remap(blockIdx.x, nb) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double4 q = pos[i];
+    float sum = 0.f;
+    for_each_candidate<D>(k, cstart, q.x, q.y, q.z, [&](int j) {
+        const double4 pj = pos[j];
+        const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, q.x, q.y, q.z)
+                                 : sq_dist2(pj.x, pj.y, q.x, q.y);
+        if (d2 < k.h2) {
+            const KW w = kernel_f32(d2, k.h2, k.hf, k.inv_h, k.c0h, k.c1h);
+            sum += w.w;
+        }
+    });
+    const float r = k.mass * sum;
+    rho[i] = r;
+    if (set_volume) vel[i].w = k.mass / r;
+}
+
+// Boundary density from boundary-only neighbours (the boundary set has its
+// own searcher, DualParticleSphLiquid.cpp:96-97); V_b = m / rho_b.
+template <int D>
+__global__ void __launch_bounds__(256) k_bdensity(const double4* pos,
+                                                  const int32_t* cstart,
+                                                  int64_t n, KConst k,
+                                                  float4* nrm, float4* vel) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double4 q = pos[i];
+    float sum = 0.f;
+    for_each_candidate<D>(k, cstart, q.x, q.y, q.z, [&](int j) {
+        const double4 pj = pos[j];
+        const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, q.x, q.y, q.z)
+                                 : sq_dist2(pj.x, pj.y, q.x, q.y);
+        if (d2 < k.h2) {
+            const KW w = kernel_f32(d2, k.h2, k.hf, k.inv_h, k.c0h, k.c1h);
+            sum += w.w;
+        }
+    });
+    const float r = k.mass * sum;
+    nrm[i].w = r;
+    vel[i].w = k.mass / r;
+}
+
+inline unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+// Counting sort of a particle set into the cell grid.  perm/cell_of/slot_of
+// are scratch of length n; cstart gets ncells + 1 entries.
+template <typename A, typename B>
+static int sort_set(vps_context* c, int64_t n, const double4* pin,
+                    double4* pout, const A* ain, A* aout, const B* bin,
+                    B* bout, const int32_t* idin, int32_t* idout,
+                    int32_t* cstart) {
+    const int64_t C = c->ncells;
+    VPS_HIP(c, hipMemsetAsync(c->cell_count.p, 0, (size_t)C * 4, c->stream));
+    if (c->dim == 3)
+        hipLaunchKernelGGL(k_cell_count<3>, dim3(blocks(n, 256)), dim3(256), 0,
+                           c->stream, pin, n, c->k, c->cell_count.p,
+                           c->cell_of.p, c->slot_of.p);
+    else
+        hipLaunchKernelGGL(k_cell_count<2>, dim3(blocks(n, 256)), dim3(256), 0,
+                           c->stream, pin, n, c->k, c->cell_count.p,
+                           c->cell_of.p, c->slot_of.p);
+    VPS_TRY(scan_exclusive_i32(c, c->cell_count.p, cstart, C, cstart + C));
+    hipLaunchKernelGGL(k_cell_scatter, dim3(blocks(n, 256)), dim3(256), 0,
+                       c->stream, c->cell_of.p, c->slot_of.p, cstart, n,
+                       c->perm.p);
+    hipLaunchKernelGGL(k_cell_order, dim3(blocks(C, 256)), dim3(256), 0,
+                       c->stream, cstart, C, c->perm.p, idin);
+    hipLaunchKernelGGL((k_gather3<A, B>), dim3(blocks(n, 256)), dim3(256), 0,
+                       c->stream, c->perm.p, n, pin, pout, ain, aout, bin,
+                       bout, idin, idout);
+    VPS_HIP(c, hipGetLastError());
+    return VPS_OK;
+}
+
+int cells_sort_real(vps_context* c) {
+    const int s = c->cur, d = 1 - c->cur;
+    VPS_TRY((sort_set<float4, float4>(c, c->n, c->rpos[s].p, c->rpos[d].p,
+                                      c->rvel[s].p, c->rvel[d].p, nullptr,
+                                      nullptr, c->rid[s].p, c->rid[d].p,
+                                      c->cell_start.p)));
+    c->cur = d;
+    return VPS_OK;
+}
+
+int cells_sort_boundary(vps_context* c) {
+    // sort in place through temporaries
+    const int64_t nb = c->nb;
+    DBuf<double4> p2;
+    DBuf<float4> v2, n2;
+    DBuf<int32_t> i2;
+    VPS_TRY(dalloc(c, p2, nb));
+    VPS_TRY(dalloc(c, v2, nb));
+    VPS_TRY(dalloc(c, n2, nb));
+    VPS_TRY(dalloc(c, i2, nb));
+    VPS_TRY(dalloc(c, c->cell_of, nb > c->n ? nb : c->n));
+    VPS_TRY(dalloc(c, c->slot_of, nb > c->n ? nb : c->n));
+    VPS_TRY(dalloc(c, c->perm, nb > c->n ? nb : c->n));
+    VPS_TRY((sort_set<float4, float4>(c, nb, c->bpos.p, p2.p, c->bvel.p, v2.p,
+                                      c->bnrm.p, n2.p, c->bid.p, i2.p,
+                                      c->bcell_start.p)));
+    std::swap(c->bpos, p2);
+    std::swap(c->bvel, v2);
+    std::swap(c->bnrm, n2);
+    std::swap(c->bid, i2);
+    VPS_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(p2);
+    dfree(v2);
+    dfree(n2);
+    dfree(i2);
+    return VPS_OK;
+}
+
+int real_density(vps_context* c, bool set_volume) {
+    const int64_t n = c->n;
+    const int s = c->cur;
+    if (c->dim == 3)
+        hipLaunchKernelGGL(k_density<3>, dim3(blocks(n, 256)), dim3(256), 0,
+                           c->stream, c->rpos[s].p, c->cell_start.p, n, c->k,
+                           c->rrho.p, c->rvel[s].p, set_volume ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_density<2>, dim3(blocks(n, 256)), dim3(256), 0,
+                           c->stream, c->rpos[s].p, c->cell_start.p, n, c->k,
+                           c->rrho.p, c->rvel[s].p, set_volume ? 1 : 0);
+    VPS_HIP(c, hipGetLastError());
+    return VPS_OK;
+}
+
+int boundary_density(vps_context* c) {
+    const int64_t nb = c->nb;
+    if (c->dim == 3)
+        hipLaunchKernelGGL(k_bdensity<3>, dim3(blocks(nb, 256)), dim3(256), 0,
+                           c->stream, c->bpos.p, c->bcell_start.p, nb, c->k,
+                           c->bnrm.p, c->bvel.p);
+    else
+        hipLaunchKernelGGL(k_bdensity<2>, dim3(blocks(nb, 256)), dim3(256), 0,
+                           c->stream, c->bpos.p, c->bcell_start.p, nb, c->k,
+                           c->bnrm.p, c->bvel.p);
+    VPS_HIP(c, hipGetLastError());
+    return VPS_OK;
+}
+
+}  // namespace vps

@@ -1,0 +1,403 @@
+// vps_internal.h -- shared definitions of the MI355X projection library.
+//
+// Data layout in HBM (one context, one device):
+//   real particles   sorted by cell every step; SoA-of-vectors:
+//                    rpos  double4 (x, y, z, 0)      exact FP64 membership
+//                    rvel  float4  (vx, vy, vz, V)   V frozen at initialize
+//                    rrho  float                      current density
+//                    rid   int32                      caller index
+//   boundary         same layout, sorted once (bpos, bvel = (v, V),
+//                    bnrm = (n, rho))
+//   cell grid        uniform, cell size h (1 + 1e-6), x-fastest linear id,
+//                    cell_start[C + 1]; the 3 cells of an x-row are
+//                    contiguous in the sorted arrays, so a query walks
+//                    3^(D-1) contiguous ranges
+//   virtual lattice  bricks of 8^D nodes; brick_slot[brick] -> slot or -1;
+//                    every per-node field is stored slot-major
+//                    (slot * 8^D + local, local x fastest) so a brick is one
+//                    contiguous 2 KB (3D) run per field
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/vps.h"
+
+namespace vps {
+
+constexpr int kBrick = 8;                  // brick side in lattice nodes
+constexpr int kBrickNodes3 = 512;
+constexpr int kBrickNodes2 = 64;
+constexpr int kStencilR = 3;               // |offset| < 4 lattice spacings
+constexpr int kMaxReduceBlocks = 65536;
+
+// Constants every kernel needs, passed by value.
+struct KConst {
+    int dim;
+    // kernel radii: h for real/boundary (4 * particle radius), H for virtual
+    // (4 * lattice spacing), squared in FP64 exactly as the reference does
+    double h, h2, H, H2;
+    double inv_h64, c0h64;                 // FP64 weight for the 1e-6 test
+    float hf, inv_h, c0h, c1h;             // FP32 kernel constants, radius h
+    float Hf, inv_H, c0H, c1H;             // FP32 kernel constants, radius H
+    // lattice
+    double origin[3];
+    double spacing;
+    int ndims[3];
+    int bdims[3];
+    // cell grid
+    double cell_lo[3];
+    double cell_inv;
+    int cdims[3];
+    // physics
+    float mass;
+    float rho0;
+    float kappa;
+    float pad;
+};
+
+// ---- FP64 helpers with the reference's rounding ----------------------
+// Node position origin + c * spacing (Grid::dataPosition, Grid.h:51): one
+// product and one sum, each rounded.
+__device__ __forceinline__ double node_coord(double o, int c, double s) {
+#pragma clang fp contract(off)
+    return o + (double)c * s;
+}
+
+// (a - b).squaredNorm() as ((dx*dx + dy*dy) + dz*dz), no contraction.
+__device__ __forceinline__ double sq_dist3(double ax, double ay, double az,
+                                           double bx, double by, double bz) {
+#pragma clang fp contract(off)
+    const double dx = ax - bx, dy = ay - by, dz = az - bz;
+    return (dx * dx + dy * dy) + dz * dz;
+}
+__device__ __forceinline__ double sq_dist2(double ax, double ay, double bx,
+                                           double by) {
+#pragma clang fp contract(off)
+    const double dx = ax - bx, dy = ay - by;
+    return dx * dx + dy * dy;
+}
+template <int D>
+__device__ __forceinline__ double sq_dist(const double* a, const double* b) {
+    if constexpr (D == 2) return sq_dist2(a[0], a[1], b[0], b[1]);
+    else return sq_dist3(a[0], a[1], a[2], b[0], b[1], b[2]);
+}
+__device__ __forceinline__ double dsub(double a, double b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
+__device__ __forceinline__ double dsq(double a) {
+#pragma clang fp contract(off)
+    return a * a;
+}
+__device__ __forceinline__ double dadd(double a, double b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+
+// FP64 cubic spline W (SmoothedParticles.h:60-65), for the virtual-set
+// threshold test only.
+__device__ __forceinline__ double kernel_w64(double d2, double inv_kr,
+                                             double c0) {
+#pragma clang fp contract(off)
+    const double x = sqrt(d2) * inv_kr;
+    if (x < .5) return c0 * (6 * x * x * (x - 1) + 1);
+    else if (x < 1) return c0 * 2 * (1 - x) * (1 - x) * (1 - x);
+    return 0;
+}
+
+// FP32 cubic spline value and first derivative at squared distance d2 < kr2.
+// For x >= 1/2 the factor (1 - x) is formed as (kr^2 - d^2) / (kr (kr + r))
+// from the exact FP64 difference kr2 - d2, so values near the support edge
+// keep full FP32 relative accuracy.
+struct KW {
+    float w, dw, r;
+};
+__device__ __forceinline__ KW kernel_f32(double d2, double kr2, float kr,
+                                         float inv_kr, float c0, float c1) {
+    KW o;
+    const float r = sqrtf((float)d2);
+    const float x = r * inv_kr;
+    o.r = r;
+    if (!(d2 < kr2)) {
+        o.w = 0.f;
+        o.dw = 0.f;
+    } else if (x < 0.5f) {
+        o.w = c0 * (6.f * x * x * (x - 1.f) + 1.f);
+        o.dw = c1 * x * (3.f * x - 2.f);
+    } else {
+        const float omx = (float)(kr2 - d2) / (kr * (kr + r));
+        o.w = 2.f * c0 * omx * omx * omx;
+        o.dw = -c1 * omx * omx;
+    }
+    return o;
+}
+
+// ---- launch geometry ----------------------------------------------------
+// Workgroups are dealt round-robin to the 8 XCDs of an MI355X.  Remapping
+// the hardware block id so that each XCD gets a contiguous range of logical
+// blocks keeps spatial neighbours (which share halo data) in one L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
+__device__ __forceinline__ int cell_coord(double p, double lo, double inv,
+                                          int n) {
+    int c = (int)floor((p - lo) * inv);
+    return c < 0 ? 0 : (c >= n ? n - 1 : c);
+}
+
+// Walks the candidates of a radius-h query: the 3^(D-1) x-rows of 3 cells
+// around the query cell, each one contiguous range of the cell-sorted set.
+template <int D, typename F>
+__device__ __forceinline__ void for_each_candidate(const KConst& k,
+                                                   const int32_t* cstart,
+                                                   double qx, double qy,
+                                                   double qz, F&& f) {
+    const int cx = cell_coord(qx, k.cell_lo[0], k.cell_inv, k.cdims[0]);
+    const int cy = cell_coord(qy, k.cell_lo[1], k.cell_inv, k.cdims[1]);
+    const int cz = D == 3 ? cell_coord(qz, k.cell_lo[2], k.cell_inv, k.cdims[2]) : 0;
+    const int x0 = cx > 0 ? cx - 1 : 0;
+    const int x1 = cx + 1 < k.cdims[0] ? cx + 1 : k.cdims[0] - 1;
+    const int zlo = D == 3 ? (cz > 0 ? cz - 1 : 0) : 0;
+    const int zhi = D == 3 ? (cz + 1 < k.cdims[2] ? cz + 1 : k.cdims[2] - 1) : 0;
+    const int ylo = cy > 0 ? cy - 1 : 0;
+    const int yhi = cy + 1 < k.cdims[1] ? cy + 1 : k.cdims[1] - 1;
+    for (int z = zlo; z <= zhi; z++)
+        for (int y = ylo; y <= yhi; y++) {
+            const int row = (z * k.cdims[1] + y) * k.cdims[0];
+            const int b = cstart[row + x0], e = cstart[row + x1 + 1];
+            for (int j = b; j < e; j++) f(j);
+        }
+}
+
+// ---- reductions ----------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T u = __shfl_xor(v, o, 64);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+// Deterministic grid-wide sum: every block writes its partial, the last block
+// to arrive sums the partials in index order.  Returns true in the block that
+// produced the total (written to *out by thread 0).
+template <int BS>
+__device__ bool grid_sum_last(double v, double* partials, unsigned* counter,
+                              double* out) {
+    __shared__ double sred[BS / 64];
+    __shared__ bool is_last;
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) sred[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int i = 0; i < BS / 64; i++) s += sred[i];
+        partials[blockIdx.x] = s;
+        __threadfence();
+        const unsigned t = atomicAdd(counter, 1u);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    double s = 0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += BS)
+        s += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    s = wave_sum(s);
+    __syncthreads();
+    if (lane == 0) sred[wv] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int i = 0; i < BS / 64; i++) t += sred[i];
+        *out = t;
+        *counter = 0;
+    }
+    return true;
+}
+
+// ---- device buffers -------------------------------------------------------
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+};
+
+// CG scalars living on the device (no host round trip per iteration).
+struct CgState {
+    double bb;        // b.b
+    double thr;       // tol^2 b.b (or DBL_MIN)
+    double abs_new;   // r.r of the current residual
+    double rr;        // latest r.r
+    double pt;        // p.t
+    double true_rr;   // ||b - M x||^2 from the FP64 check
+    double thr_scale; // tol^2
+    float alpha, beta;
+    int iter;         // Eigen iteration counter
+    int done;         // 0 running, 1 converged, 2 max iterations, 3 nonfinite
+    int first;        // 1 before the first SpMV of a (re)start
+    int pad;
+    unsigned counter[4];
+};
+
+}  // namespace vps
+
+// ---- the context ---------------------------------------------------------
+struct vps_context {
+    vps_params p{};
+    vps::KConst k{};
+    int dim = 3;
+    int brick_nodes = 512;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::string err;
+
+    // real particles (double buffered for the reorder)
+    int64_t n = 0;
+    vps::DBuf<double4> rpos[2];
+    vps::DBuf<float4> rvel[2];
+    vps::DBuf<int32_t> rid[2];
+    int cur = 0;
+    vps::DBuf<float> rrho;
+    vps::DBuf<int32_t> cell_of, slot_of, perm;
+    int64_t ncells = 0;
+    vps::DBuf<int32_t> cell_count, cell_start;
+
+    // boundary
+    int64_t nb = 0;
+    vps::DBuf<double4> bpos;
+    vps::DBuf<float4> bvel;   // (v, V_b)
+    vps::DBuf<float4> bnrm;   // (n, rho_b)
+    vps::DBuf<int32_t> bid;
+    vps::DBuf<int32_t> bcell_start;
+    std::vector<double> host_bx, host_bn, host_bv;
+
+    // lattice / bricks
+    int64_t nnodes = 0, nbricks = 0;
+    int64_t nslots = 0;                 // active bricks this step
+    int which = 0;                      // current slot map (double buffered)
+    int last_solve_buf = -1;            // slot map holding the last pressures
+    int64_t last_solve_nvirt = 0;       // virtual count at the last solve
+    vps::DBuf<uint32_t> brick_flag;
+    vps::DBuf<int32_t> brick_slot[2];
+    vps::DBuf<int32_t> slot_brick[2];
+    // per-node fields, slot-major
+    vps::DBuf<float> vrho, vvol, vb, vdiag, vx[2], vr, vp[2], vt, vq;
+    vps::DBuf<double> vdiag64;
+    vps::DBuf<int32_t> vrank;
+    // lattice bitmask for reference (x-fastest) ranks
+    int64_t nwords = 0;
+    vps::DBuf<uint32_t> bits, word_prefix;
+    int64_t nvirt = 0;
+    vps::DBuf<float> p_by_rank[2];
+
+    // scalars
+    double rho0 = 0, kappa = 0, alpha0 = -1e6;
+    bool initialized = false;
+    bool generated = false;
+    bool check_true = true;
+    int prev_iters = 0;
+
+    // scratch
+    vps::DBuf<double> partials;
+    vps::DBuf<vps::CgState> cg;
+    vps::DBuf<uint32_t> scan_tmp;
+    vps::DBuf<int64_t> counters;        // small device counters
+    vps::CgState* host_cg = nullptr;    // pinned
+    int64_t* host_counters = nullptr;   // pinned
+
+    // timing
+    hipEvent_t ev[16] = {};
+    std::vector<hipEvent_t> kev;        // per-launch events (SpMV/update)
+    bool kernel_timing = false;
+
+    // learned warm start (vps_mlp.hip)
+    void* mlp = nullptr;
+};
+
+// ---- error handling -------------------------------------------------------
+#define VPS_HIP(ctx, call)                                                    \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);   \
+            return e_ == hipErrorOutOfMemory ? VPS_ERR_OOM : VPS_ERR_HIP;     \
+        }                                                                     \
+    } while (0)
+
+#define VPS_TRY(expr)           \
+    do {                        \
+        int rc_ = (expr);       \
+        if (rc_) return rc_;    \
+    } while (0)
+
+namespace vps {
+
+template <typename T>
+int dalloc(vps_context* c, DBuf<T>& b, size_t n) {
+    if (b.n >= n && b.p) return VPS_OK;
+    if (b.p) {
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+    }
+    if (n == 0) n = 1;
+    VPS_HIP(c, hipMalloc((void**)&b.p, n * sizeof(T)));
+    b.n = n;
+    return VPS_OK;
+}
+template <typename T>
+void dfree(DBuf<T>& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.n = 0;
+}
+
+// launchers (defined in the .hip files)
+int scan_exclusive_i32(vps_context* c, const int32_t* in, int32_t* out,
+                       int64_t n, int32_t* total_dev);
+int cells_sort_real(vps_context* c);
+int cells_sort_boundary(vps_context* c);
+int real_density(vps_context* c, bool set_volume);
+int boundary_density(vps_context* c);
+int virtual_generate(vps_context* c);
+int virtual_diag(vps_context* c, bool set_alpha0);
+int cg_solve(vps_context* c, vps_stats* st);
+int apply_operator_f32(vps_context* c, const float* in, float* out,
+                       int op_mode);
+int pressure_gradient(vps_context* c);
+int advect(vps_context* c, double dt, double g);
+int query_neighbors(vps_context* c, int which, int64_t nq, const double* q,
+                    int64_t* offsets, int64_t* idx, int64_t cap,
+                    int64_t* needed);
+int reduce_max_f32(vps_context* c, const float* in, int64_t n, double* out);
+int gather_virtual(vps_context* c, int64_t* node, double* rho, double* vol,
+                   double* div, double* diag, double* pressure);
+int warm_start(vps_context* c, int mode, float* x0);
+int after_solve(vps_context* c);
+
+// learned warm start
+int mlp_create(vps_context* c);
+void mlp_destroy(vps_context* c);
+int mlp_warm_start(vps_context* c, float* x0);
+int mlp_after_solve(vps_context* c);
+
+}  // namespace vps

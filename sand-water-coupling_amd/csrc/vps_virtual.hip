@@ -1,0 +1,377 @@
+// vps_virtual.hip -- virtual particle generation and the fused virtual sums.
+//
+// Replaces VirtualParticle::generateParticles / computeDensities /
+// computeVolumes / calculateVelocity / calculateDiv
+// (Cores/Structures/VirtualParticle.cpp:18-62, 136-147, 171-201).
+//
+// The reference evaluates the weight w_I = sum_j W_h(x_I - x_j) at every node
+// of the lattice (6.9M nodes at 1M particles) and keeps the nodes with
+// w_I > 1e-6, in x-fastest node order.  Here:
+//   1. every real particle flags the 8^D-node bricks its h-ball touches;
+//   2. flagged bricks are compacted into slots (ordered by brick id);
+//   3. one workgroup per slot evaluates, per node, in ONE walk over the real
+//      and boundary neighbours within h: w_I (FP32, with an exact FP64 redo
+//      when it lies within 1e-4 relative of the 1e-6 threshold), rho_I,
+//      V_I, the smoothed velocity v_I and the divergence D_I (the CG
+//      right-hand side).  Folding calculateVelocity/calculateDiv into
+//      generation saves a full neighbour walk; project then uses the
+//      velocities as of generation, which is also when the reference reads
+//      them (nothing changes velocities between generateParticles and
+//      project, DualParticleSphLiquid.cpp:118-125);
+//   4. the reference order (x-fastest node rank) comes from a lattice bitmask
+//      and a popcount scan, so every node knows its index in the reference's
+//      virtual list without the list ever being materialised.
+#include "vps_internal.h"
+
+namespace vps {
+namespace {
+
+inline unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <int D>
+__global__ void __launch_bounds__(256) k_mark_bricks(const double4* pos,
+                                                     int64_t n, KConst k,
+                                                     uint32_t* flag) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double4 p = pos[i];
+    const double pc[3] = {p.x, p.y, p.z};
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const double inv = 1.0 / k.spacing;
+        int l = (int)floor((pc[a] - k.h - k.origin[a]) * inv);
+        int u = (int)floor((pc[a] + k.h - k.origin[a]) * inv) + 1;
+        l = l < 0 ? 0 : l;
+        u = u >= k.ndims[a] ? k.ndims[a] - 1 : u;
+        if (l > u) return;  // ball outside the lattice
+        lo[a] = l / kBrick;
+        hi[a] = u / kBrick;
+    }
+    for (int bz = lo[2]; bz <= hi[2]; bz++)
+        for (int by = lo[1]; by <= hi[1]; by++)
+            for (int bx = lo[0]; bx <= hi[0]; bx++)
+                flag[((int64_t)bz * k.bdims[1] + by) * k.bdims[0] + bx] = 1u;
+}
+
+__global__ void __launch_bounds__(256) k_assign_slots(const uint32_t* flag,
+                                                      int64_t nbricks,
+                                                      int32_t* slot_of_brick,
+                                                      int32_t* brick_of_slot) {
+    const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (b >= nbricks) return;
+    // slot_of_brick holds the exclusive prefix of flag on entry
+    const int s = slot_of_brick[b];
+    if (flag[b]) {
+        brick_of_slot[s] = (int32_t)b;
+    } else {
+        slot_of_brick[b] = -1;
+    }
+}
+
+// Fused per-node pass.  One workgroup = one brick; thread = node.
+template <int D, int NB>
+__global__ void __launch_bounds__(NB) k_virtual_sums(
+    KConst k, const int32_t* brick_of_slot, int64_t nslots,
+    const double4* rpos, const float4* rvel, const float* rrho,
+    const int32_t* rcstart, const double4* bpos, const float4* bvel,
+    const float4* bnrm, const int32_t* bcstart, float* vrho, float* vvol,
+    float* vb, uint32_t* bits) {
+    const int slot = xcd_remap(blockIdx.x, (int)nslots);
+    const int brick = brick_of_slot[slot];
+    const int t = threadIdx.x;
+    const int lx = t & 7, ly = (t >> 3) & 7, lz = D == 3 ? t >> 6 : 0;
+    const int bx = brick % k.bdims[0];
+    const int by = (brick / k.bdims[0]) % k.bdims[1];
+    const int bz = D == 3 ? brick / k.bdims[0] / k.bdims[1] : 0;
+    const int cx = bx * kBrick + lx, cy = by * kBrick + ly,
+              cz = bz * kBrick + lz;
+    const int64_t out = (int64_t)slot * NB + t;
+    if (cx >= k.ndims[0] || cy >= k.ndims[1] || (D == 3 && cz >= k.ndims[2])) {
+        vrho[out] = 0.f;
+        vvol[out] = 0.f;
+        vb[out] = 0.f;
+        return;
+    }
+    const double qx = node_coord(k.origin[0], cx, k.spacing);
+    const double qy = node_coord(k.origin[1], cy, k.spacing);
+    const double qz = D == 3 ? node_coord(k.origin[2], cz, k.spacing) : 0.0;
+
+    float w = 0.f, sH = 0.f, sV = 0.f, den = 0.f;
+    float nvx = 0.f, nvy = 0.f, nvz = 0.f;  // sum v_j W / rho_j (real)
+    float gx = 0.f, gy = 0.f, gz = 0.f;     // sum V_j gradK (real)
+    float dr = 0.f;                          // sum V_j v_j . gradK (real)
+    for_each_candidate<D>(k, rcstart, qx, qy, qz, [&](int j) {
+        const double4 pj = rpos[j];
+        const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz)
+                                 : sq_dist2(pj.x, pj.y, qx, qy);
+        if (d2 < k.h2) {
+            const KW kh = kernel_f32(d2, k.h2, k.hf, k.inv_h, k.c0h, k.c1h);
+            const KW kH = kernel_f32(d2, k.H2, k.Hf, k.inv_H, k.c0H, k.c1H);
+            const float4 vj = rvel[j];
+            const float Vj = vj.w;
+            const float wr = kh.w / rrho[j];
+            w += kh.w;
+            sH += kH.w;
+            sV += Vj * Vj * kh.w;
+            den += Vj * kh.w;
+            nvx += vj.x * wr;
+            nvy += vj.y * wr;
+            nvz += vj.z * wr;
+            if (kh.r > 0.f) {
+                // gradK(q - x_j) = -W'(r) (q - x_j) / r
+                const float s = -kh.dw / kh.r;
+                const float ex = (float)(qx - pj.x) * s;
+                const float ey = (float)(qy - pj.y) * s;
+                const float ez = D == 3 ? (float)(qz - pj.z) * s : 0.f;
+                gx += Vj * ex;
+                gy += Vj * ey;
+                gz += Vj * ez;
+                dr += Vj * (vj.x * ex + vj.y * ey + vj.z * ez);
+            }
+        }
+    });
+    bool virt = w > 1e-6f;
+    if (fabsf(w - 1e-6f) <= 1e-10f) {
+        // exact FP64 decision near the threshold (VirtualParticle.cpp:56)
+        double w64 = 0;
+        for_each_candidate<D>(k, rcstart, qx, qy, qz, [&](int j) {
+            const double4 pj = rpos[j];
+            const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz)
+                                     : sq_dist2(pj.x, pj.y, qx, qy);
+            if (d2 < k.h2) w64 += kernel_w64(d2, k.inv_h64, k.c0h64);
+        });
+        virt = w64 > 1e-6;
+    }
+    if (!virt) {
+        vrho[out] = 0.f;
+        vvol[out] = 0.f;
+        vb[out] = 0.f;
+        return;
+    }
+    // boundary neighbours within h
+    float bnx = 0.f, bny = 0.f, bnz = 0.f;
+    float hx = 0.f, hy = 0.f, hz = 0.f, db = 0.f;
+    for_each_candidate<D>(k, bcstart, qx, qy, qz, [&](int j) {
+        const double4 pj = bpos[j];
+        const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz)
+                                 : sq_dist2(pj.x, pj.y, qx, qy);
+        if (d2 < k.h2) {
+            const KW kh = kernel_f32(d2, k.h2, k.hf, k.inv_h, k.c0h, k.c1h);
+            const float4 vj = bvel[j];
+            const float Vj = vj.w;
+            const float wr = kh.w / bnrm[j].w;
+            den += Vj * kh.w;
+            bnx += vj.x * wr;
+            bny += vj.y * wr;
+            bnz += vj.z * wr;
+            if (kh.r > 0.f) {
+                const float s = -kh.dw / kh.r;
+                const float ex = (float)(qx - pj.x) * s;
+                const float ey = (float)(qy - pj.y) * s;
+                const float ez = D == 3 ? (float)(qz - pj.z) * s : 0.f;
+                hx += Vj * ex;
+                hy += Vj * ey;
+                hz += Vj * ez;
+                db += Vj * (vj.x * ex + vj.y * ey + vj.z * ez);
+            }
+        }
+    });
+    // v_I = (m sum v_j W/rho_j + m_b sum v_b W/rho_b) / (sum V W + sum V_b W + 1e-6)
+    const float inv_den = 1.f / (den + 1e-6f);
+    const float ux = (k.mass * nvx + k.mass * bnx) * inv_den;
+    const float uy = (k.mass * nvy + k.mass * bny) * inv_den;
+    const float uz = (k.mass * nvz + k.mass * bnz) * inv_den;
+    const float rho = k.mass * sH;
+    const float drho = rho - k.rho0;
+    const float D_I = dr - (ux * gx + uy * gy + uz * gz) + db -
+                      (ux * hx + uy * hy + uz * hz) -
+                      k.kappa * (drho > 0.f ? drho : 0.f) / k.rho0;
+    vrho[out] = rho;
+    vvol[out] = sV;
+    vb[out] = D_I;
+    const int64_t lin = ((int64_t)cz * k.ndims[1] + cy) * k.ndims[0] + cx;
+    atomicOr(&bits[lin >> 5], 1u << (lin & 31));
+}
+
+__global__ void __launch_bounds__(256) k_popc(const uint32_t* bits,
+                                              int64_t nwords, int32_t* cnt) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < nwords) cnt[i] = __popc(bits[i]);
+}
+
+template <int D, int NB>
+__global__ void __launch_bounds__(NB) k_node_rank(KConst k,
+                                                  const int32_t* brick_of_slot,
+                                                  int64_t nslots,
+                                                  const float* vvol,
+                                                  const uint32_t* bits,
+                                                  const int32_t* prefix,
+                                                  int32_t* rank) {
+    const int slot = blockIdx.x;
+    const int64_t out = (int64_t)slot * NB + threadIdx.x;
+    if (!(vvol[out] > 0.f)) {
+        rank[out] = -1;
+        return;
+    }
+    const int brick = brick_of_slot[slot];
+    const int t = threadIdx.x;
+    const int bx = brick % k.bdims[0];
+    const int by = (brick / k.bdims[0]) % k.bdims[1];
+    const int bz = D == 3 ? brick / k.bdims[0] / k.bdims[1] : 0;
+    const int cx = bx * kBrick + (t & 7), cy = by * kBrick + ((t >> 3) & 7),
+              cz = D == 3 ? bz * kBrick + (t >> 6) : 0;
+    const int64_t lin = ((int64_t)cz * k.ndims[1] + cy) * k.ndims[0] + cx;
+    const uint32_t word = bits[lin >> 5];
+    const uint32_t below = word & ((1u << (lin & 31)) - 1u);
+    rank[out] = prefix[lin >> 5] + __popc(below);
+}
+
+// CG initial guess for every node of the active bricks (0 off the set).
+template <int D, int NB>
+__global__ void __launch_bounds__(NB) k_warm_start(
+    int mode, KConst k, const int32_t* brick_of_slot, const float* vvol,
+    const int32_t* rank, const float* p_rank_prev, int64_t nprev,
+    const int32_t* prev_slot_of_brick, const float* x_prev, float* x0) {
+    const int slot = blockIdx.x;
+    const int64_t out = (int64_t)slot * NB + threadIdx.x;
+    float v = 0.f;
+    if (vvol[out] > 0.f) {
+        if (mode == VPS_WARM_INDEX_CARRY) {
+            const int r = rank[out];
+            if (r < nprev) v = p_rank_prev[r];
+        } else if (mode == VPS_WARM_NODE_KEYED || mode == VPS_WARM_LEARNED) {
+            if (prev_slot_of_brick) {
+                const int ps = prev_slot_of_brick[brick_of_slot[slot]];
+                if (ps >= 0) v = x_prev[(int64_t)ps * NB + threadIdx.x];
+            }
+        }
+    }
+    x0[out] = v;
+}
+
+// After the solve: pressures by reference rank (index-carry warm start of
+// the next step) and q = V p for the pressure gradient.
+__global__ void __launch_bounds__(256) k_after_solve(int64_t nnode,
+                                                     const float* vvol,
+                                                     const float* x,
+                                                     const int32_t* rank,
+                                                     float* p_rank, float* q) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nnode) return;
+    const float V = vvol[i];
+    const float p = x[i];
+    q[i] = V * p;
+    if (V > 0.f) p_rank[rank[i]] = p;
+}
+
+}  // namespace
+
+int virtual_generate(vps_context* c) {
+    // The new slot map goes into the buffer that does NOT hold the last solved
+    // pressures, so both warm starts can still read them.
+    const int w = c->last_solve_buf >= 0 ? 1 - c->last_solve_buf : 0;
+    const int D = c->dim;
+    VPS_HIP(c, hipMemsetAsync(c->brick_flag.p, 0, (size_t)c->nbricks * 4,
+                              c->stream));
+    const int s = c->cur;
+    if (D == 3)
+        hipLaunchKernelGGL(k_mark_bricks<3>, dim3(blocks(c->n, 256)), dim3(256),
+                           0, c->stream, c->rpos[s].p, c->n, c->k,
+                           c->brick_flag.p);
+    else
+        hipLaunchKernelGGL(k_mark_bricks<2>, dim3(blocks(c->n, 256)), dim3(256),
+                           0, c->stream, c->rpos[s].p, c->n, c->k,
+                           c->brick_flag.p);
+    int64_t* dcnt = c->counters.p;
+    VPS_TRY(scan_exclusive_i32(c, (const int32_t*)c->brick_flag.p,
+                               c->brick_slot[w].p, c->nbricks,
+                               (int32_t*)dcnt));
+    hipLaunchKernelGGL(k_assign_slots, dim3(blocks(c->nbricks, 256)), dim3(256),
+                       0, c->stream, c->brick_flag.p, c->nbricks,
+                       c->brick_slot[w].p, c->slot_brick[w].p);
+    VPS_HIP(c, hipMemcpyAsync(c->host_counters, dcnt, 4, hipMemcpyDeviceToHost,
+                              c->stream));
+    VPS_HIP(c, hipStreamSynchronize(c->stream));
+    const int64_t nslots = *(int32_t*)c->host_counters;
+    c->nslots = nslots;
+    c->which = w;
+    VPS_HIP(c, hipMemsetAsync(c->bits.p, 0, (size_t)c->nwords * 4, c->stream));
+    if (nslots == 0) {
+        c->nvirt = 0;
+        return VPS_OK;
+    }
+    const int rs = c->cur;
+    if (D == 3)
+        hipLaunchKernelGGL((k_virtual_sums<3, 512>), dim3((unsigned)nslots),
+                           dim3(512), 0, c->stream, c->k, c->slot_brick[w].p,
+                           nslots, c->rpos[rs].p, c->rvel[rs].p, c->rrho.p,
+                           c->cell_start.p, c->bpos.p, c->bvel.p, c->bnrm.p,
+                           c->bcell_start.p, c->vrho.p, c->vvol.p, c->vb.p,
+                           c->bits.p);
+    else
+        hipLaunchKernelGGL((k_virtual_sums<2, 64>), dim3((unsigned)nslots),
+                           dim3(64), 0, c->stream, c->k, c->slot_brick[w].p,
+                           nslots, c->rpos[rs].p, c->rvel[rs].p, c->rrho.p,
+                           c->cell_start.p, c->bpos.p, c->bvel.p, c->bnrm.p,
+                           c->bcell_start.p, c->vrho.p, c->vvol.p, c->vb.p,
+                           c->bits.p);
+    hipLaunchKernelGGL(k_popc, dim3(blocks(c->nwords, 256)), dim3(256), 0,
+                       c->stream, c->bits.p, c->nwords,
+                       (int32_t*)c->word_prefix.p);
+    VPS_TRY(scan_exclusive_i32(c, (const int32_t*)c->word_prefix.p,
+                               (int32_t*)c->word_prefix.p, c->nwords,
+                               (int32_t*)(dcnt + 1)));
+    if (D == 3)
+        hipLaunchKernelGGL((k_node_rank<3, 512>), dim3((unsigned)nslots),
+                           dim3(512), 0, c->stream, c->k, c->slot_brick[w].p,
+                           nslots, c->vvol.p, c->bits.p,
+                           (const int32_t*)c->word_prefix.p, c->vrank.p);
+    else
+        hipLaunchKernelGGL((k_node_rank<2, 64>), dim3((unsigned)nslots),
+                           dim3(64), 0, c->stream, c->k, c->slot_brick[w].p,
+                           nslots, c->vvol.p, c->bits.p,
+                           (const int32_t*)c->word_prefix.p, c->vrank.p);
+    VPS_HIP(c, hipMemcpyAsync(c->host_counters + 1, dcnt + 1, 4,
+                              hipMemcpyDeviceToHost, c->stream));
+    VPS_HIP(c, hipStreamSynchronize(c->stream));
+    c->nvirt = *(int32_t*)(c->host_counters + 1);
+    VPS_HIP(c, hipGetLastError());
+    return VPS_OK;
+}
+
+int warm_start(vps_context* c, int mode, float* x0) {
+    const int w = c->which;
+    const int lb = c->last_solve_buf;
+    const int32_t* prev_map = lb >= 0 ? c->brick_slot[lb].p : nullptr;
+    const float* p_prev = lb >= 0 ? c->p_by_rank[lb].p : nullptr;
+    const int64_t nprev = lb >= 0 ? c->last_solve_nvirt : 0;
+    const float* x_prev = lb >= 0 ? c->vx[lb].p : nullptr;
+    if (c->dim == 3)
+        hipLaunchKernelGGL((k_warm_start<3, 512>), dim3((unsigned)c->nslots),
+                           dim3(512), 0, c->stream, mode, c->k,
+                           c->slot_brick[w].p, c->vvol.p, c->vrank.p, p_prev,
+                           nprev, prev_map, x_prev, x0);
+    else
+        hipLaunchKernelGGL((k_warm_start<2, 64>), dim3((unsigned)c->nslots),
+                           dim3(64), 0, c->stream, mode, c->k,
+                           c->slot_brick[w].p, c->vvol.p, c->vrank.p, p_prev,
+                           nprev, prev_map, x_prev, x0);
+    VPS_HIP(c, hipGetLastError());
+    return VPS_OK;
+}
+
+int after_solve(vps_context* c) {
+    const int w = c->which;
+    const int64_t nn = c->nslots * c->brick_nodes;
+    if (nn == 0) return VPS_OK;
+    hipLaunchKernelGGL(k_after_solve, dim3(blocks(nn, 256)), dim3(256), 0,
+                       c->stream, nn, c->vvol.p, c->vx[w].p, c->vrank.p,
+                       c->p_by_rank[w].p, c->vq.p);
+    VPS_HIP(c, hipGetLastError());
+    c->last_solve_buf = w;
+    c->last_solve_nvirt = c->nvirt;
+    return VPS_OK;
+}
+
+}  // namespace vps

@@ -1,0 +1,192 @@
+"""ctypes mirror of include/vps.h (the C ABI of libvps).
+
+The struct layouts here must match the header field for field; the library
+exports ``vps_sizeof_params`` / ``vps_sizeof_stats`` and the binding checks
+them when it loads.
+"""
+import ctypes as C
+import os
+
+VPS_OK = 0
+VPS_ERR_INVALID_ARG = 1
+VPS_ERR_NO_CONVERGENCE = 2
+VPS_ERR_NONFINITE = 3
+VPS_ERR_OOM = 4
+VPS_ERR_HIP = 5
+VPS_ERR_STATE = 6
+
+STATUS_NAMES = {
+    0: "VPS_OK",
+    1: "VPS_ERR_INVALID_ARG",
+    2: "VPS_ERR_NO_CONVERGENCE",
+    3: "VPS_ERR_NONFINITE",
+    4: "VPS_ERR_OOM",
+    5: "VPS_ERR_HIP",
+    6: "VPS_ERR_STATE",
+}
+
+OP_TRANSPOSE = 0
+OP_ROW = 1
+
+WARM_INDEX_CARRY = 0
+WARM_ZERO = 1
+WARM_NODE_KEYED = 2
+WARM_LEARNED = 3
+
+FROM_INIT = 0
+FROM_PARAMS = 1
+
+ABI_VERSION = 1
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("op_mode", C.c_int32),
+        ("warm_start", C.c_int32),
+        ("rest_density_mode", C.c_int32),
+        ("kappa_mode", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("particle_radius", C.c_double),
+        ("lattice_spacing", C.c_double),
+        ("lattice_origin", C.c_double * 3),
+        ("lattice_dims", C.c_int32 * 3),
+        ("reserved1", C.c_int32),
+        ("mass", C.c_double),
+        ("rest_density", C.c_double),
+        ("kappa", C.c_double),
+        ("tolerance", C.c_double),
+        ("max_iterations", C.c_int64),
+        ("domain_lo", C.c_double * 3),
+        ("domain_hi", C.c_double * 3),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("n_real", C.c_int64),
+        ("n_boundary", C.c_int64),
+        ("n_virtual", C.c_int64),
+        ("n_active_bricks", C.c_int64),
+        ("cg_iterations", C.c_int64),
+        ("cg_rel_residual", C.c_double),
+        ("cg_true_rel_residual", C.c_double),
+        ("rhs_norm", C.c_double),
+        ("alpha0", C.c_double),
+        ("kappa", C.c_double),
+        ("rest_density", C.c_double),
+        ("ms_sort", C.c_double),
+        ("ms_generate", C.c_double),
+        ("ms_diag", C.c_double),
+        ("ms_cg", C.c_double),
+        ("ms_gradient", C.c_double),
+        ("ms_total", C.c_double),
+        ("ms_spmv_avg", C.c_double),
+        ("ms_update_avg", C.c_double),
+        ("spmv_launches", C.c_int64),
+        ("reserved", C.c_int64 * 4),
+    ]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            if name == "reserved":
+                continue
+            out[name] = getattr(self, name)
+        return out
+
+
+def make_params(scene, **overrides):
+    """Builds a Params block from a scene dict (see scenes.py)."""
+    p = Params()
+    p.dim = scene["dim"]
+    p.op_mode = overrides.pop("op_mode", OP_TRANSPOSE)
+    p.warm_start = overrides.pop("warm_start", WARM_INDEX_CARRY)
+    p.rest_density_mode = overrides.pop("rest_density_mode", scene.get("rest_density_mode", FROM_PARAMS))
+    p.kappa_mode = overrides.pop("kappa_mode", scene.get("kappa_mode", FROM_PARAMS))
+    p.particle_radius = scene["particle_radius"]
+    p.lattice_spacing = scene["lattice_spacing"]
+    for a in range(3):
+        p.lattice_origin[a] = scene["lattice_origin"][a]
+        p.lattice_dims[a] = int(scene["lattice_dims"][a])
+        p.domain_lo[a] = scene["domain_lo"][a]
+        p.domain_hi[a] = scene["domain_hi"][a]
+    p.mass = scene["mass"]
+    p.rest_density = scene["rest_density"]
+    p.kappa = scene["kappa"]
+    p.tolerance = overrides.pop("tolerance", 1e-6)
+    p.max_iterations = overrides.pop("max_iterations", -1)
+    if overrides:
+        raise TypeError("unknown params: %s" % sorted(overrides))
+    return p
+
+
+def _proto(lib, name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
+_I64 = C.POINTER(C.c_int64)
+
+
+def declare(lib):
+    """Declares every exported symbol of libvps on a CDLL handle."""
+    _proto(lib, "vps_abi_version", C.c_int32, [])
+    _proto(lib, "vps_sizeof_params", C.c_int64, [])
+    _proto(lib, "vps_sizeof_stats", C.c_int64, [])
+    _proto(lib, "vps_create", C.c_int, [C.POINTER(Params), C.POINTER(_P)])
+    _proto(lib, "vps_destroy", None, [_P])
+    _proto(lib, "vps_last_error", C.c_char_p, [_P])
+    _proto(lib, "vps_set_boundary", C.c_int, [_P, C.c_int64, _D, _D, _D])
+    _proto(lib, "vps_refresh_boundary", C.c_int, [_P])
+    _proto(lib, "vps_set_particles", C.c_int, [_P, C.c_int64, _D, _D])
+    _proto(lib, "vps_get_particles", C.c_int, [_P, _D, _D])
+    _proto(lib, "vps_initialize", C.c_int, [_P, C.POINTER(Stats)])
+    _proto(lib, "vps_refresh_real", C.c_int, [_P])
+    _proto(lib, "vps_generate_particles", C.c_int, [_P])
+    _proto(lib, "vps_project", C.c_int, [_P, C.POINTER(Stats)])
+    _proto(lib, "vps_projection_step", C.c_int, [_P, C.POINTER(Stats)])
+    _proto(lib, "vps_advect", C.c_int, [_P, C.c_double, C.c_double])
+    _proto(lib, "vps_set_check_true_residual", C.c_int, [_P, C.c_int32])
+    _proto(lib, "vps_synchronize", C.c_int, [_P])
+    _proto(lib, "vps_get_virtual_count", C.c_int, [_P, _I64])
+    _proto(lib, "vps_get_virtual", C.c_int, [_P, _I64, _D, _D, _D, _D, _D])
+    _proto(lib, "vps_get_real_fields", C.c_int, [_P, _D, _D])
+    _proto(lib, "vps_get_boundary_fields", C.c_int, [_P, _D, _D])
+    _proto(lib, "vps_get_scalars", C.c_int, [_P, _D, _D, _D])
+    _proto(lib, "vps_query_neighbors", C.c_int,
+           [_P, C.c_int32, C.c_int64, _D, _I64, _I64, C.c_int64, _I64])
+    _proto(lib, "vps_mlp_set_training", C.c_int,
+           [_P, C.c_int32, C.c_int32, C.c_double, C.c_uint64])
+    _proto(lib, "vps_mlp_parameter_count", C.c_int64, [])
+    _proto(lib, "vps_mlp_get_parameters", C.c_int, [_P, _F])
+    _proto(lib, "vps_mlp_set_parameters", C.c_int, [_P, _F])
+    _proto(lib, "vps_mlp_loss_and_grad", C.c_int, [_P, C.c_int64, _F, _F, _D, _F])
+    _proto(lib, "vps_mlp_forward", C.c_int, [_P, C.c_int64, _F, _F])
+    _proto(lib, "vps_mlp_get_features", C.c_int, [_P, _F])
+    _proto(lib, "vps_debug_apply_operator", C.c_int, [_P, _D, _D])
+    return lib
+
+
+# Every symbol include/vps.h declares (checked by the CPU test suite).
+EXPORTED = [
+    "vps_abi_version", "vps_sizeof_params", "vps_sizeof_stats", "vps_create",
+    "vps_destroy", "vps_last_error", "vps_set_boundary", "vps_refresh_boundary",
+    "vps_set_particles", "vps_get_particles", "vps_initialize",
+    "vps_refresh_real", "vps_generate_particles", "vps_project",
+    "vps_projection_step", "vps_advect", "vps_set_check_true_residual",
+    "vps_synchronize", "vps_get_virtual_count", "vps_get_virtual",
+    "vps_get_real_fields", "vps_get_boundary_fields", "vps_get_scalars",
+    "vps_query_neighbors", "vps_mlp_set_training", "vps_mlp_parameter_count",
+    "vps_mlp_get_parameters", "vps_mlp_set_parameters",
+    "vps_mlp_loss_and_grad", "vps_mlp_forward", "vps_mlp_get_features",
+    "vps_debug_apply_operator",
+]
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libvps.so")

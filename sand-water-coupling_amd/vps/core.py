@@ -1,0 +1,203 @@
+"""ctypes wrapper around libvps (see include/vps.h)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+
+_lib = None
+
+
+class VpsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (_abi.STATUS_NAMES.get(code, code), msg))
+        self.code = code
+
+
+def load_library(path=None):
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or _abi.LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError(
+            "libvps.so not found at %s: build it with __graft_entry__.build() "
+            "(make -C sand-water-coupling_amd/csrc)" % path)
+    lib = C.CDLL(path)
+    _abi.declare(lib)
+    lib.vps_set_kernel_timing.restype = C.c_int
+    lib.vps_set_kernel_timing.argtypes = [C.c_void_p, C.c_int32]
+    if lib.vps_sizeof_params() != C.sizeof(_abi.Params):
+        raise RuntimeError("vps_params layout mismatch")
+    if lib.vps_sizeof_stats() != C.sizeof(_abi.Stats):
+        raise RuntimeError("vps_stats layout mismatch")
+    _lib = lib
+    return lib
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Projection:
+    """One projection context on the current GPU.
+
+    Call order mirrors the reference: ``initialize()`` once
+    (DualParticleSphLiquid::initialize), then per step ``refresh_real()``,
+    ``generate_particles()`` and ``project()`` -- or ``projection_step()``
+    for all three.
+    """
+
+    def __init__(self, scene, **overrides):
+        self.L = load_library()
+        self.dim = scene["dim"]
+        self.scene = scene
+        self.params = _abi.make_params(scene, **overrides)
+        h = C.c_void_p()
+        rc = self.L.vps_create(C.byref(self.params), C.byref(h))
+        if rc:
+            raise VpsError(rc, self.L.vps_last_error(None).decode())
+        self.h = h
+        bx = np.ascontiguousarray(scene["bx"], dtype=np.float64)
+        bn = np.ascontiguousarray(scene["bn"], dtype=np.float64)
+        bv = np.ascontiguousarray(scene["bv"], dtype=np.float64)
+        self._check(self.L.vps_set_boundary(self.h, len(bx), _d(bx), _d(bn), _d(bv)))
+        self.nb = len(bx)
+        self.set_particles(scene["x"], scene["v"])
+
+    # -- plumbing
+    def _check(self, rc):
+        if rc:
+            raise VpsError(rc, self.L.vps_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.vps_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- state
+    def set_particles(self, x, v=None):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        self.n = len(x)
+        if v is None:
+            self._check(self.L.vps_set_particles(self.h, self.n, _d(x), None))
+        else:
+            v = np.ascontiguousarray(v, dtype=np.float64)
+            self._check(self.L.vps_set_particles(self.h, self.n, _d(x), _d(v)))
+
+    def particles(self):
+        x = np.zeros((self.n, self.dim))
+        v = np.zeros((self.n, self.dim))
+        self._check(self.L.vps_get_particles(self.h, _d(x), _d(v)))
+        return x, v
+
+    # -- path
+    def initialize(self):
+        st = _abi.Stats()
+        self._check(self.L.vps_initialize(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def refresh_real(self):
+        self._check(self.L.vps_refresh_real(self.h))
+
+    def generate_particles(self):
+        self._check(self.L.vps_generate_particles(self.h))
+
+    def project(self, allow_fail=False):
+        st = _abi.Stats()
+        rc = self.L.vps_project(self.h, C.byref(st))
+        if not allow_fail:
+            self._check(rc)
+        d = st.as_dict()
+        d["status"] = rc
+        return d
+
+    def projection_step(self):
+        st = _abi.Stats()
+        self._check(self.L.vps_projection_step(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def advect(self, dt, gravity):
+        self._check(self.L.vps_advect(self.h, dt, gravity))
+
+    def synchronize(self):
+        self._check(self.L.vps_synchronize(self.h))
+
+    def set_check_true_residual(self, on):
+        self._check(self.L.vps_set_check_true_residual(self.h, 1 if on else 0))
+
+    def set_kernel_timing(self, on):
+        self._check(self.L.vps_set_kernel_timing(self.h, 1 if on else 0))
+
+    # -- read-back
+    def virtual_count(self):
+        n = C.c_int64()
+        self._check(self.L.vps_get_virtual_count(self.h, C.byref(n)))
+        return n.value
+
+    def virtual(self):
+        n = self.virtual_count()
+        out = {k: np.zeros(n) for k in ("rho", "vol", "div", "diag", "p")}
+        out["node"] = np.zeros(n, dtype=np.int64)
+        self._check(self.L.vps_get_virtual(
+            self.h, _i64(out["node"]), _d(out["rho"]), _d(out["vol"]),
+            _d(out["div"]), _d(out["diag"]), _d(out["p"])))
+        return out
+
+    def real_fields(self):
+        rho = np.zeros(self.n)
+        vol = np.zeros(self.n)
+        self._check(self.L.vps_get_real_fields(self.h, _d(rho), _d(vol)))
+        return rho, vol
+
+    def boundary_fields(self):
+        rho = np.zeros(self.nb)
+        vol = np.zeros(self.nb)
+        self._check(self.L.vps_get_boundary_fields(self.h, _d(rho), _d(vol)))
+        return rho, vol
+
+    def scalars(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        self._check(self.L.vps_get_scalars(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"rest_density": a.value, "kappa": b.value, "alpha0": c.value}
+
+    def query(self, which, q):
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, self.dim)
+        nq = len(q)
+        offs = np.zeros(nq + 1, dtype=np.int64)
+        need = C.c_int64(0)
+        self._check(self.L.vps_query_neighbors(self.h, which, nq, _d(q), _i64(offs),
+                                               None, 0, C.byref(need)))
+        idx = np.zeros(max(need.value, 1), dtype=np.int64)
+        self._check(self.L.vps_query_neighbors(self.h, which, nq, _d(q), _i64(offs),
+                                               _i64(idx), len(idx), C.byref(need)))
+        return offs, idx[: need.value]
+
+    def apply_operator(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        out = np.zeros_like(p)
+        self._check(self.L.vps_debug_apply_operator(self.h, _d(p), _d(out)))
+        return out

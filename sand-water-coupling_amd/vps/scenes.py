@@ -1,0 +1,152 @@
+"""Seeded synthetic scenes for the projection path (dam breaks in 2D and 3D).
+
+The reference builds its scenes in C++ builders with non-deterministic
+samplers (Cores/Utilities/PoissonDiskSampler.h:28-32) and a hard-coded
+boundary shell (Cores/Physics/DualParticleSphLiquid.cpp:79-82).  Here every
+scene is a pure function of its parameters and a seed:
+
+* fluid particles on a lattice of spacing dx, cell-centred ((i + 0.5) dx),
+  each coordinate jittered by a uniform value in [-0.1 dx, 0.1 dx];
+* solid walls: the same cell-centred lattice, ``wall_layers`` layers thick,
+  outside the box, zero velocity;
+* wall normals: the normal of the complement of the box, as
+  ComplementarySurface<ImplicitBox>::closestNormal computes it
+  (Cores/Geometries/Surface.h:65, ImplicitSurface.h:56-66), pointing into the
+  fluid;
+* the virtual-particle lattice is the node grid of
+  StaggeredGrid(boundaryWidth, spacing = dx, resolution = box / dx,
+  center = box centre) (Cores/Structures/StaggeredGrid.cpp:20-34), with the
+  boundary width 2 the reference's SPH builder uses
+  (Demos/SmthPartHydrodLiquidTest/SmthPartHydrodLiquidBuilder.h:32-33);
+* particle radius r = dx / 2, so h = 4 r = 2 dx and H = 4 dx;
+* mass m = rho0 dx^dim, kappa = m / rho0, rho0 = 1000.
+"""
+import numpy as np
+
+DEFAULT_SEED = 20261202
+
+
+def staggered_node_grid(box_lo, box_hi, dx, dim, boundary_width=2):
+    """Node grid of StaggeredGrid<Dim>(bw, dx, resolution, center)."""
+    box_lo = np.asarray(box_lo, dtype=np.float64)
+    box_hi = np.asarray(box_hi, dtype=np.float64)
+    res = np.rint((box_hi - box_lo) / dx).astype(np.int64)
+    center = (box_lo + box_hi) / 2
+    full = res + 2 * boundary_width
+    origin = np.zeros(3)
+    dims = np.ones(3, dtype=np.int64)
+    for a in range(dim):
+        # _origin = center - _resolution.cast<real>() * _spacing / 2
+        origin[a] = center[a] - float(full[a]) * dx / 2
+        dims[a] = full[a] + 1
+    return origin, dims
+
+
+def box_complement_normals(x, box_lo, box_hi):
+    """-ImplicitBox::closestNormal for points outside the box."""
+    dim = x.shape[1]
+    c = (np.asarray(box_lo[:dim]) + np.asarray(box_hi[:dim])) / 2
+    half = (np.asarray(box_hi[:dim]) - np.asarray(box_lo[:dim])) / 2
+    rel = x - c
+    phi = np.abs(rel) - half
+    inside = np.all(phi <= 0, axis=1)
+    n = np.where(phi > 0, phi, 0.0)
+    if np.any(inside):
+        ax = np.argmax(phi[inside], axis=1)
+        e = np.zeros((inside.sum(), dim))
+        e[np.arange(inside.sum()), ax] = 1
+        n[inside] = e
+    n = n * np.sign(rel)
+    norm = np.linalg.norm(n, axis=1, keepdims=True)
+    norm[norm == 0] = 1
+    return -(n / norm)
+
+
+def dam_break(dim, dx, box, block, seed=DEFAULT_SEED, wall_layers=3,
+              jitter=0.1, rho0=1000.0, boundary_width=2, name=None):
+    """Fluid block [0, block] in the corner of the box [0, box]."""
+    box = np.asarray(box, dtype=np.float64)[:dim]
+    block = np.asarray(block, dtype=np.float64)[:dim]
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nb = np.rint(block / dx).astype(np.int64)
+    axes = [(np.arange(nb[a]) + 0.5) * dx for a in range(dim)]
+    grid = np.meshgrid(*axes, indexing="ij")
+    # order x fastest for locality (the reference adds particles in its
+    # builders' order; any fixed order is fine)
+    x = np.stack([g.transpose().reshape(-1) for g in grid], axis=1)
+    x = x + rng.uniform(-jitter * dx, jitter * dx, size=x.shape)
+    v = np.zeros_like(x)
+    # walls: cell-centred lattice in [-L dx, box + L dx] minus the box
+    nbox = np.rint(box / dx).astype(np.int64)
+    waxes = [(np.arange(-wall_layers, nbox[a] + wall_layers) + 0.5) * dx
+             for a in range(dim)]
+    wgrid = np.meshgrid(*waxes, indexing="ij")
+    w = np.stack([g.transpose().reshape(-1) for g in wgrid], axis=1)
+    outside = np.any((w < 0) | (w > box), axis=1)
+    w = w[outside]
+    wn = box_complement_normals(w, np.zeros(dim), box)
+    wv = np.zeros_like(w)
+    origin, dims = staggered_node_grid(np.zeros(dim), box, dx, dim,
+                                       boundary_width)
+    mass = rho0 * dx ** dim
+    h = 2 * dx
+    lo = np.zeros(3)
+    hi = np.zeros(3)
+    lo[:dim] = -(wall_layers + 1) * dx - h
+    hi[:dim] = box + (wall_layers + 1) * dx + h
+    return {
+        "name": name or "dam_break_%dd" % dim,
+        "dim": dim,
+        "dx": dx,
+        "x": np.ascontiguousarray(x),
+        "v": np.ascontiguousarray(v),
+        "bx": np.ascontiguousarray(w),
+        "bn": np.ascontiguousarray(wn),
+        "bv": np.ascontiguousarray(wv),
+        "particle_radius": dx / 2,
+        "lattice_spacing": dx,
+        "lattice_origin": origin,
+        "lattice_dims": dims,
+        "mass": mass,
+        "rest_density": rho0,
+        "kappa": mass / rho0,
+        "domain_lo": lo,
+        "domain_hi": hi,
+        "seed": seed,
+    }
+
+
+def config1(seed=DEFAULT_SEED):
+    """2D dam break: 100 x 100 particles at dx = 0.005 in the 0.5 x 0.5
+    lower-left block of a unit box, 3 wall layers."""
+    return dam_break(2, 0.005, (1.0, 1.0), (0.5, 0.5), seed, name="cfg1_2d_10k")
+
+
+def config2(seed=DEFAULT_SEED, dx=1.0 / 150):
+    """3D dam break: a 1 x 0.6 x 0.5 block in a 2 x 1 x 1 box.  dx = 1/150
+    gives 150 x 90 x 75 = 1,012,500 particles (dx = 0.01 gives 300,000)."""
+    return dam_break(3, dx, (2.0, 1.0, 1.0), (1.0, 0.6, 0.5), seed,
+                     name="cfg2_3d_%dk" % round(
+                         np.prod(np.rint(np.array([1.0, 0.6, 0.5]) / dx)) / 1000))
+
+
+def config3(seed=DEFAULT_SEED, dx=1.0 / 300):
+    """Config 2's scene at dx = 1/300: 8,100,000 particles."""
+    return config2(seed, dx)
+
+
+def small_2d(n=20, seed=DEFAULT_SEED, jitter=0.1):
+    """Small 2D dam break for oracle parity (n x n fluid particles)."""
+    dx = 0.5 / n
+    return dam_break(2, dx, (1.0, 1.0), (0.5, 0.5), seed, jitter=jitter,
+                     name="small_2d_%d" % (n * n))
+
+
+def small_3d(nx=12, ny=8, nz=6, seed=DEFAULT_SEED, jitter=0.1):
+    """Small 3D dam break: nx x ny x nz fluid particles at dx = 1/(2 nx) in a
+    box twice the block's x length."""
+    dx = 1.0 / (2 * nx)
+    block = (nx * dx, ny * dx, nz * dx)
+    box = (2 * nx * dx, (ny + 4) * dx, (nz + 4) * dx)
+    return dam_break(3, dx, box, block, seed, jitter=jitter,
+                     name="small_3d_%d" % (nx * ny * nz))
