@@ -267,7 +267,9 @@ int vps_create(const vps_params* params, vps_context** out) {
         delete c;
         return fail(nullptr, VPS_ERR_INVALID_ARG, "domain or lattice too large");
     }
-    const size_t cap = (size_t)c->nbricks * c->brick_nodes;
+    // every per-node array has one extra all-zero brick at index nbricks,
+    // read by the stencil in place of absent neighbour bricks
+    const size_t cap = (size_t)(c->nbricks + 1) * c->brick_nodes;
     int rc = VPS_OK;
 #define ALLOC(buf, n)                                   \
     if (rc == VPS_OK) rc = dalloc(c, buf, (size_t)(n));
@@ -292,7 +294,7 @@ int vps_create(const vps_params* params, vps_context** out) {
     ALLOC(c->vrank, cap);
     ALLOC(c->bits, c->nwords + 1);
     ALLOC(c->word_prefix, c->nwords + 1);
-    ALLOC(c->partials, 2 * (cap / 256 + c->nbricks) + 4096);
+    ALLOC(c->partials, partials_size(c->nbricks));
     ALLOC(c->cg, 1);
     ALLOC(c->counters, 64);
 #undef ALLOC
@@ -314,6 +316,17 @@ int vps_create(const vps_params* params, vps_context** out) {
         if (hipMemset(c->cg.p, 0, sizeof(CgState)) != hipSuccess ||
             hipMemset(c->counters.p, 0, 64 * sizeof(int64_t)) != hipSuccess ||
             hipMemset(c->vq.p, 0, cap * 4) != hipSuccess ||
+            hipMemset(c->vrho.p, 0, cap * 4) != hipSuccess ||
+            hipMemset(c->vvol.p, 0, cap * 4) != hipSuccess ||
+            hipMemset(c->vb.p, 0, cap * 4) != hipSuccess ||
+            hipMemset(c->vdiag.p, 0, cap * 4) != hipSuccess ||
+            hipMemset(c->vdiag64.p, 0, cap * 8) != hipSuccess ||
+            hipMemset(c->vr.p, 0, cap * 4) != hipSuccess ||
+            hipMemset(c->vt.p, 0, cap * 4) != hipSuccess ||
+            hipMemset(c->vx[0].p, 0, cap * 4) != hipSuccess ||
+            hipMemset(c->vx[1].p, 0, cap * 4) != hipSuccess ||
+            hipMemset(c->vp[0].p, 0, cap * 4) != hipSuccess ||
+            hipMemset(c->vp[1].p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->brick_slot[0].p, 0xff, (size_t)c->nbricks * 4) != hipSuccess ||
             hipMemset(c->brick_slot[1].p, 0xff, (size_t)c->nbricks * 4) != hipSuccess)
             rc = VPS_ERR_HIP;
@@ -727,9 +740,12 @@ int vps_debug_apply_operator(vps_context* c, const double* p, double* out) {
     std::vector<float> in((size_t)nn, 0.f), res((size_t)nn);
     for (int64_t g = 0; g < nn; g++)
         if (rank[(size_t)g] >= 0) in[(size_t)g] = (float)p[rank[(size_t)g]];
+    // input needs the zero brick at index nbricks, like every node array
+    const size_t cap = (size_t)(c->nbricks + 1) * c->brick_nodes;
     DBuf<float> din, dout;
-    VPS_TRY(dalloc(c, din, (size_t)nn));
+    VPS_TRY(dalloc(c, din, cap));
     VPS_TRY(dalloc(c, dout, (size_t)nn));
+    VPS_HIP(c, hipMemsetAsync(din.p, 0, cap * 4, c->stream));
     VPS_HIP(c, hipMemcpyAsync(din.p, in.data(), (size_t)nn * 4,
                               hipMemcpyHostToDevice, c->stream));
     VPS_TRY(apply_operator_f32(c, din.p, dout.p, c->p.op_mode));

@@ -16,6 +16,9 @@ namespace vps {
 
 namespace {
 
+// Cell id and a slot inside the cell.  Particles arrive in last step's cell
+// order, so a wave holds runs of equal cells: one atomic per run (its head
+// lane adds the run length), lanes take consecutive slots of their run.
 template <int D>
 __global__ void __launch_bounds__(256) k_cell_count(const double4* pos,
                                                     int64_t n, KConst k,
@@ -23,14 +26,34 @@ __global__ void __launch_bounds__(256) k_cell_count(const double4* pos,
                                                     int32_t* cell_of,
                                                     int32_t* slot_of) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const double4 p = pos[i];
-    const int cx = cell_coord(p.x, k.cell_lo[0], k.cell_inv, k.cdims[0]);
-    const int cy = cell_coord(p.y, k.cell_lo[1], k.cell_inv, k.cdims[1]);
-    const int cz = D == 3 ? cell_coord(p.z, k.cell_lo[2], k.cell_inv, k.cdims[2]) : 0;
-    const int c = (cz * k.cdims[1] + cy) * k.cdims[0] + cx;
-    cell_of[i] = c;
-    slot_of[i] = atomicAdd(&count[c], 1);
+    const bool valid = i < n;
+    int c = -1;
+    if (valid) {
+        const double4 p = pos[i];
+        const int cx = cell_coord(p.x, k.cell_lo[0], k.cell_inv, k.cdims[0]);
+        const int cy = cell_coord(p.y, k.cell_lo[1], k.cell_inv, k.cdims[1]);
+        const int cz = D == 3 ? cell_coord(p.z, k.cell_lo[2], k.cell_inv, k.cdims[2]) : 0;
+        c = (cz * k.cdims[1] + cy) * k.cdims[0] + cx;
+        cell_of[i] = c;
+    }
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(c, 1, 64);
+    const bool head = valid && (lane == 0 || prev != c);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long vmask = __ballot(valid);
+    const unsigned long long upto =
+        lane == 63 ? ~0ull : ((2ull << lane) - 1ull);   // lanes 0..lane
+    const int leader = 63 - __clzll(heads & upto);
+    int base = 0;
+    if (head) {
+        const unsigned long long after = heads & ~upto;
+        const int next = after ? __ffsll((long long)after) - 1 : 64;
+        const int nvalid = __popcll(vmask);
+        const int end = next < nvalid ? next : nvalid;
+        base = atomicAdd(&count[c], end - lane);
+    }
+    base = __shfl(base, leader < 0 ? 0 : leader, 64);
+    if (valid) slot_of[i] = base + (lane - leader);
 }
 
 __global__ void __launch_bounds__(256) k_cell_scatter(const int32_t* cell_of,
@@ -91,8 +114,7 @@ __global__ void __launch_bounds__(256) k_density(const double4* pos,
     if (i >= n) return;
     const double4 q = pos[i];
     float sum = 0.f;
-    for_each_candidate<D>(k, cstart, q.x, q.y, q.z, [&](int j) {
-        const double4 pj = pos[j];
+    for_each_candidate<D>(k, cstart, pos, q.x, q.y, q.z, [&](int j, const double4& pj) {
         const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, q.x, q.y, q.z)
                                  : sq_dist2(pj.x, pj.y, q.x, q.y);
         if (d2 < k.h2) {
@@ -116,8 +138,7 @@ __global__ void __launch_bounds__(256) k_bdensity(const double4* pos,
     if (i >= n) return;
     const double4 q = pos[i];
     float sum = 0.f;
-    for_each_candidate<D>(k, cstart, q.x, q.y, q.z, [&](int j) {
-        const double4 pj = pos[j];
+    for_each_candidate<D>(k, cstart, pos, q.x, q.y, q.z, [&](int j, const double4& pj) {
         const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, q.x, q.y, q.z)
                                  : sq_dist2(pj.x, pj.y, q.x, q.y);
         if (d2 < k.h2) {

@@ -70,21 +70,34 @@ __global__ void __launch_bounds__(256) k_gradient(
                 const int64_t brow = ((int64_t)(D == 3 ? cz >> 3 : 0) * k.bdims[1] +
                                       (cy >> 3)) * k.bdims[0];
                 const int lrow = 8 * ((cy & 7) + (D == 3 ? 8 * (cz & 7) : 0));
-                const int NB = D == 3 ? 512 : 64;
+                constexpr int NB = D == 3 ? 512 : 64;
+                // the row's <= 10 nodes lie in at most 3 bricks along x
+                const int bx0 = lo[0] >> 3;
+                int s3[3];
+#pragma unroll
+                for (int u = 0; u < 3; u++)
+                    s3[u] = bx0 + u < k.bdims[0] ? slot_of_brick[brow + bx0 + u] : -1;
+                // issue the q loads of every node inside the sphere at once
+                float qv[kBox];
 #pragma unroll
                 for (int c = 0; c < kBox; c++) {
                     const double d2 = D == 3 ? dadd(dadd(dx2[c], dy2), dz2)
                                              : dadd(dx2[c], dy2);
-                    if (c >= cnt[0] || !(d2 < k.H2)) continue;
                     const int cx = lo[0] + c;
-                    const int s = slot_of_brick[brow + (cx >> 3)];
-                    if (s < 0) continue;
-                    const float qv = vq[(int64_t)s * NB + lrow + (cx & 7)];
-                    if (qv == 0.f) continue;
+                    const int u = (cx >> 3) - bx0;
+                    const int s = u == 0 ? s3[0] : (u == 1 ? s3[1] : s3[2]);
+                    qv[c] = (c < cnt[0] && d2 < k.H2 && s >= 0)
+                                ? vq[(int64_t)s * NB + lrow + (cx & 7)] : 0.f;
+                }
+#pragma unroll
+                for (int c = 0; c < kBox; c++) {
+                    if (qv[c] == 0.f) continue;
+                    const double d2 = D == 3 ? dadd(dadd(dx2[c], dy2), dz2)
+                                             : dadd(dx2[c], dy2);
                     const KW kH = kernel_f32(d2, k.H2, k.Hf, k.inv_H, k.c0H, k.c1H);
                     if (kH.r > 0.f) {
                         // gradK_H(x_i - x_J) = -W'(r) (x_i - x_J) / r
-                        const float sc = qv * (-kH.dw / kH.r);
+                        const float sc = qv[c] * (-kH.dw / kH.r);
                         fx -= sc * (float)dxs[c];
                         fy -= sc * (float)dy;
                         if (D == 3) fz -= sc * (float)dz;
@@ -94,8 +107,7 @@ __global__ void __launch_bounds__(256) k_gradient(
         }
     }
     // boundary term
-    for_each_candidate<D>(k, bcstart, q.x, q.y, q.z, [&](int j) {
-        const double4 pj = bpos[j];
+    for_each_candidate<D>(k, bcstart, bpos, q.x, q.y, q.z, [&](int j, const double4& pj) {
         const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, q.x, q.y, q.z)
                                  : sq_dist2(pj.x, pj.y, q.x, q.y);
         if (d2 < k.h2) {
@@ -146,8 +158,7 @@ __global__ void __launch_bounds__(128) k_query(
     int64_t cnt = 0;
     const int64_t base = offsets ? offsets[t] : 0;
     if (which < 2) {
-        for_each_candidate<D>(k, cstart, qx, qy, qz, [&](int j) {
-            const double4 pj = pos[j];
+        for_each_candidate<D>(k, cstart, pos, qx, qy, qz, [&](int j, const double4& pj) {
             const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz)
                                      : sq_dist2(pj.x, pj.y, qx, qy);
             if (d2 < k.h2) {

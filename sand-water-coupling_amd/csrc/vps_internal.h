@@ -155,11 +155,15 @@ __device__ __forceinline__ int cell_coord(double p, double lo, double inv,
 
 // Walks the candidates of a radius-h query: the 3^(D-1) x-rows of 3 cells
 // around the query cell, each one contiguous range of the cell-sorted set.
+// Positions are loaded 4 at a time so their latencies overlap; f(j, x_j)
+// is called for every candidate in order.
 template <int D, typename F>
 __device__ __forceinline__ void for_each_candidate(const KConst& k,
                                                    const int32_t* cstart,
+                                                   const double4* pos,
                                                    double qx, double qy,
                                                    double qz, F&& f) {
+    constexpr int U = 4;
     const int cx = cell_coord(qx, k.cell_lo[0], k.cell_inv, k.cdims[0]);
     const int cy = cell_coord(qy, k.cell_lo[1], k.cell_inv, k.cdims[1]);
     const int cz = D == 3 ? cell_coord(qz, k.cell_lo[2], k.cell_inv, k.cdims[2]) : 0;
@@ -173,7 +177,15 @@ __device__ __forceinline__ void for_each_candidate(const KConst& k,
         for (int y = ylo; y <= yhi; y++) {
             const int row = (z * k.cdims[1] + y) * k.cdims[0];
             const int b = cstart[row + x0], e = cstart[row + x1 + 1];
-            for (int j = b; j < e; j++) f(j);
+            for (int j = b; j < e; j += U) {
+                double4 pj[U];
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (j + u < e) pj[u] = pos[j + u];
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (j + u < e) f(j + u, pj[u]);
+            }
         }
 }
 
@@ -250,6 +262,7 @@ struct CgState {
     double pt;        // p.t
     double true_rr;   // ||b - M x||^2 from the FP64 check
     double thr_scale; // tol^2
+    double abs0;      // r.r at the (re)start of the iteration
     float alpha, beta;
     int iter;         // Eigen iteration counter
     int done;         // 0 running, 1 converged, 2 max iterations, 3 nonfinite
@@ -381,6 +394,7 @@ int boundary_density(vps_context* c);
 int virtual_generate(vps_context* c);
 int virtual_diag(vps_context* c, bool set_alpha0);
 int cg_solve(vps_context* c, vps_stats* st);
+int64_t partials_size(int64_t nbricks);
 int apply_operator_f32(vps_context* c, const float* in, float* out,
                        int op_mode);
 int pressure_gradient(vps_context* c);

@@ -18,24 +18,29 @@
 //
 // Kernel shape (3D): one 64-lane wave per 8x8x8 brick.  The 14^3 halo tile
 // (the brick plus 3 nodes on each side, read from the 27 neighbouring
-// bricks) is staged in LDS; lane (x, y) owns the z column of 8 outputs.
-// Each LDS plane is reduced once into four plane sums P_k (k = |dz|) using
-// the 8-fold symmetry of cc: the (+-a, +-b) points of a class are added
-// first and multiplied once per k, so one output costs ~130 VALU operations
-// instead of 250 FMAs.  Non-virtual nodes hold 0 in every vector, which
-// makes the masked stencil exact.
+// bricks) is staged in LDS with unconditional loads: a constant table maps
+// each tile element to (neighbour brick, local node), and absent bricks are
+// redirected to an all-zero brick kept at the end of every per-node array.
+// Lane (x, y) owns the z column of 8 outputs.  Each LDS plane is reduced
+// once into four plane sums P_k (k = |dz|) using the 8-fold symmetry of cc
+// (the (+-a, +-b) points of a class are added first and multiplied once per
+// k), and a 7-slot sliding window turns plane sums into outputs.
 //
 // CG (Eigen's algorithm, identity preconditioner, recursive residual):
-//   init      r = b - M x0, p = r, b.b, r.r           (one stencil launch)
-//   iterate   t = M p, p.t -> alpha                  (stencil launch; the
-//             p = r + beta p of the previous iteration is fused into its
-//             halo load, p double-buffered)
-//             x += alpha p, r -= alpha t, r.r -> beta  (streaming launch)
-// FP32 vectors, FP64 dot products and norms; every scalar stays on the
-// device (the last block of each launch finishes the reduction in a fixed
-// order, so results do not depend on scheduling).  At convergence the FP64
-// true residual ||b - M x|| is checked and, if it is above the tolerance,
-// the residual is replaced by it and CG continues (residual replacement).
+//   init      r = b - M x0, p = r, b.b, r.r     (stencil launch + 1-block
+//                                                finish)
+//   iterate   SpMV(k):   rr_{k-1} -> converged? beta; p = r + beta p_old
+//                        (fused into the halo load, p double-buffered);
+//                        t = M p; partial p.t per block
+//             update(k): alpha = rr_{k-1} / p.t; x += alpha p;
+//                        r -= alpha t; partial r.r per block
+// Every grid-wide sum is formed by the CONSUMING kernel: its blocks re-sum
+// the producer's per-block partials in a fixed order (a ring of three
+// iterations).  No device-scope fence or atomic is issued inside the loop --
+// on a multi-XCD part an agent-scope release writes back the L2 -- and the
+// result is bitwise reproducible.  FP32 vectors, FP64 dots and norms.  At
+// convergence the FP64 true residual ||b - M x|| is checked; if it is above
+// the tolerance CG restarts from the true residual (residual replacement).
 #include <cfloat>
 #include <cmath>
 
@@ -55,14 +60,57 @@ namespace {
 constexpr int kT = 14;             // halo tile side (8 + 2 * 3)
 constexpr int kT2 = kT * kT;
 constexpr int kT3 = kT * kT * kT;
+constexpr int kHaloIters3 = (kT3 + 63) / 64;   // 43
+constexpr int kHaloIters2 = (kT2 + 63) / 64;   // 4
+constexpr int kUpdBlocks = 256;                // grid of the CG update
+constexpr int kUpdThreads = 1024;
+
+// Tile element e -> (neighbour brick 0..26 or 0..8) << 9 | local node.
+// Elements past the tile map to the zero brick slot 27 (3D) / 9 (2D).
+struct HaloTab3 {
+    uint16_t v[kHaloIters3 * 64];
+};
+struct HaloTab2 {
+    uint16_t v[kHaloIters2 * 64];
+};
+constexpr HaloTab3 make_halo3() {
+    HaloTab3 t{};
+    for (int e = 0; e < kHaloIters3 * 64; e++) {
+        if (e >= kT3) {
+            t.v[e] = (uint16_t)(27 << 9);
+            continue;
+        }
+        const int ux = e % kT + 5, uy = (e / kT) % kT + 5, uz = e / kT2 + 5;
+        const int nb = (uz >> 3) * 9 + (uy >> 3) * 3 + (ux >> 3);
+        const int loc = (ux & 7) + 8 * ((uy & 7) + 8 * (uz & 7));
+        t.v[e] = (uint16_t)((nb << 9) | loc);
+    }
+    return t;
+}
+constexpr HaloTab2 make_halo2() {
+    HaloTab2 t{};
+    for (int e = 0; e < kHaloIters2 * 64; e++) {
+        if (e >= kT2) {
+            t.v[e] = (uint16_t)(9 << 9);
+            continue;
+        }
+        const int ux = e % kT + 5, uy = e / kT + 5;
+        const int nb = (uy >> 3) * 3 + (ux >> 3);
+        const int loc = (ux & 7) + 8 * (uy & 7);
+        t.v[e] = (uint16_t)((nb << 9) | loc);
+    }
+    return t;
+}
+__constant__ HaloTab3 kHalo3 = make_halo3();
+__constant__ HaloTab2 kHalo2 = make_halo2();
 
 enum Kind {
     K_DIAG = 0,        // s = -sum cc V_J; diag = max(s, alpha0)   (double)
     K_APPLY = 1,       // out = M in                                 (float)
-    K_CG_INIT = 2,     // r = b - M x0; p = r; b.b, r.r             (float)
-    K_SPMV_FIRST = 3,  // t = M p; p.t                               (float)
-    K_SPMV_FUSED = 4,  // p = r + beta p_old; t = M p; p.t           (float)
-    K_TRUE_RES = 5     // rt = b - M x (double), sum rt^2            (double)
+    K_CG_INIT = 2,     // r = b - M x0; p = r; partial b.b, r.r     (float)
+    K_SPMV_FIRST = 3,  // t = M p; partial p.t                       (float)
+    K_SPMV_FUSED = 4,  // p = r + beta p_old; t = M p; partial p.t   (float)
+    K_TRUE_RES = 5     // rt = b - M x (double); partial rt.rt      (double)
 };
 
 struct StencilArgs {
@@ -70,11 +118,12 @@ struct StencilArgs {
     const int32_t* slot_of_brick;
     const int32_t* brick_of_slot;
     int64_t nslots;
+    int64_t zero_slot;     // index of the all-zero brick
     const float* vvol;
     const float* vdiag;
     const double* vdiag64;
     const float* vb;
-    const float* in;       // p / x / x0 / in
+    const float* in;       // p / x / x0 / in (p_old when fused)
     const float* in2;      // r for the fused update
     float* out;            // t / r / out
     float* out2;           // p_new (fused) or p (init)
@@ -82,63 +131,35 @@ struct StencilArgs {
     float* out_f;          // diag32
     double alpha0;
     CgState* st;
-    double* partials;
+    double* part0;         // this kernel's per-block partials (sum 0)
+    double* part1;         // per-block partials (sum 1, init only)
+    const double* rr1;     // update partials of iteration kk-1
+    const double* rr2;     // update partials of iteration kk-2
+    int kk;                // iteration index since the (re)start
+    int iter_base;         // Eigen iteration count at the (re)start
+    int64_t max_iter;
 };
 
-// 2-value deterministic grid sum (b.b and r.r at CG init).
+// Sum of n per-block partials, identical in every block and lane: lane l
+// adds partials l, l + 64, ... in order, then a fixed xor tree.
+__device__ __forceinline__ double sum_partials64(const double* p, int n) {
+    const int lane = threadIdx.x & 63;
+    double s = 0;
+    for (int i = lane; i < n; i += 64) s += p[i];
+    return wave_sum(s);
+}
+
 template <int BS>
-__device__ bool grid_sum2_last(double v0, double v1, double* partials,
-                               unsigned* counter, double* out0, double* out1) {
-    __shared__ double s0[BS / 64], s1[BS / 64];
-    __shared__ bool is_last;
-    v0 = wave_sum(v0);
-    v1 = wave_sum(v1);
+__device__ __forceinline__ double block_sum(double v, double* sbuf) {
+    v = wave_sum(v);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) {
-        s0[wv] = v0;
-        s1[wv] = v1;
-    }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = 0, b = 0;
-        for (int i = 0; i < BS / 64; i++) {
-            a += s0[i];
-            b += s1[i];
-        }
-        partials[2 * blockIdx.x] = a;
-        partials[2 * blockIdx.x + 1] = b;
-        __threadfence();
-        is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
-    }
+    if (lane == 0) sbuf[wv] = v;
     __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    double a = 0, b = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += BS) {
-        a += __hip_atomic_load(&partials[2 * i], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        b += __hip_atomic_load(&partials[2 * i + 1], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-    }
-    a = wave_sum(a);
-    b = wave_sum(b);
-    __syncthreads();
-    if (lane == 0) {
-        s0[wv] = a;
-        s1[wv] = b;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double x = 0, y = 0;
-        for (int i = 0; i < BS / 64; i++) {
-            x += s0[i];
-            y += s1[i];
-        }
-        *out0 = x;
-        *out1 = y;
-        *counter = 0;
-    }
-    return true;
+    double t = 0;
+#pragma unroll
+    for (int i = 0; i < BS / 64; i++) t += sbuf[i];
+    return t;
 }
 
 // Plane sums P_k, k = 0..3, at the LDS point pl (row stride kT).
@@ -189,9 +210,35 @@ __device__ __forceinline__ T plane1(const T* pl, const C& cc) {
     return P;
 }
 
-// Loads the stencil input value of node index g (slot-major) for this kind.
+// Prologue of the CG SpMV: convergence test of iteration kk-1, beta.
+// Returns false when the solve is over (block 0 records why).
+__device__ __forceinline__ bool spmv_prologue(const StencilArgs& a,
+                                              float* beta) {
+    CgState* st = a.st;
+    if (st->done) return false;
+    *beta = 0.f;
+    if (a.kk == 0) return true;
+    const double r1 = sum_partials64(a.rr1, kUpdBlocks);
+    const double r2 = a.kk >= 2 ? sum_partials64(a.rr2, kUpdBlocks) : st->abs0;
+    const int k = a.iter_base + a.kk;
+    int done = 0;
+    if (!isfinite(r1)) done = 3;
+    else if (r1 < st->thr) done = 1;
+    else if (k >= a.max_iter) done = 2;
+    if (done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->rr = r1;
+            st->iter = done == 1 ? k - 1 : k;
+            st->done = done;
+        }
+        return false;
+    }
+    *beta = (float)(r1 / r2);
+    return true;
+}
+
 template <int KIND, int OP, typename T>
-__device__ __forceinline__ T load_input(const StencilArgs& a, int64_t g,
+__device__ __forceinline__ T load_input(const StencilArgs& a, uint32_t g,
                                         float beta, float* pnew_out) {
     if constexpr (KIND == K_DIAG) {
         return (T)a.vvol[g];
@@ -207,55 +254,89 @@ __device__ __forceinline__ T load_input(const StencilArgs& a, int64_t g,
 
 template <int KIND, int OP, typename T, typename C>
 __global__ void __launch_bounds__(64) k_stencil3(StencilArgs a, C cc) {
-    __shared__ T tile[kT3];
+    __shared__ T tile[kHaloIters3 * 64];
     __shared__ float own[512];   // raw own-brick input (p / x)
-    __shared__ int nslot[27];
+    __shared__ uint32_t sbase[28];
     float beta = 0.f;
     if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_FUSED) {
-        if (a.st->done) return;
-        beta = a.st->beta;
+        if (!spmv_prologue(a, &beta)) return;
     }
     const int slot = xcd_remap(blockIdx.x, (int)a.nslots);
     const int brick = a.brick_of_slot[slot];
     const int t = threadIdx.x;
-    const int bx = brick % a.k.bdims[0];
-    const int by = (brick / a.k.bdims[0]) % a.k.bdims[1];
-    const int bz = brick / a.k.bdims[0] / a.k.bdims[1];
-    if (t < 27) {
-        const int nx = bx + (t % 3) - 1, ny = by + (t / 3) % 3 - 1,
-                  nz = bz + t / 9 - 1;
-        int s = -1;
-        if (nx >= 0 && ny >= 0 && nz >= 0 && nx < a.k.bdims[0] &&
-            ny < a.k.bdims[1] && nz < a.k.bdims[2])
-            s = a.slot_of_brick[((int64_t)nz * a.k.bdims[1] + ny) * a.k.bdims[0] + nx];
-        nslot[t] = s;
-    }
-    __syncthreads();
-    for (int e = t; e < kT3; e += 64) {
-        const int xx = e % kT, yy = (e / kT) % kT, zz = e / kT2;
-        const int ux = xx + 5, uy = yy + 5, uz = zz + 5;   // = local + 8
-        const int s = nslot[(uz >> 3) * 9 + (uy >> 3) * 3 + (ux >> 3)];
-        T v = T(0);
-        if (s >= 0) {
-            const int loc = (ux & 7) + 8 * ((uy & 7) + 8 * (uz & 7));
-            const int64_t g = (int64_t)s * 512 + loc;
-            float pn = 0.f;
-            v = load_input<KIND, OP, T>(a, g, beta, &pn);
-            if constexpr (KIND != K_DIAG) {
-                if ((ux >> 3) == 1 && (uy >> 3) == 1 && (uz >> 3) == 1) {
-                    own[loc] = pn;
-                    if constexpr (KIND == K_SPMV_FUSED) a.out2[g] = pn;
-                }
+    if (t < 28) {
+        int64_t s = a.zero_slot;
+        if (t < 27) {
+            const int bx = brick % a.k.bdims[0];
+            const int by = (brick / a.k.bdims[0]) % a.k.bdims[1];
+            const int bz = brick / a.k.bdims[0] / a.k.bdims[1];
+            const int nx = bx + (t % 3) - 1, ny = by + (t / 3) % 3 - 1,
+                      nz = bz + t / 9 - 1;
+            if (nx >= 0 && ny >= 0 && nz >= 0 && nx < a.k.bdims[0] &&
+                ny < a.k.bdims[1] && nz < a.k.bdims[2]) {
+                const int sl = a.slot_of_brick[((int64_t)nz * a.k.bdims[1] + ny) *
+                                                   a.k.bdims[0] + nx];
+                if (sl >= 0) s = sl;
             }
-        } else if constexpr (KIND != K_DIAG) {
-            if ((ux >> 3) == 1 && (uy >> 3) == 1 && (uz >> 3) == 1)
-                own[(ux & 7) + 8 * ((uy & 7) + 8 * (uz & 7))] = 0.f;
         }
-        tile[e] = v;
+        sbase[t] = (uint32_t)(s * 512);
     }
-    __syncthreads();
     const int x = t & 7, y = t >> 3;
     const int64_t base = (int64_t)slot * 512;
+    // epilogue operands of this lane's 8 outputs, fetched early
+    float Vz[8], Dz[8], Bz[8];
+#pragma unroll
+    for (int z = 0; z < 8; z++) {
+        const int64_t g = base + x + 8 * y + 64 * z;
+        Vz[z] = a.vvol[g];
+        if constexpr (KIND != K_DIAG && KIND != K_TRUE_RES) Dz[z] = a.vdiag[g];
+        else Dz[z] = 0.f;
+        if constexpr (KIND == K_CG_INIT || KIND == K_TRUE_RES) Bz[z] = a.vb[g];
+        else Bz[z] = 0.f;
+    }
+    __syncthreads();
+    // Halo load: element e = t + 64 i of the 14^3 tile, decoded
+    // arithmetically; 4 chunks of 11 loads, each chunk in flight at once.
+    constexpr int kChunk = 11;
+#pragma unroll 1
+    for (int i0 = 0; i0 < kHaloIters3; i0 += kChunk) {
+        T val[kChunk];
+        float pnv[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; j++) {
+            const int e = t + 64 * (i0 + j);
+            const int zz = e / kT2;
+            const int rem = e - zz * kT2;
+            const int yy = rem / kT;
+            const int xx = rem - yy * kT;
+            const int ux = xx + 5, uy = yy + 5, uz = zz + 5;
+            const int nb = e < kT3 ? (uz >> 3) * 9 + (uy >> 3) * 3 + (ux >> 3) : 27;
+            const uint32_t g =
+                sbase[nb] + (uint32_t)((ux & 7) + 8 * ((uy & 7) + 8 * (uz & 7)));
+            float pn = 0.f;
+            val[j] = i0 + j < kHaloIters3
+                         ? load_input<KIND, OP, T>(a, g, beta, &pn) : T(0);
+            pnv[j] = pn;
+        }
+#pragma unroll
+        for (int j = 0; j < kChunk; j++) {
+            if (i0 + j >= kHaloIters3) continue;
+            const int e = t + 64 * (i0 + j);
+            tile[e] = val[j];
+            if constexpr (KIND != K_DIAG) {
+                const int zz = e / kT2;
+                const int rem = e - zz * kT2;
+                const int yy = rem / kT;
+                const int xx = rem - yy * kT;
+                if (xx >= 3 && xx < 11 && yy >= 3 && yy < 11 && zz >= 3 && zz < 11) {
+                    const int loc = (xx - 3) + 8 * ((yy - 3) + 8 * (zz - 3));
+                    own[loc] = pnv[j];
+                    if constexpr (KIND == K_SPMV_FUSED) a.out2[base + loc] = pnv[j];
+                }
+            }
+        }
+    }
+    __syncthreads();
     double red0 = 0, red1 = 0;
     // Sliding window: after plane zp (z offset zz = zp - 3) the outputs
     // z = zz - 3 .. zz + 3 have pending sums W[0..6]; z = zz - 3 is complete.
@@ -273,7 +354,15 @@ __global__ void __launch_bounds__(64) k_stencil3(StencilArgs a, C cc) {
             const T acc = W[0];
             const int loc = x + 8 * y + 64 * z;
             const int64_t g = base + loc;
-            const float V = a.vvol[g];
+            // z is uniform across the wave: select the prefetched operands
+            float V = 0.f, Dg = 0.f, Bg = 0.f;
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (q == z) {
+                    V = Vz[q];
+                    Dg = Dz[q];
+                    Bg = Bz[q];
+                }
             const bool on = V > 0.f;
             if constexpr (KIND == K_DIAG) {
                 const double sd = -(double)acc;
@@ -285,17 +374,17 @@ __global__ void __launch_bounds__(64) k_stencil3(StencilArgs a, C cc) {
                 const double st = OP == VPS_OP_ROW ? (double)acc
                                                    : (double)V * (double)acc;
                 const double mx = a.vdiag64[g] * xi + st;
-                const double rt = on ? (double)a.vb[g] - mx : 0.0;
+                const double rt = on ? (double)Bg - mx : 0.0;
                 a.out[g] = (float)rt;
                 red0 += rt * rt;
             } else {
                 const float pi = own[loc];
                 const float st = OP == VPS_OP_ROW ? (float)acc : V * (float)acc;
-                const float mp = on ? a.vdiag[g] * pi + st : 0.f;
+                const float mp = on ? Dg * pi + st : 0.f;
                 if constexpr (KIND == K_APPLY) {
                     a.out[g] = mp;
                 } else if constexpr (KIND == K_CG_INIT) {
-                    const float b = a.vb[g];
+                    const float b = Bg;
                     const float r = on ? b - mp : 0.f;
                     a.out[g] = r;
                     a.out2[g] = r;
@@ -311,40 +400,12 @@ __global__ void __launch_bounds__(64) k_stencil3(StencilArgs a, C cc) {
         for (int j = 0; j < 6; j++) W[j] = W[j + 1];
         W[6] = T(0);
     }
-    if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_FUSED) {
-        double pt;
-        if (grid_sum_last<64>(red0, a.partials, &a.st->counter[0], &pt) &&
-            t == 0) {
-            a.st->pt = pt;
-            const double al = a.st->abs_new / pt;
-            a.st->alpha = (float)al;
-            if (!isfinite(al)) a.st->done = 3;
-        }
-    } else if constexpr (KIND == K_TRUE_RES) {
-        double s;
-        if (grid_sum_last<64>(red0, a.partials, &a.st->counter[0], &s) && t == 0)
-            a.st->true_rr = s;
-    } else if constexpr (KIND == K_CG_INIT) {
-        double bb, rr;
-        if (grid_sum2_last<64>(red0, red1, a.partials, &a.st->counter[0], &bb,
-                               &rr) &&
-            t == 0) {
-            CgState* st = a.st;
-            st->bb = bb;
-            st->rr = rr;
-            st->abs_new = rr;
-            st->iter = 0;
-            st->beta = 0.f;
-            st->true_rr = -1.0;
-            if (!isfinite(bb) || !isfinite(rr)) {
-                st->done = 3;
-            } else if (bb == 0.0) {
-                st->done = 4;   // Eigen: x = 0, zero iterations
-            } else {
-                double thr = st->thr_scale * bb;
-                st->thr = thr < DBL_MIN ? DBL_MIN : thr;
-                st->done = rr < st->thr ? 1 : 0;
-            }
+    if constexpr (KIND != K_DIAG && KIND != K_APPLY) {
+        red0 = wave_sum(red0);
+        if (KIND == K_CG_INIT) red1 = wave_sum(red1);
+        if (t == 0) {
+            a.part0[blockIdx.x] = red0;
+            if (KIND == K_CG_INIT) a.part1[blockIdx.x] = red1;
         }
     }
 }
@@ -352,48 +413,43 @@ __global__ void __launch_bounds__(64) k_stencil3(StencilArgs a, C cc) {
 // 2D: one 64-lane wave per 8x8 brick, one output per lane.
 template <int KIND, int OP, typename T, typename C>
 __global__ void __launch_bounds__(64) k_stencil2(StencilArgs a, C cc) {
-    __shared__ T tile[kT2];
+    __shared__ T tile[kHaloIters2 * 64];
     __shared__ float own[64];
-    __shared__ int nslot[9];
+    __shared__ int64_t sbase[10];
     float beta = 0.f;
     if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_FUSED) {
-        if (a.st->done) return;
-        beta = a.st->beta;
+        if (!spmv_prologue(a, &beta)) return;
     }
     const int slot = xcd_remap(blockIdx.x, (int)a.nslots);
     const int brick = a.brick_of_slot[slot];
     const int t = threadIdx.x;
-    const int bx = brick % a.k.bdims[0];
-    const int by = brick / a.k.bdims[0];
-    if (t < 9) {
-        const int nx = bx + (t % 3) - 1, ny = by + t / 3 - 1;
-        int s = -1;
-        if (nx >= 0 && ny >= 0 && nx < a.k.bdims[0] && ny < a.k.bdims[1])
-            s = a.slot_of_brick[(int64_t)ny * a.k.bdims[0] + nx];
-        nslot[t] = s;
+    if (t < 10) {
+        int64_t s = a.zero_slot;
+        if (t < 9) {
+            const int bx = brick % a.k.bdims[0];
+            const int by = brick / a.k.bdims[0];
+            const int nx = bx + (t % 3) - 1, ny = by + t / 3 - 1;
+            if (nx >= 0 && ny >= 0 && nx < a.k.bdims[0] && ny < a.k.bdims[1]) {
+                const int sl = a.slot_of_brick[(int64_t)ny * a.k.bdims[0] + nx];
+                if (sl >= 0) s = sl;
+            }
+        }
+        sbase[t] = s * 64;
     }
     __syncthreads();
-    for (int e = t; e < kT2; e += 64) {
-        const int xx = e % kT, yy = e / kT;
-        const int ux = xx + 5, uy = yy + 5;
-        const int s = nslot[(uy >> 3) * 3 + (ux >> 3)];
-        T v = T(0);
-        if (s >= 0) {
-            const int loc = (ux & 7) + 8 * (uy & 7);
-            const int64_t g = (int64_t)s * 64 + loc;
-            float pn = 0.f;
-            v = load_input<KIND, OP, T>(a, g, beta, &pn);
-            if constexpr (KIND != K_DIAG) {
-                if ((ux >> 3) == 1 && (uy >> 3) == 1) {
-                    own[loc] = pn;
-                    if constexpr (KIND == K_SPMV_FUSED) a.out2[g] = pn;
-                }
+#pragma unroll
+    for (int i = 0; i < kHaloIters2; i++) {
+        const int e = t + 64 * i;
+        const int v = kHalo2.v[e];
+        const int64_t g = sbase[v >> 9] + (v & 511);
+        float pn = 0.f;
+        tile[e] = load_input<KIND, OP, T>(a, g, beta, &pn);
+        if constexpr (KIND != K_DIAG) {
+            if ((v >> 9) == 4) {
+                own[v & 511] = pn;
+                if constexpr (KIND == K_SPMV_FUSED) a.out2[g] = pn;
             }
-        } else if constexpr (KIND != K_DIAG) {
-            if ((ux >> 3) == 1 && (uy >> 3) == 1)
-                own[(ux & 7) + 8 * (uy & 7)] = 0.f;
         }
-        tile[e] = v;
     }
     __syncthreads();
     const int x = t & 7, y = t >> 3;
@@ -433,56 +489,43 @@ __global__ void __launch_bounds__(64) k_stencil2(StencilArgs a, C cc) {
             red0 = (double)pi * (double)mp;
         }
     }
-    if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_FUSED) {
-        double pt;
-        if (grid_sum_last<64>(red0, a.partials, &a.st->counter[0], &pt) &&
-            t == 0) {
-            a.st->pt = pt;
-            const double al = a.st->abs_new / pt;
-            a.st->alpha = (float)al;
-            if (!isfinite(al)) a.st->done = 3;
-        }
-    } else if constexpr (KIND == K_TRUE_RES) {
-        double s;
-        if (grid_sum_last<64>(red0, a.partials, &a.st->counter[0], &s) && t == 0)
-            a.st->true_rr = s;
-    } else if constexpr (KIND == K_CG_INIT) {
-        double bb, rr;
-        if (grid_sum2_last<64>(red0, red1, a.partials, &a.st->counter[0], &bb,
-                               &rr) &&
-            t == 0) {
-            CgState* st = a.st;
-            st->bb = bb;
-            st->rr = rr;
-            st->abs_new = rr;
-            st->iter = 0;
-            st->beta = 0.f;
-            st->true_rr = -1.0;
-            if (!isfinite(bb) || !isfinite(rr)) {
-                st->done = 3;
-            } else if (bb == 0.0) {
-                st->done = 4;
-            } else {
-                double thr = st->thr_scale * bb;
-                st->thr = thr < DBL_MIN ? DBL_MIN : thr;
-                st->done = rr < st->thr ? 1 : 0;
-            }
+    if constexpr (KIND != K_DIAG && KIND != K_APPLY) {
+        red0 = wave_sum(red0);
+        if (KIND == K_CG_INIT) red1 = wave_sum(red1);
+        if (t == 0) {
+            a.part0[blockIdx.x] = red0;
+            if (KIND == K_CG_INIT) a.part1[blockIdx.x] = red1;
         }
     }
 }
 
-// x += alpha p; r -= alpha t; r.r -> convergence test and beta.
-__global__ void __launch_bounds__(256) k_cg_update(int64_t n4, float4* x,
-                                                   float4* r, const float4* p,
-                                                   const float4* tt,
-                                                   CgState* st,
-                                                   double* partials,
-                                                   int64_t max_iter) {
+// x += alpha p; r -= alpha t; partial r.r.  alpha = rr_{k-1} / (p.t)_k,
+// both re-summed from the producers' partials.
+__global__ void __launch_bounds__(kUpdThreads) k_cg_update(
+    int64_t n4, float4* x, float4* r, const float4* p, const float4* tt,
+    CgState* st, const double* pt_part, int npt, const double* rr_prev,
+    int kk, double* rr_out) {
+    __shared__ double sbuf[kUpdThreads / 64];
     if (st->done) return;
-    const float al = st->alpha;
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double pt = 0, rp = 0;
+    for (int i = threadIdx.x; i < npt; i += kUpdThreads) pt += pt_part[i];
+    pt = block_sum<kUpdThreads>(pt, sbuf);
+    if (kk == 0) {
+        rp = st->abs0;
+    } else {
+        double s = 0;
+        for (int i = threadIdx.x; i < kUpdBlocks; i += kUpdThreads) s += rr_prev[i];
+        rp = block_sum<kUpdThreads>(s, sbuf);
+    }
+    const double ald = rp / pt;
+    if (!isfinite(ald)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 3;
+        return;
+    }
+    const float al = (float)ald;
     double rr = 0;
-    if (i < n4) {
+    for (int64_t i = (int64_t)blockIdx.x * kUpdThreads + threadIdx.x; i < n4;
+         i += (int64_t)kUpdBlocks * kUpdThreads) {
         float4 xv = x[i], rv = r[i];
         const float4 pv = p[i], tv = tt[i];
         xv.x += al * pv.x;
@@ -495,83 +538,109 @@ __global__ void __launch_bounds__(256) k_cg_update(int64_t n4, float4* x,
         rv.w -= al * tv.w;
         x[i] = xv;
         r[i] = rv;
-        rr = ((double)rv.x * rv.x + (double)rv.y * rv.y) +
-             ((double)rv.z * rv.z + (double)rv.w * rv.w);
+        rr += ((double)rv.x * rv.x + (double)rv.y * rv.y) +
+              ((double)rv.z * rv.z + (double)rv.w * rv.w);
     }
-    double tot;
-    if (grid_sum_last<256>(rr, partials, &st->counter[1], &tot) &&
-        threadIdx.x == 0) {
-        st->rr = tot;
-        if (!isfinite(tot)) {
-            st->done = 3;
-        } else if (tot < st->thr) {
-            st->done = 1;
-        } else {
-            st->beta = (float)(tot / st->abs_new);
-            st->abs_new = tot;
-            st->iter += 1;
-            if (st->iter >= max_iter) st->done = 2;
-        }
-    }
+    rr = block_sum<kUpdThreads>(rr, sbuf);
+    if (threadIdx.x == 0) rr_out[blockIdx.x] = rr;
 }
 
-// Residual replacement: r = rt (already in r_true), beta, restart the loop.
-__global__ void __launch_bounds__(256) k_replace(int64_t n4, const float4* rt,
-                                                 float4* r, float4* p0,
-                                                 int write_p, CgState* st,
-                                                 double* partials) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+// CG start: b.b and r0.r0 from the init partials -> tolerance, flags.
+__global__ void __launch_bounds__(1024) k_cg_start(const double* pbb,
+                                                   const double* prr, int n,
+                                                   double tol_in, CgState* st) {
+    __shared__ double sbuf[16];
+    double a = 0, b = 0;
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        a += pbb[i];
+        b += prr[i];
+    }
+    const double bb = block_sum<1024>(a, sbuf);
+    const double rr = block_sum<1024>(b, sbuf);
+    if (threadIdx.x != 0) return;
+    st->bb = bb;
+    st->rr = rr;
+    st->abs0 = rr;
+    st->iter = 0;
+    st->true_rr = -1.0;
+    // IterativeSolver.h:25 setTolerance(max(tol, eps / ||b||)); Eigen's
+    // threshold max(tol^2 ||b||^2, DBL_MIN) (ConjugateGradient.h)
+    const double eps_tol = DBL_EPSILON / sqrt(bb);
+    const double tol = tol_in > eps_tol ? tol_in : eps_tol;
+    const double thr = tol * tol * bb;
+    st->thr = thr < DBL_MIN ? DBL_MIN : thr;
+    if (!isfinite(bb) || !isfinite(rr)) st->done = 3;
+    else if (bb == 0.0) st->done = 4;   // Eigen: x = 0, zero iterations
+    else st->done = rr < st->thr ? 1 : 0;
+}
+
+// Deterministic one-block sum of n partials into *out.
+__global__ void __launch_bounds__(1024) k_sum_partials(const double* p, int n,
+                                                       double* out) {
+    __shared__ double sbuf[16];
+    double a = 0;
+    for (int i = threadIdx.x; i < n; i += 1024) a += p[i];
+    a = block_sum<1024>(a, sbuf);
+    if (threadIdx.x == 0) *out = a;
+}
+
+// Residual replacement: r = rt; p = rt (restart direction); partial r.r.
+__global__ void __launch_bounds__(kUpdThreads) k_replace(int64_t n4,
+                                                         const float4* rt,
+                                                         float4* r, float4* p0,
+                                                         double* part) {
+    __shared__ double sbuf[kUpdThreads / 64];
     double rr = 0;
-    if (i < n4) {
+    for (int64_t i = (int64_t)blockIdx.x * kUpdThreads + threadIdx.x; i < n4;
+         i += (int64_t)kUpdBlocks * kUpdThreads) {
         const float4 v = rt[i];
         r[i] = v;
-        if (write_p) p0[i] = v;
-        rr = ((double)v.x * v.x + (double)v.y * v.y) +
-             ((double)v.z * v.z + (double)v.w * v.w);
+        p0[i] = v;
+        rr += ((double)v.x * v.x + (double)v.y * v.y) +
+              ((double)v.z * v.z + (double)v.w * v.w);
     }
-    double tot;
-    if (grid_sum_last<256>(rr, partials, &st->counter[2], &tot) &&
-        threadIdx.x == 0) {
-        st->rr = tot;
-        st->beta = (float)(tot / st->abs_new);
-        st->abs_new = tot;
-        st->done = tot < st->thr ? 1 : 0;
+    rr = block_sum<kUpdThreads>(rr, sbuf);
+    if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+
+// After a replacement: the restart's r.r becomes abs0.
+__global__ void __launch_bounds__(1024) k_restart(const double* part,
+                                                  CgState* st) {
+    __shared__ double sbuf[16];
+    double a = 0;
+    for (int i = threadIdx.x; i < kUpdBlocks; i += 1024) a += part[i];
+    a = block_sum<1024>(a, sbuf);
+    if (threadIdx.x == 0) {
+        st->abs0 = a;
+        st->rr = a;
+        st->done = a < st->thr ? 1 : 0;
     }
 }
 
 __global__ void __launch_bounds__(256) k_alpha0(int64_t n, const float* vvol,
                                                 const double* diag,
-                                                double* partials,
-                                                unsigned* counter,
-                                                double* out) {
+                                                double* part) {
     double m = -DBL_MAX;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * 256)
         if (vvol[i] > 0.f) m = diag[i] > m ? diag[i] : m;
     m = wave_max(m);
     __shared__ double sw[4];
-    __shared__ bool last;
     if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = sw[0];
         for (int i = 1; i < 4; i++) t = sw[i] > t ? sw[i] : t;
-        partials[blockIdx.x] = t;
-        __threadfence();
-        last = atomicAdd(counter, 1u) == gridDim.x - 1;
+        part[blockIdx.x] = t;
     }
-    __syncthreads();
-    if (last && threadIdx.x == 0) {
-        __threadfence();
-        double t = -DBL_MAX;
-        for (unsigned i = 0; i < gridDim.x; i++) {
-            const double v = __hip_atomic_load(&partials[i], __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-            t = v > t ? v : t;
-        }
-        *out = t;
-        *counter = 0;
-    }
+}
+
+__global__ void __launch_bounds__(64) k_max_partials(const double* p, int n,
+                                                     double* out) {
+    double m = -DBL_MAX;
+    for (int i = threadIdx.x; i < n; i += 64) m = p[i] > m ? p[i] : m;
+    m = wave_max(m);
+    if (threadIdx.x == 0) *out = m;
 }
 
 inline unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -600,19 +669,47 @@ static void make_coef(const vps_context* c, CoefD* d, CoefF* f) {
     }
 }
 
+// Partials layout (doubles): [3 x nbricks] SpMV ring, [3 x 256] update
+// ring, [2 x nbricks] init, [nbricks] true residual, [256] replacement,
+// [2048] misc.
+static double* part_pt(vps_context* c, int k) {
+    return c->partials.p + (int64_t)(k % 3) * c->nbricks;
+}
+static double* part_rr(vps_context* c, int k) {
+    return c->partials.p + 3 * c->nbricks + (int64_t)(k % 3) * kUpdBlocks;
+}
+static double* part_init(vps_context* c, int which) {
+    return c->partials.p + 3 * c->nbricks + 3 * kUpdBlocks +
+           (int64_t)which * c->nbricks;
+}
+static double* part_true(vps_context* c) {
+    return c->partials.p + 5 * c->nbricks + 3 * kUpdBlocks;
+}
+static double* part_repl(vps_context* c) {
+    return c->partials.p + 6 * c->nbricks + 3 * kUpdBlocks;
+}
+static double* part_misc(vps_context* c) {
+    return c->partials.p + 6 * c->nbricks + 4 * kUpdBlocks;
+}
+
+int64_t partials_size(int64_t nbricks) {
+    return 6 * nbricks + 4 * kUpdBlocks + 2048;
+}
+
 static StencilArgs base_args(vps_context* c) {
     StencilArgs a{};
     a.k = c->k;
     a.slot_of_brick = c->brick_slot[c->which].p;
     a.brick_of_slot = c->slot_brick[c->which].p;
     a.nslots = c->nslots;
+    a.zero_slot = c->nbricks;
     a.vvol = c->vvol.p;
     a.vdiag = c->vdiag.p;
     a.vdiag64 = c->vdiag64.p;
     a.vb = c->vb.p;
     a.st = c->cg.p;
-    a.partials = c->partials.p;
     a.alpha0 = c->alpha0;
+    a.max_iter = 1;
     return a;
 }
 
@@ -647,10 +744,12 @@ int virtual_diag(vps_context* c, bool set_alpha0) {
         const int64_t nn = c->nslots * c->brick_nodes;
         int64_t nb = blocks(nn, 256);
         if (nb > 1024) nb = 1024;
+        double* part = part_misc(c);
         double* dout = (double*)(c->counters.p + 4);
         hipLaunchKernelGGL(k_alpha0, dim3((unsigned)nb), dim3(256), 0,
-                           c->stream, nn, c->vvol.p, c->vdiag64.p,
-                           c->partials.p, &c->cg.p->counter[3], dout);
+                           c->stream, nn, c->vvol.p, c->vdiag64.p, part);
+        hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(64), 0, c->stream,
+                           part, (int)nb, dout);
         VPS_HIP(c, hipMemcpyAsync(c->host_counters + 4, dout, 8,
                                   hipMemcpyDeviceToHost, c->stream));
         VPS_HIP(c, hipStreamSynchronize(c->stream));
@@ -683,6 +782,14 @@ static int upload_state(vps_context* c, const CgState& s) {
     return VPS_OK;
 }
 
+static int read_state(vps_context* c, CgState* out) {
+    VPS_HIP(c, hipMemcpyAsync(c->host_cg, c->cg.p, sizeof(CgState),
+                              hipMemcpyDeviceToHost, c->stream));
+    VPS_HIP(c, hipStreamSynchronize(c->stream));
+    *out = *c->host_cg;
+    return VPS_OK;
+}
+
 static double ms_between(hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
@@ -711,46 +818,29 @@ int cg_solve(vps_context* c, vps_stats* stats) {
     float* t = c->vt.p;
     float* p0 = c->vp[0].p;
     float* p1 = c->vp[1].p;
-    // tolerance: max(tol, eps / ||b||) needs ||b||: the init kernel computes
-    // b.b; the scale tol^2 is fixed up after a first read when eps/||b|| wins
+    const int nslots = (int)c->nslots;
+
     CgState init{};
-    init.thr_scale = tol_in * tol_in;
     VPS_TRY(upload_state(c, init));
-    StencilArgs a = base_args(c);
-    a.in = x;
-    a.out = r;
-    a.out2 = p0;
-    launch_op<K_CG_INIT, float, CoefF>(c, a, cf);
-    VPS_HIP(c, hipGetLastError());
-    VPS_HIP(c, hipMemcpyAsync(c->host_cg, c->cg.p, sizeof(CgState),
-                              hipMemcpyDeviceToHost, c->stream));
-    VPS_HIP(c, hipStreamSynchronize(c->stream));
-    CgState hs = *c->host_cg;
-    double tol = tol_in;
-    if (hs.done == 0 || hs.done == 1) {
-        const double eps_tol = DBL_EPSILON / std::sqrt(hs.bb);
-        if (eps_tol > tol) {
-            // IterativeSolver.h:25: setTolerance(max(tol, eps/||b||))
-            tol = eps_tol;
-            hs.thr_scale = tol * tol;
-            double thr = hs.thr_scale * hs.bb;
-            hs.thr = thr < DBL_MIN ? DBL_MIN : thr;
-            hs.done = hs.rr < hs.thr ? 1 : 0;
-            VPS_TRY(upload_state(c, hs));
-        }
+    {
+        StencilArgs a = base_args(c);
+        a.in = x;
+        a.out = r;
+        a.out2 = p0;
+        a.part0 = part_init(c, 0);
+        a.part1 = part_init(c, 1);
+        launch_op<K_CG_INIT, float, CoefF>(c, a, cf);
+        hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(1024), 0, c->stream,
+                           part_init(c, 0), part_init(c, 1), nslots, tol_in,
+                           c->cg.p);
+        VPS_HIP(c, hipGetLastError());
     }
+    CgState hs;
+    VPS_TRY(read_state(c, &hs));
     if (hs.done == 3) {
         c->err = "non-finite right-hand side or initial residual";
         return VPS_ERR_NONFINITE;
     }
-    int64_t k = 0;
-    int64_t spmv_launches = 0;
-    const bool prof = c->kernel_timing;
-    const int64_t max_timed = (int64_t)c->kev.size() / 3;
-    double spmv_ms = 0, upd_ms = 0;
-    int64_t timed = 0;
-    int replacements = 0;
-    const int update_blocks = (int)blocks(n4, 256);
     if (hs.done == 4) {
         VPS_HIP(c, hipMemsetAsync(x, 0, (size_t)nn * 4, c->stream));
         hs.done = 1;
@@ -759,50 +849,59 @@ int cg_solve(vps_context* c, vps_stats* stats) {
         c->err = "CG reached the iteration limit without converging";
         return VPS_ERR_NO_CONVERGENCE;
     }
-    int predicted = c->prev_iters > 0 ? c->prev_iters : 8;
+    const bool prof = c->kernel_timing;
+    const int64_t max_timed = (int64_t)c->kev.size() / 3;
+    int64_t timed = 0, spmv_launches = 0;
+    int kk = 0;          // iteration since the (re)start
+    int iter_base = 0;   // Eigen iterations before the (re)start
+    int replacements = 0;
+    int predicted = c->prev_iters > 0 ? c->prev_iters + 1 : 8;
+    std::vector<int> timed_iter;
     while (true) {
         if (hs.done == 0) {
-            // launch a batch, then poll the done flag
-            int64_t batch = k + 2 < predicted ? predicted - k - 1 : 4;
+            int batch = iter_base + kk + 2 < predicted
+                            ? predicted - (iter_base + kk) : 4;
             if (batch > 64) batch = 64;
-            for (int64_t j = 0; j < batch; j++) {
+            for (int j = 0; j < batch; j++) {
                 StencilArgs s = base_args(c);
-                const bool first = (k == 0);
-                float* dir = first ? p0 : ((k & 1) ? p1 : p0);
+                s.kk = kk;
+                s.iter_base = iter_base;
+                s.max_iter = max_iter;
+                s.rr1 = part_rr(c, kk + 2);   // (kk - 1) mod 3
+                s.rr2 = part_rr(c, kk + 1);   // (kk - 2) mod 3
+                s.part0 = part_pt(c, kk);
+                s.out = t;
+                float* dir = (kk == 0) ? p0 : ((kk & 1) ? p1 : p0);
                 const bool do_time = prof && timed < max_timed;
-                if (do_time)
-                    VPS_HIP(c, hipEventRecord(c->kev[3 * timed], c->stream));
-                if (first) {
+                if (do_time) VPS_HIP(c, hipEventRecord(c->kev[3 * timed], c->stream));
+                if (kk == 0) {
                     s.in = p0;
-                    s.out = t;
                     launch_op<K_SPMV_FIRST, float, CoefF>(c, s, cf);
                 } else {
-                    s.in = (k & 1) ? p0 : p1;   // previous direction
+                    s.in = (kk & 1) ? p0 : p1;   // previous direction
                     s.in2 = r;
-                    s.out = t;
                     s.out2 = dir;
                     launch_op<K_SPMV_FUSED, float, CoefF>(c, s, cf);
                 }
                 if (do_time)
                     VPS_HIP(c, hipEventRecord(c->kev[3 * timed + 1], c->stream));
-                hipLaunchKernelGGL(k_cg_update, dim3(update_blocks), dim3(256),
+                hipLaunchKernelGGL(k_cg_update, dim3(kUpdBlocks), dim3(kUpdThreads),
                                    0, c->stream, n4, (float4*)x, (float4*)r,
-                                   (const float4*)dir, (const float4*)t,
-                                   c->cg.p, c->partials.p, max_iter);
+                                   (const float4*)dir, (const float4*)t, c->cg.p,
+                                   part_pt(c, kk), nslots, part_rr(c, kk + 2), kk,
+                                   part_rr(c, kk));
                 if (do_time) {
                     VPS_HIP(c, hipEventRecord(c->kev[3 * timed + 2], c->stream));
+                    timed_iter.push_back(iter_base + kk);
                     timed++;
                 }
                 spmv_launches++;
-                k++;
+                kk++;
             }
             VPS_HIP(c, hipGetLastError());
-            VPS_HIP(c, hipMemcpyAsync(c->host_cg, c->cg.p, sizeof(CgState),
-                                      hipMemcpyDeviceToHost, c->stream));
-            VPS_HIP(c, hipStreamSynchronize(c->stream));
-            hs = *c->host_cg;
+            VPS_TRY(read_state(c, &hs));
             if (hs.done == 0) {
-                if (k > max_iter + 8) {
+                if (iter_base + kk > max_iter + 8) {
                     c->err = "CG loop did not terminate";
                     return VPS_ERR_NO_CONVERGENCE;
                 }
@@ -818,35 +917,36 @@ int cg_solve(vps_context* c, vps_stats* stats) {
         if (!c->check_true) break;
         StencilArgs s = base_args(c);
         s.in = x;
-        s.out = t;  // r_true
+        s.out = t;   // r_true
+        s.part0 = part_true(c);
         launch_op<K_TRUE_RES, double, CoefD>(c, s, cd);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, c->stream,
+                           part_true(c), nslots, &c->cg.p->true_rr);
         VPS_HIP(c, hipGetLastError());
-        VPS_HIP(c, hipMemcpyAsync(c->host_cg, c->cg.p, sizeof(CgState),
-                                  hipMemcpyDeviceToHost, c->stream));
-        VPS_HIP(c, hipStreamSynchronize(c->stream));
-        hs = *c->host_cg;
+        VPS_TRY(read_state(c, &hs));
         if (hs.true_rr < hs.thr || replacements >= 4) break;
+        // restart from the true residual
         replacements++;
-        hipLaunchKernelGGL(k_replace, dim3(update_blocks), dim3(256), 0,
+        iter_base = hs.iter;
+        kk = 0;
+        hipLaunchKernelGGL(k_replace, dim3(kUpdBlocks), dim3(kUpdThreads), 0,
                            c->stream, n4, (const float4*)t, (float4*)r,
-                           (float4*)p0, k == 0 ? 1 : 0, c->cg.p,
-                           c->partials.p);
-        VPS_HIP(c, hipMemcpyAsync(c->host_cg, c->cg.p, sizeof(CgState),
-                                  hipMemcpyDeviceToHost, c->stream));
-        VPS_HIP(c, hipStreamSynchronize(c->stream));
-        hs = *c->host_cg;
-        predicted = (int)k + 4;
+                           (float4*)p0, part_repl(c));
+        hipLaunchKernelGGL(k_restart, dim3(1), dim3(1024), 0, c->stream,
+                           part_repl(c), c->cg.p);
+        VPS_HIP(c, hipGetLastError());
+        VPS_TRY(read_state(c, &hs));
+        predicted = iter_base + 4;
     }
     c->prev_iters = hs.iter;
-    // timed launches: both kernels of an iteration that ran (the done flag
-    // turns launches after convergence into no-ops; those are not counted)
+    double spmv_ms = 0, upd_ms = 0;
     int64_t counted = 0;
-    for (int64_t j = 0; j < timed && j <= hs.iter; j++) {
+    for (int64_t j = 0; j < timed; j++) {
+        if (timed_iter[(size_t)j] > hs.iter) break;   // no-op launches
         spmv_ms += ms_between(c->kev[3 * j], c->kev[3 * j + 1]);
         upd_ms += ms_between(c->kev[3 * j + 1], c->kev[3 * j + 2]);
         counted++;
     }
-    timed = counted;
     if (stats) {
         stats->cg_iterations = hs.iter;
         stats->cg_rel_residual = std::sqrt(hs.rr / hs.bb);
@@ -854,8 +954,8 @@ int cg_solve(vps_context* c, vps_stats* stats) {
             hs.true_rr >= 0 ? std::sqrt(hs.true_rr / hs.bb) : -1.0;
         stats->rhs_norm = std::sqrt(hs.bb);
         stats->spmv_launches = spmv_launches;
-        stats->ms_spmv_avg = timed ? spmv_ms / timed : 0;
-        stats->ms_update_avg = timed ? upd_ms / timed : 0;
+        stats->ms_spmv_avg = counted ? spmv_ms / counted : 0;
+        stats->ms_update_avg = counted ? upd_ms / counted : 0;
         stats->reserved[0] = replacements;
     }
     if (hs.done == 2) {

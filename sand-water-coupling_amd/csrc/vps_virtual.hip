@@ -101,8 +101,7 @@ __global__ void __launch_bounds__(NB) k_virtual_sums(
     float nvx = 0.f, nvy = 0.f, nvz = 0.f;  // sum v_j W / rho_j (real)
     float gx = 0.f, gy = 0.f, gz = 0.f;     // sum V_j gradK (real)
     float dr = 0.f;                          // sum V_j v_j . gradK (real)
-    for_each_candidate<D>(k, rcstart, qx, qy, qz, [&](int j) {
-        const double4 pj = rpos[j];
+    for_each_candidate<D>(k, rcstart, rpos, qx, qy, qz, [&](int j, const double4& pj) {
         const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz)
                                  : sq_dist2(pj.x, pj.y, qx, qy);
         if (d2 < k.h2) {
@@ -135,8 +134,7 @@ __global__ void __launch_bounds__(NB) k_virtual_sums(
     if (fabsf(w - 1e-6f) <= 1e-10f) {
         // exact FP64 decision near the threshold (VirtualParticle.cpp:56)
         double w64 = 0;
-        for_each_candidate<D>(k, rcstart, qx, qy, qz, [&](int j) {
-            const double4 pj = rpos[j];
+        for_each_candidate<D>(k, rcstart, rpos, qx, qy, qz, [&](int j, const double4& pj) {
             const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz)
                                      : sq_dist2(pj.x, pj.y, qx, qy);
             if (d2 < k.h2) w64 += kernel_w64(d2, k.inv_h64, k.c0h64);
@@ -152,8 +150,7 @@ __global__ void __launch_bounds__(NB) k_virtual_sums(
     // boundary neighbours within h
     float bnx = 0.f, bny = 0.f, bnz = 0.f;
     float hx = 0.f, hy = 0.f, hz = 0.f, db = 0.f;
-    for_each_candidate<D>(k, bcstart, qx, qy, qz, [&](int j) {
-        const double4 pj = bpos[j];
+    for_each_candidate<D>(k, bcstart, bpos, qx, qy, qz, [&](int j, const double4& pj) {
         const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz)
                                  : sq_dist2(pj.x, pj.y, qx, qy);
         if (d2 < k.h2) {
