@@ -243,7 +243,8 @@ int vps_get_scalars(vps_context* ctx, double* rest_density, double* kappa,
  * (x_j - q).squaredNorm() < radius^2 in FP64 (ParticlesNearbySearcher.h:30-34,
  * .cpp:32), radius = h for sets 0/1 and H for set 2.  Writes a CSR:
  * offsets[nq+1] and, per query, the caller-order indices (reference virtual
- * order for set 2) sorted ascending.  Returns VPS_ERR_INVALID_ARG with
+ * order for set 2) sorted ascending.  With indices == NULL only offsets and
+ * *needed are written (size query).  Returns VPS_ERR_INVALID_ARG with
  * *needed set when capacity is too small. */
 int vps_query_neighbors(vps_context* ctx, int32_t which, int64_t nq,
                         const double* q, int64_t* offsets, int64_t* indices,

@@ -237,6 +237,7 @@ static void set_for_each(const or_set* s, const double* q, or_cb cb,
     int64_t keys[27];
     int cnt = 0;
     int32_t lo[3];
+    if (s->hash.nbuckets == 0) return; /* not built: the reference's % 0 */
     quadratic_lower(&s->hash, q, lo);
     if (dim == 2) {
         for (int j = 0; j < 3; j++)
@@ -1082,6 +1083,8 @@ static int query_impl(or_sim* s, int32_t which, int64_t nq, const double* q,
                       int64_t* needed, int sorted) {
     const or_set* set = which == 0 ? &s->real : which == 1 ? &s->bnd : &s->virt;
     int64_t total = 0;
+    if (set->search_mode == OR_SEARCH_HASH && set->n > 0 && set->hash.nbuckets == 0)
+        return VPS_ERR_STATE;
     offsets[0] = 0;
     for (int64_t k = 0; k < nq; k++) {
         double qq[3] = {q[s->dim * k], q[s->dim * k + 1],

@@ -267,9 +267,11 @@ int query_neighbors(vps_context* c, int which, int64_t nq, const double* q,
     const int64_t total = offsets[nq];
     if (needed) *needed = total;
     int rc = VPS_OK;
-    if (total > cap || !idx) {
-        rc = total > cap ? VPS_ERR_INVALID_ARG : VPS_OK;
-        if (rc) c->err = "vps_query_neighbors: capacity too small";
+    if (!idx) {
+        // size query: offsets and *needed only
+    } else if (total > cap) {
+        rc = VPS_ERR_INVALID_ARG;
+        c->err = "vps_query_neighbors: capacity too small";
     } else if (total > 0) {
         VPS_TRY(dalloc(c, didx, (size_t)total));
         VPS_HIP(c, hipMemcpyAsync(doff.p, offsets, (size_t)(nq + 1) * 8,

@@ -1,0 +1,224 @@
+"""HIP path vs the FP64 oracle, through the C ABI (needs an MI355X).
+
+Bars: neighbour sets and the virtual set bit-exact;
+CG to a true relative residual below 1e-6; velocities within 1e-4 relative.
+Field tolerances below are FP32-sum bounds (about 30 ulp of FP32)."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from vps import Projection, VpsError, scenes, _abi
+
+pytestmark = pytest.mark.gpu
+
+INIT = dict(rest_density_mode=_abi.FROM_INIT, kappa_mode=_abi.FROM_INIT)
+
+
+def scene(name):
+    if name == "2d":
+        return scenes.small_2d(20)
+    if name == "3d":
+        return scenes.small_3d(12, 8, 6)
+    if name == "cfg1":
+        return scenes.config1()
+    raise KeyError(name)
+
+
+def pair(name, **kw):
+    sc = scene(name)
+    o = O.Oracle(sc, **kw)
+    o.initialize()
+    g = Projection(sc, **kw)
+    g.initialize()
+    return sc, o, g
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+
+
+def brute_sets(x, q, r2):
+    out = []
+    for p in q:
+        d = x - p
+        s = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1])
+        if x.shape[1] == 3:
+            s = s + d[:, 2] * d[:, 2]
+        out.append(np.nonzero(s < r2)[0])
+    return out
+
+
+@pytest.mark.parametrize("name", ["2d", "3d"])
+def test_neighbour_sets_bit_exact(name):
+    sc, o, g = pair(name, **INIT)
+    ob = O.Oracle(sc, search=O.SEARCH_BRUTE, **INIT)
+    ob.initialize()
+    dim = sc["dim"]
+    rng = np.random.default_rng(7)
+    x = sc["x"]
+    h = 4 * sc["particle_radius"]
+    q = np.concatenate([x[::3], rng.uniform(x.min(0) - h, x.max(0) + h, size=(300, dim))])
+    for which in (0, 1, 2):
+        og, ig = g.query(which, q)
+        oo, io = ob.query(which, q)
+        assert np.array_equal(og, oo), which
+        assert np.array_equal(ig, io), which
+    # lattice-node queries (exact |off|^2 = 16 ties for the virtual set)
+    virt = o.virtual()
+    nodes = q_nodes = np.asarray(_node_pos(sc, virt["node"][::7]))
+    og, ig = g.query(2, q_nodes)
+    oo, io = ob.query(2, q_nodes)
+    assert np.array_equal(og, oo) and np.array_equal(ig, io)
+    g.close()
+
+
+def _node_pos(sc, lin):
+    dims = sc["lattice_dims"]
+    o = sc["lattice_origin"]
+    s = sc["lattice_spacing"]
+    cx = lin % dims[0]
+    cy = (lin // dims[0]) % dims[1]
+    cols = [o[0] + cx * s, o[1] + cy * s]
+    if sc["dim"] == 3:
+        cols.append(o[2] + (lin // dims[0] // dims[1]) * s)
+    return np.stack(cols, axis=1)
+
+
+@pytest.mark.parametrize("name", ["2d", "3d", "cfg1"])
+def test_initialize_fields(name):
+    sc, o, g = pair(name, **INIT)
+    ro, vo = o.real_fields()
+    rg, vg = g.real_fields()
+    assert rel(rg, ro) < 2e-6 and rel(vg, vo) < 2e-6
+    rbo, vbo = o.boundary_fields()
+    rbg, vbg = g.boundary_fields()
+    assert rel(rbg, rbo) < 2e-6 and rel(vbg, vbo) < 2e-6
+    so, sg = o.scalars(), g.scalars()
+    assert sg["rest_density"] == pytest.approx(so["rest_density"], rel=1e-6)
+    assert sg["kappa"] == so["kappa"]
+    assert sg["alpha0"] == pytest.approx(so["alpha0"], rel=2e-6)
+    vo_, vg_ = o.virtual(), g.virtual()
+    assert np.array_equal(vo_["node"], vg_["node"])
+    assert rel(vg_["rho"], vo_["rho"]) < 2e-6
+    assert rel(vg_["vol"], vo_["vol"]) < 2e-6
+    assert rel(vg_["diag"], vo_["diag"]) < 2e-6
+    g.close()
+
+
+@pytest.mark.parametrize("op", [_abi.OP_TRANSPOSE, _abi.OP_ROW])
+@pytest.mark.parametrize("name", ["2d", "3d"])
+def test_operator_apply(name, op):
+    sc, o, g = pair(name, op_mode=op, **INIT)
+    o.advect(1e-3, 9.8)
+    g.advect(1e-3, 9.8)
+    o.projection_step()
+    g.projection_step()
+    n = len(o.virtual()["node"])
+    p = np.random.default_rng(3).normal(size=n)
+    Mo = o.apply_operator(p)
+    Mg = g.apply_operator(p)
+    rp, col, val = o.matrix()
+    absrow = np.add.reduceat(np.abs(val), rp[:-1]) * np.abs(p).max()
+    assert np.max(np.abs(Mg - Mo) / absrow) < 1e-6
+    g.close()
+
+
+@pytest.mark.parametrize("warm", [_abi.WARM_INDEX_CARRY, _abi.WARM_ZERO,
+                                  _abi.WARM_NODE_KEYED])
+@pytest.mark.parametrize("name", ["2d", "3d", "cfg1"])
+def test_projection_steps(name, warm):
+    sc, o, g = pair(name, warm_start=warm, **INIT)
+    dt = 1e-4 if name == "cfg1" else 1e-3
+    for step in range(5):
+        o.advect(dt, 9.8)
+        g.advect(dt, 9.8)
+        so = o.projection_step()
+        sg = g.projection_step()
+        vo, vg = o.virtual(), g.virtual()
+        assert np.array_equal(vo["node"], vg["node"]), step
+        dnorm = np.linalg.norm(vo["div"])
+        assert np.linalg.norm(vg["div"] - vo["div"]) / dnorm < 2e-5, step
+        assert abs(sg["cg_iterations"] - so["cg_iterations"]) <= 1, step
+        assert sg["cg_true_rel_residual"] < 1e-6, step
+        assert np.linalg.norm(vg["p"] - vo["p"]) / np.linalg.norm(vo["p"]) < 1e-4, step
+        xo, v_o = o.particles()
+        xg, v_g = g.particles()
+        assert np.linalg.norm(v_g - v_o) / np.linalg.norm(v_o) < 1e-4, step
+        assert np.abs(xg - xo).max() < 1e-9, step
+    g.close()
+
+
+def test_row_operator_step():
+    sc, o, g = pair("3d", op_mode=_abi.OP_ROW, **INIT)
+    for step in range(3):
+        o.advect(1e-3, 9.8)
+        g.advect(1e-3, 9.8)
+        so = o.projection_step()
+        sg = g.projection_step()
+        assert abs(sg["cg_iterations"] - so["cg_iterations"]) <= 1
+        _, v_o = o.particles()
+        _, v_g = g.particles()
+        assert np.linalg.norm(v_g - v_o) / np.linalg.norm(v_o) < 1e-4
+    g.close()
+
+
+def test_config_rest_density_and_kappa():
+    kw = dict(rest_density_mode=_abi.FROM_PARAMS, kappa_mode=_abi.FROM_PARAMS)
+    sc, o, g = pair("cfg1", **kw)
+    assert g.scalars()["rest_density"] == 1000.0
+    assert g.scalars()["kappa"] == sc["mass"] / 1000.0
+    for _ in range(3):
+        o.advect(1e-4, 9.8)
+        g.advect(1e-4, 9.8)
+        o.projection_step()
+        g.projection_step()
+    _, v_o = o.particles()
+    _, v_g = g.particles()
+    assert np.linalg.norm(v_g - v_o) / np.linalg.norm(v_o) < 1e-4
+    g.close()
+
+
+def test_deterministic():
+    sc = scene("3d")
+    runs = []
+    for _ in range(2):
+        g = Projection(sc, **INIT)
+        g.initialize()
+        for _ in range(3):
+            g.advect(1e-3, 9.8)
+            g.projection_step()
+        runs.append(g.particles()[1])
+        g.close()
+    assert np.array_equal(runs[0], runs[1])
+
+
+def test_error_paths():
+    sc = scene("2d")
+    g = Projection(sc, **INIT)
+    with pytest.raises(VpsError) as e:
+        g.project()
+    assert e.value.code == _abi.VPS_ERR_STATE
+    g.initialize()
+    # a velocity field with NaN is refused at the boundary of the library
+    bad = sc["v"].copy()
+    bad[3, 0] = np.nan
+    with pytest.raises(VpsError) as e:
+        g.set_particles(sc["x"], bad)
+    assert e.value.code == _abi.VPS_ERR_NONFINITE
+    g.close()
+    # empty boundary: the reference hashes into zero buckets there
+    sc0 = dict(sc, bx=np.zeros((0, 2)), bn=np.zeros((0, 2)), bv=np.zeros((0, 2)))
+    g = Projection(sc0, **INIT)
+    with pytest.raises(VpsError) as e:
+        g.initialize()
+    assert e.value.code == _abi.VPS_ERR_INVALID_ARG
+    g.close()
+    # CG iteration cap: reference exit(-1) becomes a status code
+    g = Projection(sc, max_iterations=1, **INIT)
+    g.initialize()
+    g.advect(1e-3, 9.8)
+    g.refresh_real()
+    g.generate_particles()
+    st = g.project(allow_fail=True)
+    assert st["status"] == _abi.VPS_ERR_NO_CONVERGENCE
+    g.close()
