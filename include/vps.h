@@ -69,9 +69,14 @@ enum vps_warm_start {
     VPS_WARM_ZERO = 1,         /* x0 = 0 */
     VPS_WARM_NODE_KEYED = 2,   /* x0 = previous pressure at the same lattice
                                   node, zero for nodes that are new */
-    VPS_WARM_LEARNED = 3       /* x0 from the per-node MLP (no reference
-                                  counterpart); falls back to node keyed
-                                  until the network has been trained */
+    VPS_WARM_LEARNED = 3,      /* x0 = p1 + e + dps * MLP(features), the
+                                  learned correction to the extrapolation
+                                  below (no reference counterpart); an
+                                  untrained network gives the extrapolation */
+    VPS_WARM_EXTRAPOLATE = 4   /* x0 = p1 + e: polynomial extrapolation of the
+                                  node's last three pressures, e = dp1 with two
+                                  known, 2 dp1 - dp2 with three (no reference
+                                  counterpart) */
 };
 
 /* Where rho0 and kappa come from. */
@@ -258,11 +263,15 @@ int vps_debug_apply_operator(vps_context* ctx, const double* p, double* out);
 
 /* ---- learned warm start (no reference counterpart) ------------------- */
 
-/* Online training of the warm-start MLP on (features, p) pairs produced by
- * the plain solver: after each vps_project with training enabled, one Adam
- * step (batch = min(batch, N_V)) runs on the GPU. */
+/* Online training of the warm-start MLP on the (features, solution) pairs
+ * of the converged solves: after each vps_project with training enabled
+ * (and two earlier solves with history), steps_per_solve Adam steps
+ * (beta 0.9 / 0.999, eps 1e-8) run on the GPU, each on `batch` virtual
+ * particles sampled uniformly with replacement.  Training keeps the node
+ * history even when the warm start in use is a plain one. */
 int vps_mlp_set_training(vps_context* ctx, int32_t enable, int32_t batch,
-                         double learning_rate, uint64_t seed);
+                         double learning_rate, int32_t steps_per_solve,
+                         uint64_t seed);
 
 /* Parameter I/O for checks and checkpoints: 4,801 FP32 values in the order
  * W1[64x8], b1[64], W2[64x64], b2[64], W3[1x64], b3[1] (row-major). */
@@ -280,6 +289,11 @@ int vps_mlp_loss_and_grad(vps_context* ctx, int64_t n, const float* features,
 /* Inference over given features: out[n]. */
 int vps_mlp_forward(vps_context* ctx, int64_t n, const float* features,
                     float* out);
+
+/* Loss of the last in-situ Adam step (mean squared error of the scaled
+ * pressure change on its batch, before the step) and the number of Adam
+ * steps taken (1 after vps_mlp_set_parameters on an untrained network). */
+int vps_mlp_training_state(vps_context* ctx, double* last_loss, int64_t* steps);
 
 /* Per-virtual-particle features of the current step in reference order
  * (n_virtual x 8 FP32). */

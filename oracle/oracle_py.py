@@ -80,6 +80,7 @@ def lib():
     pr("or_virtual_count", C.c_int64, [P])
     pr("or_get_virtual", C.c_int, [P, I64, D, D, D, D, D, D])
     pr("or_set_pressure", C.c_int, [P, D])
+    pr("or_get_initial_guess", C.c_int, [P, D])
     pr("or_pressure_gradient", C.c_int, [P])
     pr("or_get_real_fields", C.c_int, [P, D, D])
     pr("or_get_boundary_fields", C.c_int, [P, D, D])
@@ -195,6 +196,11 @@ class Oracle:
                               _d(out["vol"]), _d(out["vbar"]), _d(out["div"]),
                               _d(out["diag"]), _d(out["p"]))
         return out
+
+    def initial_guess(self):
+        x0 = np.zeros(self.L.or_virtual_count(self.h))
+        _check(self.L.or_get_initial_guess(self.h, _d(x0)), "or_get_initial_guess")
+        return x0
 
     def set_pressure(self, p):
         p = np.ascontiguousarray(p, dtype=np.float64)

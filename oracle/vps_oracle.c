@@ -396,8 +396,14 @@ struct or_sim {
     double* div;
     double* pres;
     int64_t npres;    /* current length of the pressure array */
-    double* prev_node_p; /* node-keyed previous pressure (nnodes) or NULL */
-    unsigned char* prev_node_valid;
+    /* node-keyed history of the last solves (nnodes each, or NULL):
+     * pressure p1, increments dp1 = p1 - p2 and dp2 = p2 - p3, and the depth
+     * h (0: node absent from the last solve, else how many are known, <= 3) */
+    double* x0;          /* CG initial guess of the last project (npres) */
+    double* prev_node_p;
+    double* prev_node_dp1;
+    double* prev_node_dp2;
+    unsigned char* prev_node_h;
     /* assembled operator M (CSR) */
     int64_t* rowptr;
     int32_t* col;
@@ -465,8 +471,11 @@ void or_destroy(or_sim* s) {
     free(s->vbar);
     free(s->div);
     free(s->pres);
+    free(s->x0);
     free(s->prev_node_p);
-    free(s->prev_node_valid);
+    free(s->prev_node_dp1);
+    free(s->prev_node_dp2);
+    free(s->prev_node_h);
     free_matrix(s);
     free(s);
 }
@@ -817,9 +826,11 @@ int or_initialize(or_sim* s) {
     /* pressures start empty (zero-filled on the first resize) */
     s->npres = 0;
     free(s->prev_node_p);
-    free(s->prev_node_valid);
-    s->prev_node_p = NULL;
-    s->prev_node_valid = NULL;
+    free(s->prev_node_dp1);
+    free(s->prev_node_dp2);
+    free(s->prev_node_h);
+    s->prev_node_p = s->prev_node_dp1 = s->prev_node_dp2 = NULL;
+    s->prev_node_h = NULL;
     s->initialized = 1;
     return 0;
 }
@@ -912,16 +923,30 @@ int or_project(or_sim* s, or_stats* st) {
     if (s->p.warm_start == VPS_WARM_INDEX_CARRY) {
         const int64_t keep = s->npres < n ? s->npres : n;
         if (s->pres) memcpy(p_new, s->pres, (size_t)keep * sizeof(double));
-    } else if (s->p.warm_start == VPS_WARM_NODE_KEYED ||
-               s->p.warm_start == VPS_WARM_LEARNED) {
+    } else if (s->p.warm_start == VPS_WARM_NODE_KEYED) {
         if (s->prev_node_p)
             for (int64_t I = 0; I < n; I++)
-                if (s->prev_node_valid[s->node[I]])
+                if (s->prev_node_h[s->node[I]])
                     p_new[I] = s->prev_node_p[s->node[I]];
+    } else if (s->p.warm_start == VPS_WARM_EXTRAPOLATE ||
+               s->p.warm_start == VPS_WARM_LEARNED) {
+        /* polynomial extrapolation through the known history (the learned
+         * start of the HIP library with an untrained network) */
+        if (s->prev_node_p)
+            for (int64_t I = 0; I < n; I++) {
+                const int64_t nd = s->node[I];
+                const int h = s->prev_node_h[nd];
+                if (h >= 1)
+                    p_new[I] = s->prev_node_p[nd] +
+                               (h >= 3 ? 2 * s->prev_node_dp1[nd] - s->prev_node_dp2[nd]
+                                       : (h == 2 ? s->prev_node_dp1[nd] : 0.));
+            }
     }
     free(s->pres);
     s->pres = p_new;
     s->npres = n;
+    s->x0 = (double*)realloc(s->x0, (size_t)(n ? n : 1) * sizeof(double));
+    memcpy(s->x0, p_new, (size_t)n * sizeof(double));
     s->div = (double*)realloc(s->div, (size_t)(n ? n : 1) * sizeof(double));
     s->vbar = (double*)realloc(s->vbar, (size_t)(n ? n : 1) * 3 * sizeof(double));
     int64_t I;
@@ -957,16 +982,28 @@ int or_project(or_sim* s, or_stats* st) {
     rc = or_cg_csr(n, s->rowptr, s->col, s->val, s->div, s->pres,
                    s->p.tolerance, s->p.max_iterations, &iters, &err);
     double t2 = now_s();
-    /* node-keyed record of this step's pressures */
+    /* node-keyed history: this solve's pressures, their increments over
+     * the previous solve where the node was in it, depth h + 1 (at most 3);
+     * nodes absent from this solve lose their history */
     if (!s->prev_node_p) {
         s->prev_node_p = (double*)calloc((size_t)s->nnodes, sizeof(double));
-        s->prev_node_valid = (unsigned char*)calloc((size_t)s->nnodes, 1);
-    } else {
-        memset(s->prev_node_valid, 0, (size_t)s->nnodes);
+        s->prev_node_dp1 = (double*)calloc((size_t)s->nnodes, sizeof(double));
+        s->prev_node_dp2 = (double*)calloc((size_t)s->nnodes, sizeof(double));
+        s->prev_node_h = (unsigned char*)calloc((size_t)s->nnodes, 1);
     }
-    for (I = 0; I < n; I++) {
-        s->prev_node_p[s->node[I]] = s->pres[I];
-        s->prev_node_valid[s->node[I]] = 1;
+    {
+        unsigned char* hnew = (unsigned char*)calloc((size_t)s->nnodes, 1);
+        for (I = 0; I < n; I++) {
+            const int64_t nd = s->node[I];
+            const int h = s->prev_node_h[nd];
+            const double dp1 = h >= 1 ? s->pres[I] - s->prev_node_p[nd] : 0.;
+            s->prev_node_dp2[nd] = h >= 2 ? s->prev_node_dp1[nd] : 0.;
+            s->prev_node_dp1[nd] = dp1;
+            s->prev_node_p[nd] = s->pres[I];
+            hnew[nd] = (unsigned char)(h + 1 > 3 ? 3 : h + 1);
+        }
+        free(s->prev_node_h);
+        s->prev_node_h = hnew;
     }
     pressure_gradient(s);
     double t3 = now_s();
@@ -1028,6 +1065,12 @@ int or_get_virtual(or_sim* s, int64_t* node, double* rho, double* vol,
         if (diag && s->diag) diag[I] = s->diag[I];
         if (p && s->pres && I < s->npres) p[I] = s->pres[I];
     }
+    return 0;
+}
+
+int or_get_initial_guess(or_sim* s, double* x0) {
+    if (!s->x0) return VPS_ERR_STATE;
+    memcpy(x0, s->x0, (size_t)s->npres * sizeof(double));
     return 0;
 }
 

@@ -57,6 +57,8 @@ int64_t or_virtual_count(or_sim* s);
 int or_get_virtual(or_sim* s, int64_t* node, double* rho, double* vol,
                    double* vbar, double* div, double* diag, double* p);
 int or_set_pressure(or_sim* s, const double* p);
+/* CG initial guess of the last project, in virtual order */
+int or_get_initial_guess(or_sim* s, double* x0);
 /* applyPressureGradient with the current pressures (updates velocities) */
 int or_pressure_gradient(or_sim* s);
 int or_get_real_fields(or_sim* s, double* rho, double* vol);

@@ -173,6 +173,7 @@ void free_all(vps_context* c) {
     dfree(c->vt);
     dfree(c->vq);
     dfree(c->vdiag64);
+    dfree(c->vrowsum);
     dfree(c->vrank);
     dfree(c->bits);
     dfree(c->word_prefix);
@@ -243,7 +244,7 @@ int vps_create(const vps_params* params, vps_context** out) {
     }
     if (p.op_mode != VPS_OP_TRANSPOSE && p.op_mode != VPS_OP_ROW)
         return fail(nullptr, VPS_ERR_INVALID_ARG, "unknown op_mode");
-    if (p.warm_start < VPS_WARM_INDEX_CARRY || p.warm_start > VPS_WARM_LEARNED)
+    if (p.warm_start < VPS_WARM_INDEX_CARRY || p.warm_start > VPS_WARM_EXTRAPOLATE)
         return fail(nullptr, VPS_ERR_INVALID_ARG, "unknown warm_start");
     if (p.rest_density_mode == VPS_FROM_PARAMS && !(p.rest_density > 0))
         return fail(nullptr, VPS_ERR_INVALID_ARG, "rest_density must be > 0");
@@ -288,6 +289,7 @@ int vps_create(const vps_params* params, vps_context** out) {
     ALLOC(c->vb, cap);
     ALLOC(c->vdiag, cap);
     ALLOC(c->vdiag64, cap);
+    ALLOC(c->vrowsum, cap);
     ALLOC(c->vr, cap);
     ALLOC(c->vt, cap);
     ALLOC(c->vq, cap);
@@ -321,6 +323,7 @@ int vps_create(const vps_params* params, vps_context** out) {
             hipMemset(c->vb.p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->vdiag.p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->vdiag64.p, 0, cap * 8) != hipSuccess ||
+            hipMemset(c->vrowsum.p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->vr.p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->vt.p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->vx[0].p, 0, cap * 4) != hipSuccess ||
@@ -557,9 +560,11 @@ static int project_impl(vps_context* c, vps_stats* st, int e0) {
     VPS_HIP(c, hipEventRecord(c->ev[e0 + 1], c->stream));
     const int w = c->which;
     float* x0 = c->vx[w].p;
-    int mode = c->p.warm_start;
-    if (mode == VPS_WARM_LEARNED) {
-        VPS_TRY(mlp_warm_start(c, x0));
+    const int mode = c->p.warm_start;
+    // the history of pressures (and the learned start) is kept only when
+    // the network is used or trained
+    if (mlp_active(c)) {
+        VPS_TRY(mlp_warm_start(c, mode, x0));
     } else {
         VPS_TRY(warm_start(c, mode, x0));
     }
@@ -567,11 +572,13 @@ static int project_impl(vps_context* c, vps_stats* st, int e0) {
     vps_stats* s = st ? st : &local;
     const int rc = cg_solve(c, s);
     if (rc != VPS_OK && rc != VPS_ERR_NO_CONVERGENCE) return rc;
+    // history and training read this solve's pressures before the slot map
+    // changes, so they run before after_solve flips the buffers
+    if (mlp_active(c) && rc == VPS_OK) VPS_TRY(mlp_after_solve(c, s));
     VPS_TRY(after_solve(c));
     VPS_HIP(c, hipEventRecord(c->ev[e0 + 2], c->stream));
     if (rc == VPS_OK) VPS_TRY(pressure_gradient(c));
     VPS_HIP(c, hipEventRecord(c->ev[e0 + 3], c->stream));
-    VPS_TRY(mlp_after_solve(c));
     return rc;
 }
 

@@ -313,7 +313,7 @@ struct vps_context {
     vps::DBuf<int32_t> brick_slot[2];
     vps::DBuf<int32_t> slot_brick[2];
     // per-node fields, slot-major
-    vps::DBuf<float> vrho, vvol, vb, vdiag, vx[2], vr, vp[2], vt, vq;
+    vps::DBuf<float> vrho, vvol, vb, vdiag, vx[2], vr, vp[2], vt, vq, vrowsum;
     vps::DBuf<double> vdiag64;
     vps::DBuf<int32_t> vrank;
     // lattice bitmask for reference (x-fastest) ranks
@@ -411,7 +411,8 @@ int after_solve(vps_context* c);
 // learned warm start
 int mlp_create(vps_context* c);
 void mlp_destroy(vps_context* c);
-int mlp_warm_start(vps_context* c, float* x0);
-int mlp_after_solve(vps_context* c);
+int mlp_warm_start(vps_context* c, int mode, float* x0);
+int mlp_after_solve(vps_context* c, const vps_stats* st);
+bool mlp_active(const vps_context* c);
 
 }  // namespace vps

@@ -1,41 +1,946 @@
-// vps_mlp.hip -- learned CG warm start (placeholder until the MFMA kernels
-// land): reports that the network is unavailable.
+// vps_mlp.hip -- learned CG warm start (no counterpart in the reference).
+//
+// Every lattice node keeps the history of its pressure over the last three
+// solves: p1 (last), dp1 = p1 - p2, dp2 = p2 - p3, and the depth h (how many
+// of them are known; zero for a node absent from the last solve).  The
+// polynomial extrapolation through the known values is the prior
+//     e = 0 (h <= 1),  dp1 (h = 2),  2 dp1 - dp2 (h = 3)   (p_next ~ p1 + e)
+// and a per-virtual-particle MLP 8 -> 64 -> 64 -> 1 (GELU, BF16 weights and
+// activations, FP32 accumulation) learns the correction to it:
+//     x0_I = p1_I + e_I + tau * net(f_I)     (nodes with h >= 1)
+//     x0_I = 0                               (nodes that are new)
+// with features
+//     f0 = p1 / ps, f1 = dp1 / dps, f2 = dp2 / dps    the node history
+//     f3 = D_I / ds                                the CG right-hand side
+//     f4 = diag_I / alpha0 - 1, f5 = s_I / alpha0 - 1   clamped / unclamped
+//          Laplacian diagonal (s < alpha0 marks the free surface)
+//     f6 = V_I / (m / rho0) - 1                    virtual volume
+//     f7 = history depth (0, 0.5, 1 for h = 1, 2, 3)
+// ps = RMS of the last solution, dps = RMS of its increment dp1 (over nodes
+// with h >= 1), ds = RMS of the last right-hand side, tau = RMS of the
+// extrapolation's own error at the last solve (over nodes with h >= 1): the
+// network's output is in units of the error it corrects, so its BF16 and
+// Adam-step noise stays far below that error as the flow settles.  The
+// scales live on the device and are refreshed after every solve, so no host
+// round trip is added.  The output layer starts at zero: an untrained network gives the
+// extrapolation (VPS_WARM_EXTRAPOLATE) exactly.  CG always runs to the same
+// tolerance, so the warm start changes the iteration count, not the answer.
+//
+// Kernels (all matrix work on MFMA v_mfma_f32_16x16x16_bf16, 64-lane
+// waves, tiles of 16 nodes):
+//   forward: activations are kept transposed (hidden x node), so the C
+//     layout of one layer's MFMA output is exactly the B operand layout of
+//     the next layer -- no data movement between layers;
+//   backward: dH1 = W2^T dZ2 on MFMA from the same layouts; the weight
+//     gradients dW2 = dZ2 H1^T and dW1 = dZ1 X^T contract over nodes, which
+//     needs the transposed layout: each wave stages its bf16 tiles through
+//     LDS (8 KB) and reads them back with the lane<->row swap;
+//   per-wave gradient partials are reduced in a fixed order, then Adam.
+// MFMA lane layouts (16x16x16): A lane l holds A[l%16][4(l/16)+j], B lane l
+// holds B[4(l/16)+j][l%16], C lane l holds C[4(l/16)+i][l%16].
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "vps_internal.h"
 
 namespace vps {
-int mlp_create(vps_context*) { return VPS_OK; }
-void mlp_destroy(vps_context*) {}
-int mlp_warm_start(vps_context* c, float* x0) {
-    return warm_start(c, VPS_WARM_NODE_KEYED, x0);
+namespace {
+
+constexpr int kIn = 8;
+constexpr int kHid = 64;
+constexpr int OFF_W1 = 0;                       // W1[64][8]
+constexpr int OFF_B1 = OFF_W1 + kHid * kIn;     // 512
+constexpr int OFF_W2 = OFF_B1 + kHid;           // 576, W2[64][64] (out, in)
+constexpr int OFF_B2 = OFF_W2 + kHid * kHid;    // 4672
+constexpr int OFF_W3 = OFF_B2 + kHid;           // 4736, W3[64]
+constexpr int OFF_B3 = OFF_W3 + kHid;           // 4800
+constexpr int kNParams = OFF_B3 + 1;            // 4801
+// device scales: inverses of ps, dps, ds, tau; tau; dps; ps
+enum {
+    SC_INV_PS = 0, SC_INV_DPS = 1, SC_INV_DS = 2, SC_INV_TAU = 3, SC_TAU = 4,
+    SC_DPS = 5, SC_PS = 6, SC_N = 8
+};
+constexpr int kTrainWaves = 256;
+constexpr int kInferBlocks = 1024;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
+    return (short)(u >> 16);
 }
-int mlp_after_solve(vps_context*) { return VPS_OK; }
+__device__ __forceinline__ float bf2f(short b) {
+    return __uint_as_float(((uint32_t)(uint16_t)b) << 16);
+}
+__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ float gelu(float z) {
+    return 0.5f * z * (1.f + erff(z * 0.70710678118654752f));
+}
+__device__ __forceinline__ float gelu_d(float z) {
+    return 0.5f * (1.f + erff(z * 0.70710678118654752f)) +
+           z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+struct Wregs {
+    bf16x4 w1[4];        // A tiles of W1 (hidden x feature), per m-tile
+    bf16x4 w2[4][4];     // A tiles of W2 [m-tile out][k-tile in]
+    bf16x4 w3[4];        // A tiles of W3 (row 0 only), per k-tile
+    f32x4 b1[4], b2[4];  // biases in C layout
+    float b3;
+};
+
+template <bool W2REG>
+__device__ __forceinline__ void load_weights(const float* P, Wregs& W) {
+    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = 4 * q + j;
+            W.w1[mt][j] = k < kIn ? f2bf(P[OFF_W1 + (mt * 16 + r) * kIn + k]) : (short)0;
+            W.b1[mt][j] = P[OFF_B1 + mt * 16 + 4 * q + j];
+            W.b2[mt][j] = P[OFF_B2 + mt * 16 + 4 * q + j];
+            W.w3[mt][j] = r == 0 ? f2bf(P[OFF_W3 + mt * 16 + 4 * q + j]) : (short)0;
+        }
+        if constexpr (W2REG) {
+#pragma unroll
+            for (int kt = 0; kt < 4; kt++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    W.w2[mt][kt][j] =
+                        f2bf(P[OFF_W2 + (mt * 16 + r) * kHid + kt * 16 + 4 * q + j]);
+        }
+    }
+    W.b3 = P[OFF_B3];
+}
+
+struct Fwd {
+    f32x4 z1[4], z2[4];
+    bf16x4 h1[4], h2[4];
+    float out;   // lanes 0..15: output of node l
+};
+
+// W2 A tile (m-tile mo, k-tile kt) read from an LDS copy W2[out][in] (bf16)
+__device__ __forceinline__ bf16x4 w2_tile(const uint16_t* sW, int mo, int kt) {
+    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+    const uint16_t* p = sW + (mo * 16 + r) * kHid + kt * 16 + 4 * q;
+    bf16x4 v;
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = (short)p[j];
+    return v;
+}
+
+template <bool LDSW2>
+__device__ __forceinline__ void forward(const Wregs& W, const uint16_t* sW2,
+                                        bf16x4 xb, Fwd& F) {
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        F.z1[mt] = mfma(W.w1[mt], xb, W.b1[mt]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) F.h1[mt][i] = f2bf(gelu(F.z1[mt][i]));
+    }
+#pragma unroll
+    for (int mo = 0; mo < 4; mo++) {
+        f32x4 acc = W.b2[mo];
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++) {
+            if constexpr (LDSW2) acc = mfma(w2_tile(sW2, mo, kt), F.h1[kt], acc);
+            else acc = mfma(W.w2[mo][kt], F.h1[kt], acc);
+        }
+        F.z2[mo] = acc;
+#pragma unroll
+        for (int i = 0; i < 4; i++) F.h2[mo][i] = f2bf(gelu(acc[i]));
+    }
+    f32x4 z3 = {W.b3, W.b3, W.b3, W.b3};
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++) z3 = mfma(W.w3[kt], F.h2[kt], z3);
+    F.out = z3[0];
+}
+
+// Feature sources: the current step's slot-major fields.
+struct FeatSrc {
+    const float* p1;
+    const float* dp1;
+    const float* dp2;
+    const uint8_t* hh;
+    const float* vb;
+    const float* vdiag;
+    const float* vrow;
+    const float* vvol;
+    const float* scale;   // device scales (SC_*)
+    float inv_a0, inv_vs;
+};
+
+// Polynomial extrapolation of the node history (the prior of the network).
+__device__ __forceinline__ float prior(int h, float dp1, float dp2) {
+    return h >= 3 ? 2.f * dp1 - dp2 : (h == 2 ? dp1 : 0.f);
+}
+
+__device__ __forceinline__ float feature(const FeatSrc& s, int64_t g, int f) {
+    switch (f) {
+        case 0: return s.p1[g] * s.scale[SC_INV_PS];
+        case 1: return s.dp1[g] * s.scale[SC_INV_DPS];
+        case 2: return s.dp2[g] * s.scale[SC_INV_DPS];
+        case 3: return s.vb[g] * s.scale[SC_INV_DS];
+        case 4: return s.vdiag[g] * s.inv_a0 - 1.f;
+        case 5: return s.vrow[g] * s.inv_a0 - 1.f;
+        case 6: return s.vvol[g] * s.inv_vs - 1.f;
+        default: {
+            const int h = s.hh[g];
+            return h >= 3 ? 1.f : (h == 2 ? 0.5f : 0.f);
+        }
+    }
+}
+
+// B operand (feature x node) of the 16-node tile: lane l carries features
+// 4(l/16)..4(l/16)+3 of node l%16 (zero beyond the 8 features).
+template <bool DIRECT>
+__device__ __forceinline__ bf16x4 tile_features(const FeatSrc& s,
+                                                const float* direct, int64_t n,
+                                                int64_t node, bool ok,
+                                                float fo[4]) {
+    const int q = (threadIdx.x & 63) >> 4;
+    bf16x4 xb = {0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        float v = 0.f;
+        if (ok && q < 2) {
+            const int f = 4 * q + j;
+            if constexpr (DIRECT) v = direct[node * kIn + f];
+            else v = feature(s, node, f);
+        }
+        fo[j] = v;
+        xb[j] = f2bf(v);
+    }
+    (void)n;
+    return xb;
+}
+
+// ---- inference: x0 = p1 + e + tau * net(f) ------------------------------
+template <bool DIRECT>
+__global__ void __launch_bounds__(256) k_mlp_infer(const float* P, FeatSrc fs,
+                                                   const float* direct,
+                                                   int64_t nn, float* out) {
+    Wregs W;
+    load_weights<true>(P, W);
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t nwaves = (int64_t)gridDim.x * 4;
+    const int64_t ntiles = (nn + 15) / 16;
+    for (int64_t tile = wave; tile < ntiles; tile += nwaves) {
+        const int64_t g = tile * 16 + (lane & 15);
+        const bool ok = g < nn;
+        float fo[4];
+        const bf16x4 xb = tile_features<DIRECT>(fs, direct, nn, g, ok, fo);
+        Fwd F;
+        forward<false>(W, nullptr, xb, F);
+        if (lane < 16 && ok) {
+            if constexpr (DIRECT) {
+                out[g] = F.out;
+            } else {
+                const int h = fs.hh[g];
+                const bool has = fs.vvol[g] > 0.f && h >= 1;
+                out[g] = has ? fs.p1[g] + prior(h, fs.dp1[g], fs.dp2[g]) +
+                                   fs.scale[SC_TAU] * F.out
+                             : 0.f;
+            }
+        }
+    }
+}
+
+// x0 = p1 + e (the untrained network's answer, without the MLP).
+__global__ void __launch_bounds__(256) k_extrapolate(FeatSrc fs, int64_t nn,
+                                                     float* out) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= nn) return;
+    const int h = fs.hh[g];
+    out[g] = fs.vvol[g] > 0.f && h >= 1 ? fs.p1[g] + prior(h, fs.dp1[g], fs.dp2[g])
+                                        : 0.f;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// ---- training: forward + backward over sampled nodes ---------------------
+// Per wave: tiles_per_wave tiles of 16 samples.  Writes the wave's gradient
+// (kNParams floats) and loss partial.  Loss = sum (out - t)^2 / nsamples.
+template <bool DIRECT>
+__global__ void __launch_bounds__(256) k_mlp_train(
+    const float* P, FeatSrc fs, const float* vx, const int32_t* slot_of_rank,
+    int64_t nv, const float* fdirect, const float* tdirect, int64_t nsamples,
+    int tiles_per_wave, uint64_t seed, float* grad_part, double* loss_part) {
+    __shared__ uint16_t s_dz2[4][kHid * 16];
+    __shared__ uint16_t s_h1[4][kHid * 16];
+    __shared__ uint16_t s_dz1[4][kHid * 16];
+    __shared__ uint16_t s_x[4][16 * 16];
+    __shared__ uint16_t sW2[kHid * kHid];    // W2[out][in]
+    __shared__ uint16_t sW2T[kHid * kHid];   // W2^T[in][out]
+    const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+    const int wl = threadIdx.x >> 6;
+    const int64_t wave = (int64_t)blockIdx.x * 4 + wl;
+    for (int e = threadIdx.x; e < kHid * kHid; e += 256) {
+        const uint16_t b = (uint16_t)f2bf(P[OFF_W2 + e]);
+        sW2[e] = b;
+        sW2T[(e % kHid) * kHid + e / kHid] = b;
+    }
+    Wregs W;
+    load_weights<false>(P, W);
+    // W3 in C layout
+    f32x4 w3c[4];
+#pragma unroll
+    for (int mi = 0; mi < 4; mi++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) w3c[mi][i] = bf2f(f2bf(P[OFF_W3 + mi * 16 + 4 * q + i]));
+    __syncthreads();
+    f32x4 gW2[4][4], gW1[4], gb1[4], gb2[4], gW3[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        gW1[a] = {0, 0, 0, 0};
+        gb1[a] = {0, 0, 0, 0};
+        gb2[a] = {0, 0, 0, 0};
+        gW3[a] = {0, 0, 0, 0};
+#pragma unroll
+        for (int b = 0; b < 4; b++) gW2[a][b] = {0, 0, 0, 0};
+    }
+    float gb3 = 0.f;
+    double loss = 0;
+    const float inv_n = 1.f / (float)nsamples;
+    for (int it = 0; it < tiles_per_wave; it++) {
+        const int64_t k = (wave * tiles_per_wave + it) * 16 + r;   // sample id
+        bool ok = k < nsamples;
+        int64_t node = k;
+        float target = 0.f;
+        if constexpr (DIRECT) {
+            if (ok) target = tdirect[k];
+        } else {
+            // uniform over the virtual particles of this solve
+            node = ok ? slot_of_rank[mix64(seed ^ (uint64_t)k) % (uint64_t)nv] : 0;
+            const int h = fs.hh[node];
+            ok = ok && h >= 1;
+            if (ok)
+                target = (vx[node] - fs.p1[node] -
+                          prior(h, fs.dp1[node], fs.dp2[node])) *
+                         fs.scale[SC_INV_TAU];
+        }
+        float fo[4];
+        const bf16x4 xb = tile_features<DIRECT>(fs, fdirect, nsamples, node, ok, fo);
+        Fwd F;
+        forward<true>(W, sW2, xb, F);
+        // dL/dout for the 16 nodes (lanes 0..15), broadcast to the columns
+        float e = (lane < 16 && ok) ? F.out - target : 0.f;
+        if (lane < 16) loss += (double)e * (double)e;
+        const float dout = 2.f * e * inv_n;
+        const float d = __shfl(dout, r, 64);
+        if (lane < 16) gb3 += d;
+        // layer 3
+        f32x4 dz2[4];
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                gW3[kt][i] += bf2f(F.h2[kt][i]) * d;
+                dz2[kt][i] = w3c[kt][i] * d * gelu_d(F.z2[kt][i]);
+                gb2[kt][i] += dz2[kt][i];
+            }
+        bf16x4 dz2b[4];
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) dz2b[kt][i] = f2bf(dz2[kt][i]);
+        // dH1 = W2^T dZ2; dZ1 = dH1 * gelu'(Z1)
+        f32x4 dz1[4];
+#pragma unroll
+        for (int mi = 0; mi < 4; mi++) {
+            f32x4 acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int kt = 0; kt < 4; kt++) acc = mfma(w2_tile(sW2T, mi, kt), dz2b[kt], acc);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                dz1[mi][i] = acc[i] * gelu_d(F.z1[mi][i]);
+                gb1[mi][i] += dz1[mi][i];
+            }
+        }
+        // stage dZ2, H1, dZ1 (hidden x node) and X (feature x node) in LDS
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int row = t * 16 + 4 * q + i;
+                s_dz2[wl][row * 16 + r] = (uint16_t)dz2b[t][i];
+                s_h1[wl][row * 16 + r] = (uint16_t)F.h1[t][i];
+                s_dz1[wl][row * 16 + r] = (uint16_t)f2bf(dz1[t][i]);
+            }
+#pragma unroll
+        for (int j = 0; j < 4; j++) s_x[wl][(4 * q + j) * 16 + r] = (uint16_t)xb[j];
+        __syncthreads();
+        // node-contracted products: A(hidden x node), B(node x hidden/feature)
+        bf16x4 a_dz2[4], b_h1[4], a_dz1[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                a_dz2[t][j] = (short)s_dz2[wl][(t * 16 + r) * 16 + 4 * q + j];
+                b_h1[t][j] = (short)s_h1[wl][(t * 16 + r) * 16 + 4 * q + j];
+                a_dz1[t][j] = (short)s_dz1[wl][(t * 16 + r) * 16 + 4 * q + j];
+            }
+        bf16x4 b_x;
+#pragma unroll
+        for (int j = 0; j < 4; j++) b_x[j] = (short)s_x[wl][r * 16 + 4 * q + j];
+        __syncthreads();
+#pragma unroll
+        for (int mo = 0; mo < 4; mo++)
+#pragma unroll
+            for (int mi = 0; mi < 4; mi++)
+                gW2[mo][mi] = mfma(a_dz2[mo], b_h1[mi], gW2[mo][mi]);
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) gW1[mt] = mfma(a_dz1[mt], b_x, gW1[mt]);
+    }
+    // reduce the per-lane bias / W3 partials over the 16 node lanes
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                gb1[t][i] += __shfl_xor(gb1[t][i], o, 64);
+                gb2[t][i] += __shfl_xor(gb2[t][i], o, 64);
+                gW3[t][i] += __shfl_xor(gW3[t][i], o, 64);
+            }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        gb3 += __shfl_xor(gb3, o, 64);
+        loss += __shfl_xor(loss, o, 64);
+    }
+    float* G = grad_part + wave * kNParams;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = t * 16 + 4 * q + i;
+            if (r < kIn) G[OFF_W1 + row * kIn + r] = gW1[t][i];
+            if (r == 0) {
+                G[OFF_B1 + row] = gb1[t][i];
+                G[OFF_B2 + row] = gb2[t][i];
+                G[OFF_W3 + row] = gW3[t][i];
+            }
+#pragma unroll
+            for (int mi = 0; mi < 4; mi++) G[OFF_W2 + row * kHid + mi * 16 + r] = gW2[t][mi][i];
+        }
+    if (lane == 0) {
+        G[OFF_B3] = gb3;
+        loss_part[wave] = loss * (double)inv_n;
+    }
+}
+
+// Sum of the per-wave partials (fixed order), optional Adam step.
+__global__ void __launch_bounds__(256) k_mlp_reduce(
+    const float* grad_part, int nwaves, const double* loss_part, float* grad,
+    double* loss, float* P, float* m, float* v, int adam, float lr, float b1t,
+    float b2t) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p == 0) {
+        double s = 0;
+        for (int w = 0; w < nwaves; w++) s += loss_part[w];
+        *loss = s;
+    }
+    if (p >= kNParams) return;
+    float g = 0.f;
+    for (int w = 0; w < nwaves; w++) g += grad_part[(int64_t)w * kNParams + p];
+    grad[p] = g;
+    if (adam) {
+        const float mm = 0.9f * m[p] + 0.1f * g;
+        const float vv = 0.999f * v[p] + 0.001f * g * g;
+        m[p] = mm;
+        v[p] = vv;
+        const float mh = mm / (1.f - b1t), vh = vv / (1.f - b2t);
+        P[p] -= lr * mh / (sqrtf(vh) + 1e-8f);
+    }
+}
+
+// Node history in the current slot layout, gathered from the last solve.
+__global__ void __launch_bounds__(256) k_history_gather(
+    int64_t nn, int NB, const int32_t* brick_of_slot, const float* vvol,
+    const int32_t* prev_slot_of_brick, const float* P_prev, const float* DP_prev,
+    const float* DP2_prev, const uint8_t* H_prev, float* hp1, float* hdp1,
+    float* hdp2, uint8_t* hh) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= nn) return;
+    float a = 0.f, b = 0.f, c = 0.f;
+    int h = 0;
+    if (prev_slot_of_brick && vvol[g] > 0.f) {
+        const int slot = (int)(g / NB), loc = (int)(g % NB);
+        const int ps = prev_slot_of_brick[brick_of_slot[slot]];
+        if (ps >= 0) {
+            const int64_t pg = (int64_t)ps * NB + loc;
+            h = H_prev[pg];
+            if (h >= 1) a = P_prev[pg];
+            if (h >= 2) b = DP_prev[pg];
+            if (h >= 3) c = DP2_prev[pg];
+        }
+    }
+    hp1[g] = a;
+    hdp1[g] = b;
+    hdp2[g] = c;
+    hh[g] = (uint8_t)h;
+}
+
+// After the solve: history of this layout, the rank -> slot map of the
+// training sampler, and per-block partials (x.x, dp1.dp1, r.r, count) for
+// the scales of the next step, with r = x - p1 - e the extrapolation error
+// and count the nodes with h >= 1.
+constexpr int kNPart = 4;
+__global__ void __launch_bounds__(256) k_history_store(
+    int64_t nn, const float* vvol, const int32_t* vrank, const float* x,
+    const float* hp1, const float* hdp1, const float* hdp2, const uint8_t* hh,
+    float* DP, float* DP2, uint8_t* H, int32_t* slot_of_rank, double* part) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double a[kNPart] = {0, 0, 0, 0};
+    if (g < nn) {
+        const bool on = vvol[g] > 0.f;
+        const int h = hh[g];
+        const float xv = x[g];
+        const float dp = on && h >= 1 ? xv - hp1[g] : 0.f;
+        H[g] = on ? (uint8_t)(h + 1 > 3 ? 3 : h + 1) : (uint8_t)0;
+        DP[g] = dp;
+        DP2[g] = on && h >= 2 ? hdp1[g] : 0.f;
+        if (on) {
+            slot_of_rank[vrank[g]] = (int32_t)g;
+            a[0] = (double)xv * (double)xv;
+            if (h >= 1) {
+                const double r = (double)(dp - prior(h, hdp1[g], hdp2[g]));
+                a[1] = (double)dp * (double)dp;
+                a[2] = r * r;
+                a[3] = 1;
+            }
+        }
+    }
+    __shared__ double sw[kNPart][4];
+#pragma unroll
+    for (int j = 0; j < kNPart; j++) {
+        a[j] = wave_sum(a[j]);
+        if ((threadIdx.x & 63) == 0) sw[j][threadIdx.x >> 6] = a[j];
+    }
+    __syncthreads();
+    if (threadIdx.x < kNPart) {
+        const double* w = sw[threadIdx.x];
+        part[(int64_t)blockIdx.x * kNPart + threadIdx.x] = (w[0] + w[1]) + (w[2] + w[3]);
+    }
+}
+
+// Sums the block partials (fixed order) and writes the scales of the next
+// step: ps = sqrt(x.x / nv), dps = sqrt(dp.dp / count), tau = sqrt(r.r /
+// count), ds = sqrt(b.b / nv).
+__global__ void __launch_bounds__(1024) k_scales(const double* part, int nblk,
+                                                 const CgState* st, int64_t nv,
+                                                 float* scale) {
+    __shared__ double sw[kNPart][16];
+    double a[kNPart] = {0, 0, 0, 0};
+    for (int i = threadIdx.x; i < nblk; i += 1024)
+#pragma unroll
+        for (int j = 0; j < kNPart; j++) a[j] += part[(int64_t)i * kNPart + j];
+#pragma unroll
+    for (int j = 0; j < kNPart; j++) {
+        a[j] = wave_sum(a[j]);
+        if ((threadIdx.x & 63) == 0) sw[j][threadIdx.x >> 6] = a[j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[kNPart] = {0, 0, 0, 0};
+        for (int j = 0; j < kNPart; j++)
+            for (int i = 0; i < 16; i++) t[j] += sw[j][i];
+        const double inv_nv = nv > 0 ? 1.0 / (double)nv : 0.0;
+        const double ps = sqrt(t[0] * inv_nv);
+        const double dps = t[3] > 0 ? sqrt(t[1] / t[3]) : 0.0;
+        const double tau = t[3] > 0 ? sqrt(t[2] / t[3]) : 0.0;
+        const double ds = sqrt(st->bb * inv_nv);
+        scale[SC_INV_PS] = ps > 0 ? (float)(1.0 / ps) : 0.f;
+        scale[SC_INV_DPS] = dps > 0 ? (float)(1.0 / dps) : 0.f;
+        scale[SC_INV_DS] = ds > 0 ? (float)(1.0 / ds) : 0.f;
+        scale[SC_INV_TAU] = tau > 0 ? (float)(1.0 / tau) : 0.f;
+        scale[SC_TAU] = (float)tau;
+        scale[SC_DPS] = (float)dps;
+        scale[SC_PS] = (float)ps;
+    }
+}
+
+// Features of every slot node (read-back for tests).
+__global__ void __launch_bounds__(256) k_features(FeatSrc fs, int64_t nn,
+                                                  float* out) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= nn) return;
+    for (int f = 0; f < kIn; f++) out[g * kIn + f] = feature(fs, g, f);
+}
+
+inline unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+struct MlpState {
+    DBuf<float> P, m, v, grad, grad_part;
+    DBuf<double> loss_part, loss;
+    DBuf<float> scale[2];        // SC_* of the current / next step
+    int cur_scale = 0;           // scales of the next warm start
+    int step_scale = 0;          // scales used by the last warm start
+    DBuf<float> DP[2], DP2[2];
+    DBuf<uint8_t> H[2];
+    DBuf<float> hp1, hdp1, hdp2;
+    DBuf<uint8_t> hh;
+    DBuf<int32_t> slot_of_rank;
+    DBuf<double> part;
+    bool training = false;
+    int batch = 65536;
+    int steps_per_solve = 1;
+    double lr = 1e-3;
+    uint64_t seed = 1;
+    int64_t adam_t = 0;
+    bool have_history = false;   // history arrays valid for last_solve_buf
+    int hist_solves = 0;         // consecutive solves with history kept
+};
+
+static MlpState* S(vps_context* c) { return (MlpState*)c->mlp; }
+
+static void init_params(std::vector<float>& p, uint64_t seed) {
+    p.assign(kNParams, 0.f);
+    uint64_t s = seed;
+    auto uni = [&](float a) {
+        const double u = (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0);
+        return (float)((2 * u - 1) * a);
+    };
+    const float a1 = std::sqrt(6.f / (kIn + kHid)), a2 = std::sqrt(6.f / (2 * kHid));
+    for (int i = 0; i < kHid * kIn; i++) p[OFF_W1 + i] = uni(a1);
+    for (int i = 0; i < kHid * kHid; i++) p[OFF_W2 + i] = uni(a2);
+    // output layer zero: the untrained network predicts "no change"
+}
+
+int mlp_create(vps_context* c) {
+    MlpState* m = new MlpState();
+    c->mlp = m;
+    VPS_TRY(dalloc(c, m->P, kNParams));
+    VPS_TRY(dalloc(c, m->m, kNParams));
+    VPS_TRY(dalloc(c, m->v, kNParams));
+    VPS_TRY(dalloc(c, m->grad, kNParams));
+    VPS_TRY(dalloc(c, m->grad_part, (size_t)kTrainWaves * kNParams));
+    VPS_TRY(dalloc(c, m->loss_part, kTrainWaves));
+    VPS_TRY(dalloc(c, m->loss, 1));
+    for (int b = 0; b < 2; b++) {
+        VPS_TRY(dalloc(c, m->scale[b], SC_N));
+        VPS_HIP(c, hipMemset(m->scale[b].p, 0, SC_N * sizeof(float)));
+    }
+    VPS_HIP(c, hipMemset(m->loss.p, 0, sizeof(double)));
+    std::vector<float> p;
+    init_params(p, 0x5EEDull);
+    VPS_HIP(c, hipMemcpy(m->P.p, p.data(), kNParams * 4, hipMemcpyHostToDevice));
+    VPS_HIP(c, hipMemset(m->m.p, 0, kNParams * 4));
+    VPS_HIP(c, hipMemset(m->v.p, 0, kNParams * 4));
+    return VPS_OK;
+}
+
+// History arrays are allocated on first use (the plain warm starts never
+// touch them).
+static int ensure_history(vps_context* c) {
+    MlpState* m = S(c);
+    if (m->hp1.p) return VPS_OK;
+    const size_t cap = (size_t)(c->nbricks + 1) * c->brick_nodes;
+    for (int b = 0; b < 2; b++) {
+        VPS_TRY(dalloc(c, m->DP[b], cap));
+        VPS_TRY(dalloc(c, m->DP2[b], cap));
+        VPS_TRY(dalloc(c, m->H[b], cap));
+        VPS_HIP(c, hipMemset(m->H[b].p, 0, cap));
+    }
+    VPS_TRY(dalloc(c, m->hp1, cap));
+    VPS_TRY(dalloc(c, m->hdp1, cap));
+    VPS_TRY(dalloc(c, m->hdp2, cap));
+    VPS_TRY(dalloc(c, m->hh, cap));
+    VPS_TRY(dalloc(c, m->slot_of_rank, cap));
+    VPS_TRY(dalloc(c, m->part, kNPart * (cap / 256 + 16)));
+    return VPS_OK;
+}
+
+void mlp_destroy(vps_context* c) {
+    MlpState* m = S(c);
+    if (!m) return;
+    dfree(m->P);
+    dfree(m->m);
+    dfree(m->v);
+    dfree(m->grad);
+    dfree(m->grad_part);
+    dfree(m->loss_part);
+    dfree(m->loss);
+    for (int b = 0; b < 2; b++) {
+        dfree(m->scale[b]);
+        dfree(m->DP[b]);
+        dfree(m->DP2[b]);
+        dfree(m->H[b]);
+    }
+    dfree(m->hp1);
+    dfree(m->hdp1);
+    dfree(m->hdp2);
+    dfree(m->hh);
+    dfree(m->slot_of_rank);
+    dfree(m->part);
+    delete m;
+    c->mlp = nullptr;
+}
+
+bool mlp_active(const vps_context* c) {
+    const MlpState* m = (const MlpState*)c->mlp;
+    return m && (m->training || c->p.warm_start == VPS_WARM_LEARNED ||
+                 c->p.warm_start == VPS_WARM_EXTRAPOLATE);
+}
+
+static FeatSrc feat_src(vps_context* c, int scale_buf) {
+    MlpState* m = S(c);
+    FeatSrc f{};
+    f.p1 = m->hp1.p;
+    f.dp1 = m->hdp1.p;
+    f.dp2 = m->hdp2.p;
+    f.hh = m->hh.p;
+    f.vb = c->vb.p;
+    f.vdiag = c->vdiag.p;
+    f.vrow = c->vrowsum.p;
+    f.vvol = c->vvol.p;
+    f.scale = m->scale[scale_buf].p;
+    f.inv_a0 = c->alpha0 > 0 ? (float)(1.0 / c->alpha0) : 0.f;
+    const double vs = c->p.mass / (c->rho0 > 0 ? c->rho0 : 1.0);
+    f.inv_vs = (float)(1.0 / vs);
+    return f;
+}
+
+// Gathers the node history into the current slot layout, then the warm
+// start of the chosen mode.
+int mlp_warm_start(vps_context* c, int mode, float* x0) {
+    MlpState* m = S(c);
+    VPS_TRY(ensure_history(c));
+    const int w = c->which;
+    const int lb = c->last_solve_buf;
+    const int64_t nn = c->nslots * c->brick_nodes;
+    const bool hist = m->have_history && lb >= 0;
+    if (!hist) m->hist_solves = 0;
+    m->step_scale = m->cur_scale;
+    if (nn == 0) return warm_start(c, mode, x0);
+    hipLaunchKernelGGL(k_history_gather, dim3(blocks(nn, 256)), dim3(256), 0,
+                       c->stream, nn, c->brick_nodes, c->slot_brick[w].p,
+                       c->vvol.p, hist ? c->brick_slot[lb].p : nullptr,
+                       hist ? c->vx[lb].p : nullptr, hist ? m->DP[lb].p : nullptr,
+                       hist ? m->DP2[lb].p : nullptr, hist ? m->H[lb].p : nullptr,
+                       m->hp1.p, m->hdp1.p, m->hdp2.p, m->hh.p);
+    VPS_HIP(c, hipGetLastError());
+    if (mode == VPS_WARM_LEARNED) {
+        hipLaunchKernelGGL(k_mlp_infer<false>, dim3(kInferBlocks), dim3(256), 0,
+                           c->stream, m->P.p, feat_src(c, m->step_scale), nullptr, nn, x0);
+    } else if (mode == VPS_WARM_EXTRAPOLATE) {
+        hipLaunchKernelGGL(k_extrapolate, dim3(blocks(nn, 256)), dim3(256), 0,
+                           c->stream, feat_src(c, m->step_scale), nn, x0);
+    } else {
+        return warm_start(c, mode, x0);
+    }
+    VPS_HIP(c, hipGetLastError());
+    return VPS_OK;
+}
+
+// After a converged solve: node history of this layout, the scales of the
+// next step, and (when training) Adam steps on sampled batches.  Everything
+// stays on the stream: no host synchronisation.
+int mlp_after_solve(vps_context* c, const vps_stats* st) {
+    (void)st;
+    MlpState* m = S(c);
+    const int w = c->which;
+    const int64_t nn = c->nslots * c->brick_nodes;
+    if (nn == 0) return VPS_OK;
+    const int nb = (int)blocks(nn, 256);
+    hipLaunchKernelGGL(k_history_store, dim3(nb), dim3(256), 0, c->stream, nn,
+                       c->vvol.p, c->vrank.p, c->vx[w].p, m->hp1.p, m->hdp1.p,
+                       m->hdp2.p, m->hh.p, m->DP[w].p, m->DP2[w].p, m->H[w].p,
+                       m->slot_of_rank.p, m->part.p);
+    // targets need dp1 known at the sampled nodes and a dps scale: two
+    // earlier solves with history
+    if (m->training && m->hist_solves >= 2 && c->nvirt > 0) {
+        const int64_t ns = m->batch;
+        const int tpw = (int)((ns + 16 * kTrainWaves - 1) / (16 * kTrainWaves));
+        for (int it = 0; it < m->steps_per_solve; it++) {
+            m->adam_t++;
+            const float b1t = (float)std::pow(0.9, (double)m->adam_t);
+            const float b2t = (float)std::pow(0.999, (double)m->adam_t);
+            hipLaunchKernelGGL(k_mlp_train<false>, dim3(kTrainWaves / 4), dim3(256),
+                               0, c->stream, m->P.p, feat_src(c, m->step_scale), c->vx[w].p,
+                               m->slot_of_rank.p, c->nvirt, nullptr, nullptr, ns,
+                               tpw, m->seed ^ (uint64_t)m->adam_t * 0x100000001ull,
+                               m->grad_part.p, m->loss_part.p);
+            hipLaunchKernelGGL(k_mlp_reduce, dim3(blocks(kNParams, 256)), dim3(256),
+                               0, c->stream, m->grad_part.p, kTrainWaves,
+                               m->loss_part.p, m->grad.p, m->loss.p, m->P.p, m->m.p,
+                               m->v.p, 1, (float)m->lr, b1t, b2t);
+        }
+    }
+    const int nxt = 1 - m->cur_scale;
+    hipLaunchKernelGGL(k_scales, dim3(1), dim3(1024), 0, c->stream, m->part.p, nb,
+                       c->cg.p, c->nvirt, m->scale[nxt].p);
+    VPS_HIP(c, hipGetLastError());
+    m->cur_scale = nxt;
+    m->have_history = true;
+    m->hist_solves++;
+    return VPS_OK;
+}
+
 }  // namespace vps
 
+using namespace vps;
+
 extern "C" {
-int vps_mlp_set_training(vps_context* c, int32_t, int32_t, double, uint64_t) {
-    if (c) c->err = "learned warm start not built";
-    return VPS_ERR_STATE;
+
+int vps_mlp_set_training(vps_context* c, int32_t enable, int32_t batch,
+                         double learning_rate, int32_t steps_per_solve,
+                         uint64_t seed) {
+    if (!c) return VPS_ERR_INVALID_ARG;
+    if (batch < 16 || batch > (1 << 22) || !(learning_rate > 0) ||
+        steps_per_solve < 1 || steps_per_solve > 64) {
+        c->err = "mlp training: batch in [16, 4M], learning_rate > 0, "
+                 "steps_per_solve in [1, 64]";
+        return VPS_ERR_INVALID_ARG;
+    }
+    MlpState* m = S(c);
+    VPS_TRY(ensure_history(c));
+    m->training = enable != 0;
+    m->batch = batch;
+    m->lr = learning_rate;
+    m->steps_per_solve = steps_per_solve;
+    m->seed = seed;
+    return VPS_OK;
 }
-int64_t vps_mlp_parameter_count(void) { return 4801; }
-int vps_mlp_get_parameters(vps_context* c, float*) {
-    if (c) c->err = "learned warm start not built";
-    return VPS_ERR_STATE;
+
+int64_t vps_mlp_parameter_count(void) { return kNParams; }
+
+int vps_mlp_get_parameters(vps_context* c, float* out) {
+    if (!c || !out) return VPS_ERR_INVALID_ARG;
+    VPS_HIP(c, hipMemcpy(out, S(c)->P.p, kNParams * 4, hipMemcpyDeviceToHost));
+    return VPS_OK;
 }
-int vps_mlp_set_parameters(vps_context* c, const float*) {
-    if (c) c->err = "learned warm start not built";
-    return VPS_ERR_STATE;
+
+int vps_mlp_set_parameters(vps_context* c, const float* in) {
+    if (!c || !in) return VPS_ERR_INVALID_ARG;
+    for (int i = 0; i < kNParams; i++)
+        if (!std::isfinite(in[i])) {
+            c->err = "mlp parameters: non-finite value";
+            return VPS_ERR_NONFINITE;
+        }
+    MlpState* m = S(c);
+    VPS_HIP(c, hipMemcpy(m->P.p, in, kNParams * 4, hipMemcpyHostToDevice));
+    return VPS_OK;
 }
-int vps_mlp_loss_and_grad(vps_context* c, int64_t, const float*, const float*,
-                          double*, float*) {
-    if (c) c->err = "learned warm start not built";
-    return VPS_ERR_STATE;
+
+int vps_mlp_loss_and_grad(vps_context* c, int64_t n, const float* features,
+                          const float* targets, double* loss, float* grad) {
+    if (!c || n <= 0 || !features || !targets) return VPS_ERR_INVALID_ARG;
+    if (n > (int64_t)16 * kTrainWaves * 4096) {
+        c->err = "mlp loss_and_grad: too many samples";
+        return VPS_ERR_INVALID_ARG;
+    }
+    MlpState* m = S(c);
+    DBuf<float> df, dt;
+    VPS_TRY(dalloc(c, df, (size_t)n * kIn));
+    VPS_TRY(dalloc(c, dt, (size_t)n));
+    VPS_HIP(c, hipMemcpy(df.p, features, (size_t)n * kIn * 4, hipMemcpyHostToDevice));
+    VPS_HIP(c, hipMemcpy(dt.p, targets, (size_t)n * 4, hipMemcpyHostToDevice));
+    const int tpw = (int)((n + 16 * kTrainWaves - 1) / (16 * kTrainWaves));
+    FeatSrc none{};
+    DBuf<double> dl;
+    VPS_TRY(dalloc(c, dl, 1));
+    hipLaunchKernelGGL(k_mlp_train<true>, dim3(kTrainWaves / 4), dim3(256), 0,
+                       c->stream, m->P.p, none, nullptr, nullptr, n, df.p, dt.p, n,
+                       tpw, 0ull, m->grad_part.p, m->loss_part.p);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(blocks(kNParams, 256)), dim3(256), 0,
+                       c->stream, m->grad_part.p, kTrainWaves, m->loss_part.p,
+                       m->grad.p, dl.p, m->P.p, m->m.p, m->v.p, 0, 0.f, 0.f, 0.f);
+    VPS_HIP(c, hipGetLastError());
+    if (grad)
+        VPS_HIP(c, hipMemcpyAsync(grad, m->grad.p, kNParams * 4,
+                                  hipMemcpyDeviceToHost, c->stream));
+    double l = 0;
+    VPS_HIP(c, hipMemcpyAsync(c->host_counters + 20, dl.p, 8,
+                              hipMemcpyDeviceToHost, c->stream));
+    VPS_HIP(c, hipStreamSynchronize(c->stream));
+    std::memcpy(&l, c->host_counters + 20, 8);
+    if (loss) *loss = l;
+    dfree(df);
+    dfree(dt);
+    dfree(dl);
+    return VPS_OK;
 }
-int vps_mlp_forward(vps_context* c, int64_t, const float*, float*) {
-    if (c) c->err = "learned warm start not built";
-    return VPS_ERR_STATE;
+
+int vps_mlp_forward(vps_context* c, int64_t n, const float* features,
+                    float* out) {
+    if (!c || n <= 0 || !features || !out) return VPS_ERR_INVALID_ARG;
+    MlpState* m = S(c);
+    DBuf<float> df, dout;
+    VPS_TRY(dalloc(c, df, (size_t)n * kIn));
+    VPS_TRY(dalloc(c, dout, (size_t)n));
+    VPS_HIP(c, hipMemcpy(df.p, features, (size_t)n * kIn * 4, hipMemcpyHostToDevice));
+    FeatSrc none{};
+    hipLaunchKernelGGL(k_mlp_infer<true>, dim3(kInferBlocks), dim3(256), 0,
+                       c->stream, m->P.p, none, df.p, n, dout.p);
+    VPS_HIP(c, hipGetLastError());
+    VPS_HIP(c, hipMemcpyAsync(out, dout.p, (size_t)n * 4, hipMemcpyDeviceToHost,
+                              c->stream));
+    VPS_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(df);
+    dfree(dout);
+    return VPS_OK;
 }
-int vps_mlp_get_features(vps_context* c, float*) {
-    if (c) c->err = "learned warm start not built";
-    return VPS_ERR_STATE;
+
+int vps_mlp_training_state(vps_context* c, double* last_loss, int64_t* steps) {
+    if (!c) return VPS_ERR_INVALID_ARG;
+    const MlpState* m = S(c);
+    double l = 0;
+    VPS_HIP(c, hipMemcpyAsync(c->host_counters + 20, m->loss.p, 8,
+                              hipMemcpyDeviceToHost, c->stream));
+    VPS_HIP(c, hipStreamSynchronize(c->stream));
+    std::memcpy(&l, c->host_counters + 20, 8);
+    if (last_loss) *last_loss = l;
+    if (steps) *steps = m->adam_t;
+    return VPS_OK;
 }
+
+int vps_mlp_get_features(vps_context* c, float* out) {
+    if (!c || !out) return VPS_ERR_INVALID_ARG;
+    MlpState* m = S(c);
+    if (!m->hp1.p || c->nslots == 0) {
+        c->err = "mlp features: no step with history tracking yet";
+        return VPS_ERR_STATE;
+    }
+    const int64_t nn = c->nslots * c->brick_nodes;
+    DBuf<float> d;
+    VPS_TRY(dalloc(c, d, (size_t)nn * kIn));
+    hipLaunchKernelGGL(k_features, dim3(blocks(nn, 256)), dim3(256), 0, c->stream,
+                       feat_src(c, m->step_scale), nn, d.p);
+    std::vector<float> f((size_t)nn * kIn);
+    std::vector<int32_t> rank((size_t)nn);
+    VPS_HIP(c, hipMemcpyAsync(f.data(), d.p, (size_t)nn * kIn * 4,
+                              hipMemcpyDeviceToHost, c->stream));
+    VPS_HIP(c, hipMemcpyAsync(rank.data(), c->vrank.p, (size_t)nn * 4,
+                              hipMemcpyDeviceToHost, c->stream));
+    VPS_HIP(c, hipStreamSynchronize(c->stream));
+    for (int64_t g = 0; g < nn; g++)
+        if (rank[(size_t)g] >= 0)
+            std::memcpy(out + (int64_t)rank[(size_t)g] * kIn, f.data() + g * kIn,
+                        kIn * 4);
+    dfree(d);
+    return VPS_OK;
 }
+
+}  // extern "C"

@@ -65,28 +65,11 @@ constexpr int kHaloIters2 = (kT2 + 63) / 64;   // 4
 constexpr int kUpdBlocks = 256;                // grid of the CG update
 constexpr int kUpdThreads = 1024;
 
-// Tile element e -> (neighbour brick 0..26 or 0..8) << 9 | local node.
-// Elements past the tile map to the zero brick slot 27 (3D) / 9 (2D).
-struct HaloTab3 {
-    uint16_t v[kHaloIters3 * 64];
-};
+// 2D tile element e -> (neighbour brick 0..8) << 9 | local node; elements
+// past the tile map to the zero brick slot 9.  (3D decodes arithmetically.)
 struct HaloTab2 {
     uint16_t v[kHaloIters2 * 64];
 };
-constexpr HaloTab3 make_halo3() {
-    HaloTab3 t{};
-    for (int e = 0; e < kHaloIters3 * 64; e++) {
-        if (e >= kT3) {
-            t.v[e] = (uint16_t)(27 << 9);
-            continue;
-        }
-        const int ux = e % kT + 5, uy = (e / kT) % kT + 5, uz = e / kT2 + 5;
-        const int nb = (uz >> 3) * 9 + (uy >> 3) * 3 + (ux >> 3);
-        const int loc = (ux & 7) + 8 * ((uy & 7) + 8 * (uz & 7));
-        t.v[e] = (uint16_t)((nb << 9) | loc);
-    }
-    return t;
-}
 constexpr HaloTab2 make_halo2() {
     HaloTab2 t{};
     for (int e = 0; e < kHaloIters2 * 64; e++) {
@@ -101,7 +84,6 @@ constexpr HaloTab2 make_halo2() {
     }
     return t;
 }
-__constant__ HaloTab3 kHalo3 = make_halo3();
 __constant__ HaloTab2 kHalo2 = make_halo2();
 
 enum Kind {
@@ -129,6 +111,7 @@ struct StencilArgs {
     float* out2;           // p_new (fused) or p (init)
     double* out_d;         // diag64
     float* out_f;          // diag32
+    float* out_s;          // unclamped row sum s (feature of the learned start)
     double alpha0;
     CgState* st;
     double* part0;         // this kernel's per-block partials (sum 0)
@@ -369,6 +352,7 @@ __global__ void __launch_bounds__(64) k_stencil3(StencilArgs a, C cc) {
                 const double d = on ? (sd < a.alpha0 ? a.alpha0 : sd) : 0.0;
                 a.out_d[g] = d;
                 a.out_f[g] = (float)d;
+                a.out_s[g] = on ? (float)sd : 0.f;
             } else if constexpr (KIND == K_TRUE_RES) {
                 const double xi = (double)own[loc];
                 const double st = OP == VPS_OP_ROW ? (double)acc
@@ -464,6 +448,7 @@ __global__ void __launch_bounds__(64) k_stencil2(StencilArgs a, C cc) {
         const double d = on ? (s < a.alpha0 ? a.alpha0 : s) : 0.0;
         a.out_d[g] = d;
         a.out_f[g] = (float)d;
+        a.out_s[g] = on ? (float)s : 0.f;
     } else if constexpr (KIND == K_TRUE_RES) {
         const double xi = (double)own[loc];
         const double st = OP == VPS_OP_ROW ? (double)acc : (double)V * (double)acc;
@@ -738,6 +723,7 @@ int virtual_diag(vps_context* c, bool set_alpha0) {
     a.alpha0 = set_alpha0 ? -1e6 : c->alpha0;
     a.out_d = c->vdiag64.p;
     a.out_f = c->vdiag.p;
+    a.out_s = c->vrowsum.p;
     launch<K_DIAG, 0, double, CoefD>(c, a, cd);
     VPS_HIP(c, hipGetLastError());
     if (set_alpha0) {

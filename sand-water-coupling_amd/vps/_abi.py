@@ -32,6 +32,7 @@ WARM_INDEX_CARRY = 0
 WARM_ZERO = 1
 WARM_NODE_KEYED = 2
 WARM_LEARNED = 3
+WARM_EXTRAPOLATE = 4
 
 FROM_INIT = 0
 FROM_PARAMS = 1
@@ -162,13 +163,14 @@ def declare(lib):
     _proto(lib, "vps_query_neighbors", C.c_int,
            [_P, C.c_int32, C.c_int64, _D, _I64, _I64, C.c_int64, _I64])
     _proto(lib, "vps_mlp_set_training", C.c_int,
-           [_P, C.c_int32, C.c_int32, C.c_double, C.c_uint64])
+           [_P, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_uint64])
     _proto(lib, "vps_mlp_parameter_count", C.c_int64, [])
     _proto(lib, "vps_mlp_get_parameters", C.c_int, [_P, _F])
     _proto(lib, "vps_mlp_set_parameters", C.c_int, [_P, _F])
     _proto(lib, "vps_mlp_loss_and_grad", C.c_int, [_P, C.c_int64, _F, _F, _D, _F])
     _proto(lib, "vps_mlp_forward", C.c_int, [_P, C.c_int64, _F, _F])
     _proto(lib, "vps_mlp_get_features", C.c_int, [_P, _F])
+    _proto(lib, "vps_mlp_training_state", C.c_int, [_P, _D, _I64])
     _proto(lib, "vps_debug_apply_operator", C.c_int, [_P, _D, _D])
     return lib
 
@@ -185,6 +187,7 @@ EXPORTED = [
     "vps_query_neighbors", "vps_mlp_set_training", "vps_mlp_parameter_count",
     "vps_mlp_get_parameters", "vps_mlp_set_parameters",
     "vps_mlp_loss_and_grad", "vps_mlp_forward", "vps_mlp_get_features",
+    "vps_mlp_training_state",
     "vps_debug_apply_operator",
 ]
 

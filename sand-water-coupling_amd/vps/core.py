@@ -201,3 +201,48 @@ class Projection:
         out = np.zeros_like(p)
         self._check(self.L.vps_debug_apply_operator(self.h, _d(p), _d(out)))
         return out
+
+    # -- learned warm start (no reference counterpart)
+    def mlp_set_training(self, enable=True, batch=65536, lr=1e-3,
+                         steps_per_solve=1, seed=1):
+        self._check(self.L.vps_mlp_set_training(self.h, 1 if enable else 0,
+                                                int(batch), float(lr),
+                                                int(steps_per_solve), int(seed)))
+
+    def mlp_parameters(self):
+        p = np.zeros(self.L.vps_mlp_parameter_count(), dtype=np.float32)
+        self._check(self.L.vps_mlp_get_parameters(self.h, _f(p)))
+        return p
+
+    def mlp_set_parameters(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.L.vps_mlp_parameter_count():
+            raise ValueError("expected %d parameters" % self.L.vps_mlp_parameter_count())
+        self._check(self.L.vps_mlp_set_parameters(self.h, _f(p)))
+
+    def mlp_loss_and_grad(self, features, targets):
+        f = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, 8)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1)
+        if len(t) != len(f):
+            raise ValueError("features and targets differ in length")
+        loss = C.c_double()
+        g = np.zeros(self.L.vps_mlp_parameter_count(), dtype=np.float32)
+        self._check(self.L.vps_mlp_loss_and_grad(self.h, len(f), _f(f), _f(t),
+                                                 C.byref(loss), _f(g)))
+        return loss.value, g
+
+    def mlp_forward(self, features):
+        f = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, 8)
+        out = np.zeros(len(f), dtype=np.float32)
+        self._check(self.L.vps_mlp_forward(self.h, len(f), _f(f), _f(out)))
+        return out
+
+    def mlp_features(self):
+        out = np.zeros((self.virtual_count(), 8), dtype=np.float32)
+        self._check(self.L.vps_mlp_get_features(self.h, _f(out)))
+        return out
+
+    def mlp_training_state(self):
+        loss, steps = C.c_double(), C.c_int64()
+        self._check(self.L.vps_mlp_training_state(self.h, C.byref(loss), C.byref(steps)))
+        return loss.value, steps.value

@@ -124,7 +124,8 @@ def test_operator_apply(name, op):
 
 
 @pytest.mark.parametrize("warm", [_abi.WARM_INDEX_CARRY, _abi.WARM_ZERO,
-                                  _abi.WARM_NODE_KEYED])
+                                  _abi.WARM_NODE_KEYED, _abi.WARM_EXTRAPOLATE,
+                                  _abi.WARM_LEARNED])
 @pytest.mark.parametrize("name", ["2d", "3d", "cfg1"])
 def test_projection_steps(name, warm):
     sc, o, g = pair(name, warm_start=warm, **INIT)
