@@ -28,7 +28,8 @@ sys.path.insert(0, os.path.join(ROOT, "sand-water-coupling_amd"))
 METRIC = "projection particle-steps/s + CG iters at 1e-6 residual, learned vs plain, 1 GPU"
 HBM_PEAK_GBS = 8000.0
 
-WARM = {"index_carry": 0, "zero": 1, "node_keyed": 2, "learned": 3}
+WARM = {"index_carry": 0, "zero": 1, "node_keyed": 2, "learned": 3,
+        "extrapolate": 4}
 
 
 def parse():
@@ -39,7 +40,17 @@ def parse():
     ap.add_argument("--dx", type=float, default=1.0 / 150)
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--seed", type=int, default=20261202)
-    ap.add_argument("--warm-start", default="index_carry", choices=sorted(WARM))
+    ap.add_argument("--warm-start", default="index_carry", choices=sorted(WARM),
+                    help="warm start of the headline (plain) run")
+    ap.add_argument("--train-steps", type=int, default=30,
+                    help="untimed steps before --warmup in every run; the "
+                         "learned run trains its network online during them")
+    ap.add_argument("--train-per-solve", type=int, default=4,
+                    help="Adam steps per solve while training")
+    ap.add_argument("--batch", type=int, default=65536)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned / extrapolated warm-start runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-dx", type=float, default=1.0 / 60,
                     help="lattice spacing of the CPU-baseline sample")
@@ -103,28 +114,37 @@ def cpu_baseline(args):
     }
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU "
-              "path (the sharding design is described in DESIGN.md); "
-              "run with --gpus 1", file=sys.stderr)
-        return 2
-    from vps import Projection, scenes, _abi
-    sc = scenes.config2(seed=args.seed, dx=args.dx)
-    n = len(sc["x"])
-    workload = "dam_break_3d_%d" % n
-    proj = Projection(sc, warm_start=WARM[args.warm_start],
+def run(args, sc, warm, train=False, kernel_timing=False):
+    """One run from the initial scene: train_steps + warmup untimed steps
+    (the learned run trains online during the first train_steps), then
+    `steps` timed steps, bracketed by device synchronisation."""
+    from vps import Projection, _abi
+    proj = Projection(sc, warm_start=WARM[warm],
                       rest_density_mode=_abi.FROM_PARAMS,
                       kappa_mode=_abi.FROM_PARAMS)
     t0 = time.perf_counter()
     proj.initialize()
-    log("init %.3f s, n=%d nb=%d" % (time.perf_counter() - t0, n, proj.nb))
+    log("%s: init %.3f s, n=%d nb=%d" % (warm, time.perf_counter() - t0, proj.n, proj.nb))
+    if train:
+        proj.mlp_set_training(True, batch=args.batch, lr=args.lr,
+                              steps_per_solve=args.train_per_solve, seed=args.seed)
+    train_iters = []
+    proj.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.train_steps):
+        proj.advect(args.dt, 9.8)
+        train_iters.append(proj.projection_step()["cg_iterations"])
+    proj.synchronize()
+    t_train = time.perf_counter() - t0
+    if train:
+        loss, adam = proj.mlp_training_state()
+        proj.mlp_set_training(False)
+        log("%s: %d training steps, %d Adam steps, last loss %.3g"
+            % (warm, args.train_steps, adam, loss))
     for k in range(args.warmup):
         proj.advect(args.dt, 9.8)
         st = proj.projection_step()
-        log("warmup %d: nv=%d iters=%d" % (k, st["n_virtual"], st["cg_iterations"]))
-    proj.set_kernel_timing(not args.no_kernel_timing)
+        log("%s warmup %d: nv=%d iters=%d" % (warm, k, st["n_virtual"], st["cg_iterations"]))
     stats = []
     proj.synchronize()
     t0 = time.perf_counter()
@@ -133,21 +153,61 @@ def main():
         stats.append(proj.projection_step())
     proj.synchronize()
     elapsed = time.perf_counter() - t0
-    ms_step = 1e3 * elapsed / args.steps
-    iters = [s["cg_iterations"] for s in stats]
-    nv = [s["n_virtual"] for s in stats]
+    # per-launch HIP events on the library stream, after the timed steps
+    # (the events themselves cost a little)
+    timed_launch = []
+    if kernel_timing:
+        proj.set_kernel_timing(True)
+        for k in range(5):
+            proj.advect(args.dt, 9.8)
+            timed_launch.append(proj.projection_step())
+        proj.set_kernel_timing(False)
+    nb = proj.nb
+    proj.close()
     if args.verbose:
         for s in stats:
             log(json.dumps({k: (round(v, 4) if isinstance(v, float) else v)
                             for k, v in s.items()}))
+    return dict(stats=stats, elapsed=elapsed, nb=nb, t_train=t_train,
+                train_iters=train_iters, timed_launch=timed_launch)
+
+
+def summary(r, n, steps):
+    st = r["stats"]
+    return {
+        "value": round(n * steps / r["elapsed"], 1),
+        "ms_per_step": round(1e3 * r["elapsed"] / steps, 4),
+        "cg_iters_mean": round(float(np.mean([s["cg_iterations"] for s in st])), 2),
+        "cg_ms_mean": round(float(np.mean([s["ms_cg"] for s in st])), 4),
+        "true_rel_residual_max": max(s["cg_true_rel_residual"] for s in st),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU "
+              "path; run with --gpus 1", file=sys.stderr)
+        return 2
+    from vps import scenes
+    sc = scenes.config2(seed=args.seed, dx=args.dx)
+    n = len(sc["x"])
+    workload = "dam_break_3d_%d" % n
+    plain = run(args, sc, args.warm_start, kernel_timing=not args.no_kernel_timing)
+    stats = plain["stats"]
+    elapsed = plain["elapsed"]
+    ms_step = 1e3 * elapsed / args.steps
+    iters = [s["cg_iterations"] for s in stats]
+    nv = [s["n_virtual"] for s in stats]
     # roofline of the CG SpMV (dominant kernel): algorithmic bytes per launch
     # = 32 B per virtual particle (p, diag, V, slot map read and M p written:
     # 20 B; the fused p = r + beta p update: 12 B)
-    spmv = [s["ms_spmv_avg"] for s in stats if s["ms_spmv_avg"] > 0]
+    kt = plain["timed_launch"]
+    spmv = [s["ms_spmv_avg"] for s in kt if s["ms_spmv_avg"] > 0]
     roof = None
     if spmv:
         t_launch = float(np.mean(spmv)) * 1e-3
-        bytes_launch = 32.0 * float(np.mean(nv))
+        bytes_launch = 32.0 * float(np.mean([s["n_virtual"] for s in kt]))
         ach = bytes_launch / t_launch / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
@@ -155,7 +215,6 @@ def main():
                 "kernel": "k_stencil3<SPMV_FUSED, TRANSPOSE>",
                 "avg_launch_us": round(t_launch * 1e6, 2),
                 "bytes_per_launch": bytes_launch}
-    # whole-step algorithmic bytes (144N + 76N_V + 80N_B' + 8C + 56 k N_V)
     out = {
         "metric": METRIC,
         "value": round(n * args.steps / elapsed, 1),
@@ -169,9 +228,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 sums/SpMV, f64 dots+membership",
         "data": "synthetic",
-        "config": {"workload": workload, "n_real": n, "n_boundary": proj.nb,
+        "config": {"workload": workload, "n_real": n, "n_boundary": plain["nb"],
                    "dx": args.dx, "dt": args.dt, "seed": args.seed,
                    "warm_start": args.warm_start, "tolerance": 1e-6,
+                   "untimed_steps": args.train_steps + args.warmup,
                    "parallelism": "single GPU"},
         "cg_iters_mean": round(float(np.mean(iters)), 2),
         "cg_iters": iters,
@@ -183,7 +243,23 @@ def main():
         "roofline": roof,
         "cpu_baseline": None,
     }
-    proj.close()
+    if not args.no_learned:
+        base = summary(plain, n, args.steps)
+        lr = run(args, sc, "learned", train=True)
+        ex = run(args, sc, "extrapolate")
+        L = summary(lr, n, args.steps)
+        E = summary(ex, n, args.steps)
+        for d in (L, E):
+            d["iters_ratio_vs_plain"] = round(base["cg_iters_mean"] / max(d["cg_iters_mean"], 1e-9), 3)
+            d["step_speedup_vs_plain"] = round(base["ms_per_step"] / d["ms_per_step"], 3)
+            d["cg_speedup_vs_plain"] = round(base["cg_ms_mean"] / max(d["cg_ms_mean"], 1e-9), 3)
+        L["training"] = {"steps": args.train_steps, "adam_per_solve": args.train_per_solve,
+                         "batch": args.batch, "lr": args.lr,
+                         "ms_per_step_while_training": round(1e3 * lr["t_train"] / max(args.train_steps, 1), 4),
+                         "cg_iters_while_training": lr["train_iters"]}
+        out["learned_warm_start"] = L
+        out["extrapolated_warm_start"] = E
+        out["plain_cg_ms_mean"] = base["cg_ms_mean"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)

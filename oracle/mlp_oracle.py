@@ -1,4 +1,4 @@
-"""FP64 restatement of the warm-start MLP (8 -> 64 -> 64 -> 1, GELU).
+"""FP64 restatement of the warm-start MLP (8 -> 64 -> 64 -> 1, tanh-form GELU).
 
 TEST INFRASTRUCTURE ONLY (the checker of the hand-written MFMA kernels).
 The network has no counterpart in the reference.  Parameter packing is the
@@ -32,12 +32,22 @@ def unpack(p):
     return W1, b1, W2, b2, W3, b3
 
 
+C_GELU = np.sqrt(2 / np.pi)
+
+
 def gelu(z):
-    return 0.5 * z * (1 + erf(z / np.sqrt(2)))
+    """GELU, tanh form: 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3)))."""
+    return 0.5 * z * (1 + np.tanh(C_GELU * (z + 0.044715 * z ** 3)))
 
 
 def gelu_d(z):
-    return 0.5 * (1 + erf(z / np.sqrt(2))) + z * np.exp(-0.5 * z * z) / np.sqrt(2 * np.pi)
+    t = np.tanh(C_GELU * (z + 0.044715 * z ** 3))
+    return 0.5 * (1 + t) + 0.5 * z * (1 - t * t) * C_GELU * (1 + 3 * 0.044715 * z * z)
+
+
+def gelu_erf(z):
+    """The exact (erf) GELU, which the tanh form approximates to ~1e-3."""
+    return 0.5 * z * (1 + erf(z / np.sqrt(2)))
 
 
 def forward(p, X, bf=False):

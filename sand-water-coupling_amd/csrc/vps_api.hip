@@ -309,7 +309,7 @@ int vps_create(const vps_params* params, vps_context** out) {
     if (rc == VPS_OK) {
         for (auto& e : c->ev)
             if (hipEventCreate(&e) != hipSuccess) rc = VPS_ERR_HIP;
-        c->kev.resize(3 * 256);
+        c->kev.resize(4 * 256);
         for (auto& e : c->kev)
             if (hipEventCreate(&e) != hipSuccess) rc = VPS_ERR_HIP;
     }

@@ -18,6 +18,7 @@
 //                    contiguous 2 KB (3D) run per field
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>

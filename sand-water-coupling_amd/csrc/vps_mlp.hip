@@ -5,7 +5,7 @@
 // of them are known; zero for a node absent from the last solve).  The
 // polynomial extrapolation through the known values is the prior
 //     e = 0 (h <= 1),  dp1 (h = 2),  2 dp1 - dp2 (h = 3)   (p_next ~ p1 + e)
-// and a per-virtual-particle MLP 8 -> 64 -> 64 -> 1 (GELU, BF16 weights and
+// and a per-virtual-particle MLP 8 -> 64 -> 64 -> 1 (tanh-form GELU, BF16 weights and
 // activations, FP32 accumulation) learns the correction to it:
 //     x0_I = p1_I + e_I + tau * net(f_I)     (nodes with h >= 1)
 //     x0_I = 0                               (nodes that are new)
@@ -61,7 +61,8 @@ enum {
     SC_INV_PS = 0, SC_INV_DPS = 1, SC_INV_DS = 2, SC_INV_TAU = 3, SC_TAU = 4,
     SC_DPS = 5, SC_PS = 6, SC_N = 8
 };
-constexpr int kTrainWaves = 256;
+constexpr int kTrainBlocks = 256;               // one block of 4 waves per CU
+constexpr int kTrainWaves = 4 * kTrainBlocks;
 constexpr int kInferBlocks = 1024;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
@@ -78,12 +79,18 @@ __device__ __forceinline__ float bf2f(short b) {
 __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
-__device__ __forceinline__ float gelu(float z) {
-    return 0.5f * z * (1.f + erff(z * 0.70710678118654752f));
+// GELU in its tanh form: gelu(z) = 0.5 z (1 + tanh(u)) = z sigmoid(2u),
+// u = sqrt(2/pi) (z + 0.044715 z^3); one exp and one reciprocal per value.
+__device__ __forceinline__ float gelu_sig(float z) {
+    const float u2 = 1.5957691216057308f * z * (1.f + 0.044715f * z * z);   // 2u
+    return __fdividef(1.f, 1.f + __expf(-u2));
 }
+__device__ __forceinline__ float gelu(float z) { return z * gelu_sig(z); }
+// d/dz [z s(2u)] = s + z s (1 - s) 2u'(z)
 __device__ __forceinline__ float gelu_d(float z) {
-    return 0.5f * (1.f + erff(z * 0.70710678118654752f)) +
-           z * 0.39894228040143268f * expf(-0.5f * z * z);
+    const float s = gelu_sig(z);
+    const float du2 = 1.5957691216057308f * (1.f + 0.134145f * z * z);
+    return s + z * s * (1.f - s) * du2;
 }
 
 struct Wregs {
@@ -234,20 +241,28 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const float* P, FeatSrc fs,
     const int64_t ntiles = (nn + 15) / 16;
     for (int64_t tile = wave; tile < ntiles; tile += nwaves) {
         const int64_t g = tile * 16 + (lane & 15);
-        const bool ok = g < nn;
+        bool ok = g < nn;
+        if constexpr (!DIRECT) {
+            // only nodes with a history get the network; whole tiles without
+            // one (empty brick slots, new nodes) skip it (wave-uniform)
+            ok = ok && fs.vvol[g] > 0.f && fs.hh[g] >= 1;
+            if (!__any(ok)) {
+                if (lane < 16 && g < nn) out[g] = 0.f;
+                continue;
+            }
+        }
         float fo[4];
         const bf16x4 xb = tile_features<DIRECT>(fs, direct, nn, g, ok, fo);
         Fwd F;
         forward<false>(W, nullptr, xb, F);
-        if (lane < 16 && ok) {
+        if (lane < 16 && g < nn) {
             if constexpr (DIRECT) {
                 out[g] = F.out;
             } else {
                 const int h = fs.hh[g];
-                const bool has = fs.vvol[g] > 0.f && h >= 1;
-                out[g] = has ? fs.p1[g] + prior(h, fs.dp1[g], fs.dp2[g]) +
-                                   fs.scale[SC_TAU] * F.out
-                             : 0.f;
+                out[g] = ok ? fs.p1[g] + prior(h, fs.dp1[g], fs.dp2[g]) +
+                                  fs.scale[SC_TAU] * F.out
+                            : 0.f;
             }
         }
     }
@@ -271,17 +286,26 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
 }
 
 // ---- training: forward + backward over sampled nodes ---------------------
-// Per wave: tiles_per_wave tiles of 16 samples.  Writes the wave's gradient
-// (kNParams floats) and loss partial.  Loss = sum (out - t)^2 / nsamples.
+// Per wave: tiles_per_wave tiles of 16 samples.  The 4 waves of a block sum
+// their gradients through LDS (fixed order); the block writes one row of
+// kNParams floats and its loss partial.  Loss = sum (out - t)^2 / nsamples.
 template <bool DIRECT>
 __global__ void __launch_bounds__(256) k_mlp_train(
     const float* P, FeatSrc fs, const float* vx, const int32_t* slot_of_rank,
     int64_t nv, const float* fdirect, const float* tdirect, int64_t nsamples,
     int tiles_per_wave, uint64_t seed, float* grad_part, double* loss_part) {
-    __shared__ uint16_t s_dz2[4][kHid * 16];
-    __shared__ uint16_t s_h1[4][kHid * 16];
-    __shared__ uint16_t s_dz1[4][kHid * 16];
-    __shared__ uint16_t s_x[4][16 * 16];
+    // per-wave bf16 staging tiles; reused as the FP32 gradient exchange of
+    // the block reduction (2 x kNParams floats) after the sample loop
+    constexpr int kTileBytes = 4 * (3 * kHid * 16 + 16 * 16) * 2;
+    constexpr int kRedBytes = 2 * kNParams * 4;
+    constexpr int kStageBytes = kTileBytes > kRedBytes ? kTileBytes : kRedBytes;
+    __shared__ __align__(16) uint8_t s_stage[kStageBytes];
+    __shared__ double s_loss[4];
+    auto s_dz2 = reinterpret_cast<uint16_t(*)[kHid * 16]>(s_stage);
+    auto s_h1 = reinterpret_cast<uint16_t(*)[kHid * 16]>(s_stage + 4 * kHid * 16 * 2);
+    auto s_dz1 = reinterpret_cast<uint16_t(*)[kHid * 16]>(s_stage + 8 * kHid * 16 * 2);
+    auto s_x = reinterpret_cast<uint16_t(*)[16 * 16]>(s_stage + 12 * kHid * 16 * 2);
+    float* s_red = reinterpret_cast<float*>(s_stage);
     __shared__ uint16_t sW2[kHid * kHid];    // W2[out][in]
     __shared__ uint16_t sW2T[kHid * kHid];   // W2^T[in][out]
     const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
@@ -420,24 +444,52 @@ __global__ void __launch_bounds__(256) k_mlp_train(
         gb3 += __shfl_xor(gb3, o, 64);
         loss += __shfl_xor(loss, o, 64);
     }
-    float* G = grad_part + wave * kNParams;
+    // Every gradient entry this lane holds, with its parameter index (each
+    // index is held by exactly one lane of the wave); f returns the new value.
+    auto each = [&](auto&& f) {
 #pragma unroll
-    for (int t = 0; t < 4; t++)
+        for (int t = 0; t < 4; t++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int row = t * 16 + 4 * q + i;
-            if (r < kIn) G[OFF_W1 + row * kIn + r] = gW1[t][i];
-            if (r == 0) {
-                G[OFF_B1 + row] = gb1[t][i];
-                G[OFF_B2 + row] = gb2[t][i];
-                G[OFF_W3 + row] = gW3[t][i];
+            for (int i = 0; i < 4; i++) {
+                const int row = t * 16 + 4 * q + i;
+                if (r < kIn) gW1[t][i] = f(OFF_W1 + row * kIn + r, gW1[t][i]);
+                if (r == 0) {
+                    gb1[t][i] = f(OFF_B1 + row, gb1[t][i]);
+                    gb2[t][i] = f(OFF_B2 + row, gb2[t][i]);
+                    gW3[t][i] = f(OFF_W3 + row, gW3[t][i]);
+                }
+#pragma unroll
+                for (int mi = 0; mi < 4; mi++)
+                    gW2[t][mi][i] = f(OFF_W2 + row * kHid + mi * 16 + r, gW2[t][mi][i]);
             }
-#pragma unroll
-            for (int mi = 0; mi < 4; mi++) G[OFF_W2 + row * kHid + mi * 16 + r] = gW2[t][mi][i];
-        }
-    if (lane == 0) {
-        G[OFF_B3] = gb3;
-        loss_part[wave] = loss * (double)inv_n;
+        if (lane == 0) gb3 = f(OFF_B3, gb3);
+    };
+    // block sum in a fixed order: (w0 + w2) + (w1 + w3)
+    if (lane == 0) s_loss[wl] = loss;
+    __syncthreads();                      // staging tiles no longer read
+    if (wl >= 2)
+        each([&](int idx, float v) {
+            s_red[(wl - 2) * kNParams + idx] = v;
+            return v;
+        });
+    __syncthreads();
+    if (wl < 2) each([&](int idx, float v) { return v + s_red[wl * kNParams + idx]; });
+    __syncthreads();
+    if (wl == 1)
+        each([&](int idx, float v) {
+            s_red[idx] = v;
+            return v;
+        });
+    __syncthreads();
+    if (wl == 0) {
+        float* G = grad_part + (int64_t)blockIdx.x * kNParams;
+        each([&](int idx, float v) {
+            G[idx] = v + s_red[idx];
+            return v;
+        });
+        if (lane == 0)
+            loss_part[blockIdx.x] = ((s_loss[0] + s_loss[2]) + (s_loss[1] + s_loss[3])) *
+                                    (double)inv_n;
     }
 }
 
@@ -635,8 +687,8 @@ int mlp_create(vps_context* c) {
     VPS_TRY(dalloc(c, m->m, kNParams));
     VPS_TRY(dalloc(c, m->v, kNParams));
     VPS_TRY(dalloc(c, m->grad, kNParams));
-    VPS_TRY(dalloc(c, m->grad_part, (size_t)kTrainWaves * kNParams));
-    VPS_TRY(dalloc(c, m->loss_part, kTrainWaves));
+    VPS_TRY(dalloc(c, m->grad_part, (size_t)kTrainBlocks * kNParams));
+    VPS_TRY(dalloc(c, m->loss_part, kTrainBlocks));
     VPS_TRY(dalloc(c, m->loss, 1));
     for (int b = 0; b < 2; b++) {
         VPS_TRY(dalloc(c, m->scale[b], SC_N));
@@ -777,13 +829,13 @@ int mlp_after_solve(vps_context* c, const vps_stats* st) {
             m->adam_t++;
             const float b1t = (float)std::pow(0.9, (double)m->adam_t);
             const float b2t = (float)std::pow(0.999, (double)m->adam_t);
-            hipLaunchKernelGGL(k_mlp_train<false>, dim3(kTrainWaves / 4), dim3(256),
+            hipLaunchKernelGGL(k_mlp_train<false>, dim3(kTrainBlocks), dim3(256),
                                0, c->stream, m->P.p, feat_src(c, m->step_scale), c->vx[w].p,
                                m->slot_of_rank.p, c->nvirt, nullptr, nullptr, ns,
                                tpw, m->seed ^ (uint64_t)m->adam_t * 0x100000001ull,
                                m->grad_part.p, m->loss_part.p);
             hipLaunchKernelGGL(k_mlp_reduce, dim3(blocks(kNParams, 256)), dim3(256),
-                               0, c->stream, m->grad_part.p, kTrainWaves,
+                               0, c->stream, m->grad_part.p, kTrainBlocks,
                                m->loss_part.p, m->grad.p, m->loss.p, m->P.p, m->m.p,
                                m->v.p, 1, (float)m->lr, b1t, b2t);
         }
@@ -861,11 +913,11 @@ int vps_mlp_loss_and_grad(vps_context* c, int64_t n, const float* features,
     FeatSrc none{};
     DBuf<double> dl;
     VPS_TRY(dalloc(c, dl, 1));
-    hipLaunchKernelGGL(k_mlp_train<true>, dim3(kTrainWaves / 4), dim3(256), 0,
+    hipLaunchKernelGGL(k_mlp_train<true>, dim3(kTrainBlocks), dim3(256), 0,
                        c->stream, m->P.p, none, nullptr, nullptr, n, df.p, dt.p, n,
                        tpw, 0ull, m->grad_part.p, m->loss_part.p);
     hipLaunchKernelGGL(k_mlp_reduce, dim3(blocks(kNParams, 256)), dim3(256), 0,
-                       c->stream, m->grad_part.p, kTrainWaves, m->loss_part.p,
+                       c->stream, m->grad_part.p, kTrainBlocks, m->loss_part.p,
                        m->grad.p, dl.p, m->P.p, m->m.p, m->v.p, 0, 0.f, 0.f, 0.f);
     VPS_HIP(c, hipGetLastError());
     if (grad)

@@ -698,20 +698,35 @@ static StencilArgs base_args(vps_context* c) {
     return a;
 }
 
+// With a pair of events the launch goes through hipExtLaunchKernelGGL, which
+// stamps them at the start and the end of the kernel's own execution (no
+// dispatch gap in the measured time).
 template <int KIND, int OP, typename T, typename C>
-static void launch(vps_context* c, const StencilArgs& a, const C& cc) {
-    if (c->dim == 3)
-        hipLaunchKernelGGL((k_stencil3<KIND, OP, T, C>), dim3((unsigned)c->nslots),
-                           dim3(64), 0, c->stream, a, cc);
-    else
-        hipLaunchKernelGGL((k_stencil2<KIND, OP, T, C>), dim3((unsigned)c->nslots),
-                           dim3(64), 0, c->stream, a, cc);
+static void launch(vps_context* c, const StencilArgs& a, const C& cc,
+                   hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    const dim3 grid((unsigned)c->nslots), block(64);
+    if (c->dim == 3) {
+        if (e0)
+            hipExtLaunchKernelGGL((k_stencil3<KIND, OP, T, C>), grid, block, 0,
+                                  c->stream, e0, e1, 0, a, cc);
+        else
+            hipLaunchKernelGGL((k_stencil3<KIND, OP, T, C>), grid, block, 0,
+                               c->stream, a, cc);
+    } else {
+        if (e0)
+            hipExtLaunchKernelGGL((k_stencil2<KIND, OP, T, C>), grid, block, 0,
+                                  c->stream, e0, e1, 0, a, cc);
+        else
+            hipLaunchKernelGGL((k_stencil2<KIND, OP, T, C>), grid, block, 0,
+                               c->stream, a, cc);
+    }
 }
 
 template <int KIND, typename T, typename C>
-static void launch_op(vps_context* c, const StencilArgs& a, const C& cc) {
-    if (c->p.op_mode == VPS_OP_ROW) launch<KIND, VPS_OP_ROW, T, C>(c, a, cc);
-    else launch<KIND, VPS_OP_TRANSPOSE, T, C>(c, a, cc);
+static void launch_op(vps_context* c, const StencilArgs& a, const C& cc,
+                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    if (c->p.op_mode == VPS_OP_ROW) launch<KIND, VPS_OP_ROW, T, C>(c, a, cc, e0, e1);
+    else launch<KIND, VPS_OP_TRANSPOSE, T, C>(c, a, cc, e0, e1);
 }
 
 int virtual_diag(vps_context* c, bool set_alpha0) {
@@ -836,7 +851,7 @@ int cg_solve(vps_context* c, vps_stats* stats) {
         return VPS_ERR_NO_CONVERGENCE;
     }
     const bool prof = c->kernel_timing;
-    const int64_t max_timed = (int64_t)c->kev.size() / 3;
+    const int64_t max_timed = (int64_t)c->kev.size() / 4;
     int64_t timed = 0, spmv_launches = 0;
     int kk = 0;          // iteration since the (re)start
     int iter_base = 0;   // Eigen iterations before the (re)start
@@ -858,8 +873,10 @@ int cg_solve(vps_context* c, vps_stats* stats) {
                 s.part0 = part_pt(c, kk);
                 s.out = t;
                 float* dir = (kk == 0) ? p0 : ((kk & 1) ? p1 : p0);
-                const bool do_time = prof && timed < max_timed;
-                if (do_time) VPS_HIP(c, hipEventRecord(c->kev[3 * timed], c->stream));
+                // the first SpMV of a (re)start is not fused: only fused
+                // launches are timed
+                const bool do_time = prof && timed < max_timed && kk > 0;
+                hipEvent_t* ev = do_time ? &c->kev[4 * timed] : nullptr;
                 if (kk == 0) {
                     s.in = p0;
                     launch_op<K_SPMV_FIRST, float, CoefF>(c, s, cf);
@@ -867,17 +884,23 @@ int cg_solve(vps_context* c, vps_stats* stats) {
                     s.in = (kk & 1) ? p0 : p1;   // previous direction
                     s.in2 = r;
                     s.out2 = dir;
-                    launch_op<K_SPMV_FUSED, float, CoefF>(c, s, cf);
+                    launch_op<K_SPMV_FUSED, float, CoefF>(c, s, cf,
+                                                          ev ? ev[0] : nullptr,
+                                                          ev ? ev[1] : nullptr);
                 }
-                if (do_time)
-                    VPS_HIP(c, hipEventRecord(c->kev[3 * timed + 1], c->stream));
-                hipLaunchKernelGGL(k_cg_update, dim3(kUpdBlocks), dim3(kUpdThreads),
-                                   0, c->stream, n4, (float4*)x, (float4*)r,
-                                   (const float4*)dir, (const float4*)t, c->cg.p,
-                                   part_pt(c, kk), nslots, part_rr(c, kk + 2), kk,
-                                   part_rr(c, kk));
+                if (ev)
+                    hipExtLaunchKernelGGL(k_cg_update, dim3(kUpdBlocks), dim3(kUpdThreads),
+                                          0, c->stream, ev[2], ev[3], 0, n4, (float4*)x,
+                                          (float4*)r, (const float4*)dir,
+                                          (const float4*)t, c->cg.p, part_pt(c, kk),
+                                          nslots, part_rr(c, kk + 2), kk, part_rr(c, kk));
+                else
+                    hipLaunchKernelGGL(k_cg_update, dim3(kUpdBlocks), dim3(kUpdThreads),
+                                       0, c->stream, n4, (float4*)x, (float4*)r,
+                                       (const float4*)dir, (const float4*)t, c->cg.p,
+                                       part_pt(c, kk), nslots, part_rr(c, kk + 2), kk,
+                                       part_rr(c, kk));
                 if (do_time) {
-                    VPS_HIP(c, hipEventRecord(c->kev[3 * timed + 2], c->stream));
                     timed_iter.push_back(iter_base + kk);
                     timed++;
                 }
@@ -929,8 +952,8 @@ int cg_solve(vps_context* c, vps_stats* stats) {
     int64_t counted = 0;
     for (int64_t j = 0; j < timed; j++) {
         if (timed_iter[(size_t)j] > hs.iter) break;   // no-op launches
-        spmv_ms += ms_between(c->kev[3 * j], c->kev[3 * j + 1]);
-        upd_ms += ms_between(c->kev[3 * j + 1], c->kev[3 * j + 2]);
+        spmv_ms += ms_between(c->kev[4 * j], c->kev[4 * j + 1]);
+        upd_ms += ms_between(c->kev[4 * j + 2], c->kev[4 * j + 3]);
         counted++;
     }
     if (stats) {

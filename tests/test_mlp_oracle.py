@@ -12,6 +12,11 @@ def test_gelu_derivative():
     np.testing.assert_allclose(M.gelu_d(z), fd, rtol=1e-7, atol=1e-9)
     assert M.gelu(0.0) == 0.0
     np.testing.assert_allclose(M.gelu(10.0), 10.0, rtol=1e-12)
+    # the tanh form stays within 1e-3 of the exact erf GELU
+    assert np.abs(M.gelu(z) - M.gelu_erf(z)).max() < 1e-3
+    # the form used by the kernels: z * sigmoid(2u)
+    u2 = 2 * M.C_GELU * (z + 0.044715 * z ** 3)
+    np.testing.assert_allclose(M.gelu(z), z / (1 + np.exp(-u2)), rtol=1e-12, atol=1e-15)
 
 
 def test_gradient_matches_finite_differences():
