@@ -69,10 +69,12 @@ enum vps_warm_start {
     VPS_WARM_ZERO = 1,         /* x0 = 0 */
     VPS_WARM_NODE_KEYED = 2,   /* x0 = previous pressure at the same lattice
                                   node, zero for nodes that are new */
-    VPS_WARM_LEARNED = 3,      /* x0 = p1 + e + dps * MLP(features), the
+    VPS_WARM_LEARNED = 3,      /* x0 = p1 + e + tau * MLP(features), the
                                   learned correction to the extrapolation
-                                  below (no reference counterpart); an
-                                  untrained network gives the extrapolation */
+                                  below, tau the RMS error of that
+                                  extrapolation at the last solve (no
+                                  reference counterpart); an untrained
+                                  network gives the extrapolation */
     VPS_WARM_EXTRAPOLATE = 4   /* x0 = p1 + e: polynomial extrapolation of the
                                   node's last three pressures, e = dp1 with two
                                   known, 2 dp1 - dp2 with three (no reference
@@ -215,8 +217,9 @@ int vps_advect(vps_context* ctx, double dt, double gravity);
  * ||b - M x||/||b|| into stats.cg_true_rel_residual. */
 int vps_set_check_true_residual(vps_context* ctx, int32_t enable);
 
-/* Records HIP events around every CG SpMV and update launch of the next
- * solves, reported as stats.ms_spmv_avg / ms_update_avg. */
+/* Stamps the start and end of every fused CG SpMV and CG update launch of
+ * the next solves (events of hipExtLaunchKernelGGL: the kernels' own
+ * execution), reported as stats.ms_spmv_avg / ms_update_avg. */
 int vps_set_kernel_timing(vps_context* ctx, int32_t enable);
 
 /* Blocks until all work on the context's stream is done. */
