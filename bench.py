@@ -57,7 +57,24 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB "
+                         "(3 times) and exit: the reference kernel that "
+                         "calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": 4 * n, "repeats": 3}), flush=True)
+    return 0
 
 
 def log(*a):
@@ -185,6 +202,8 @@ def summary(r, n, steps):
 
 def main():
     args = parse()
+    if args.copy_probe_mib > 0:
+        return copy_probe(args.copy_probe_mib)
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU "
               "path; run with --gpus 1", file=sys.stderr)

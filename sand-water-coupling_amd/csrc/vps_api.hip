@@ -134,6 +134,18 @@ void fill_kconst(vps_context* c) {
     k.mass = (float)p.mass;
     k.rho0 = (float)c->rho0;
     k.kappa = (float)c->kappa;
+    {
+        // per-axis error of the FP32 difference: the roundings of the cell
+        // offsets (< 2 cells), of the corner term and of the two FP32 sums
+        // are each 2^-24 of at most 4 cells, plus the node coordinate of a
+        // brick (< 14 s); 16 cells + 14 s + 2 h bounds their sum
+        const double u = std::ldexp(1.0, -24);
+        const double delta = u * (14 * k.spacing + 16 * cs + 2 * k.h);
+        const double err = 2 * std::sqrt(3.0) * k.h * delta + 4 * u * k.h2;
+        const double band = 4 * err;
+        k.h2f_lo = std::nextafter((float)(k.h2 - band), 0.f);
+        k.h2f_hi = std::nextafter((float)(k.h2 + band), 1e30f);
+    }
 }
 
 bool finite_all(const double* a, int64_t n) {

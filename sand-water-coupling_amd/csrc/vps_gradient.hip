@@ -49,59 +49,71 @@ __global__ void __launch_bounds__(256) k_gradient(
     }
     if (!empty) {
         // x-axis differences and squares, kept in registers
-        double dxs[kBox], dx2[kBox];
+        double dx2[kBox];
+        float dxf[kBox];
 #pragma unroll
         for (int c = 0; c < kBox; c++) {
             const double nx = node_coord(k.origin[0], lo[0] + c, k.spacing);
-            dxs[c] = dsub(nx, q.x);
-            dx2[c] = dsq(dxs[c]);
+            const double dxs = dsub(nx, q.x);
+            dx2[c] = dsq(dxs);
+            dxf[c] = (float)dxs;
         }
+        const int bx0 = lo[0] >> 3;
+        int64_t brow_cached = -1;
+        int s3[3] = {-1, -1, -1};
         for (int iz = 0; iz < cnt[2]; iz++) {
             const int cz = lo[2] + iz;
             double dz = 0, dz2 = 0;
             if (D == 3) {
                 dz = dsub(node_coord(k.origin[2], cz, k.spacing), q.z);
                 dz2 = dsq(dz);
+                if (!(dz2 < k.H2)) continue;   // no node of this plane within H
             }
             for (int iy = 0; iy < cnt[1]; iy++) {
                 const int cy = lo[1] + iy;
                 const double dy = dsub(node_coord(k.origin[1], cy, k.spacing), q.y);
                 const double dy2 = dsq(dy);
+                // dx^2 >= 0, so (dx^2 + dy^2) + dz^2 >= dy^2 + dz^2 in FP64
+                // as well: a row with dy^2 + dz^2 >= H^2 has no member
+                if (!(dadd(dy2, dz2) < k.H2)) continue;
                 const int64_t brow = ((int64_t)(D == 3 ? cz >> 3 : 0) * k.bdims[1] +
                                       (cy >> 3)) * k.bdims[0];
                 const int lrow = 8 * ((cy & 7) + (D == 3 ? 8 * (cz & 7) : 0));
                 constexpr int NB = D == 3 ? 512 : 64;
-                // the row's <= 10 nodes lie in at most 3 bricks along x
-                const int bx0 = lo[0] >> 3;
-                int s3[3];
+                // the row's <= 10 nodes lie in at most 3 bricks along x; the
+                // slots change only when the row crosses into another brick row
+                if (brow != brow_cached) {
+                    brow_cached = brow;
 #pragma unroll
-                for (int u = 0; u < 3; u++)
-                    s3[u] = bx0 + u < k.bdims[0] ? slot_of_brick[brow + bx0 + u] : -1;
+                    for (int u = 0; u < 3; u++)
+                        s3[u] = bx0 + u < k.bdims[0] ? slot_of_brick[brow + bx0 + u] : -1;
+                }
                 // issue the q loads of every node inside the sphere at once
-                float qv[kBox];
+                float qv[kBox], d2f[kBox];
 #pragma unroll
                 for (int c = 0; c < kBox; c++) {
                     const double d2 = D == 3 ? dadd(dadd(dx2[c], dy2), dz2)
                                              : dadd(dx2[c], dy2);
+                    d2f[c] = (float)d2;
                     const int cx = lo[0] + c;
                     const int u = (cx >> 3) - bx0;
                     const int s = u == 0 ? s3[0] : (u == 1 ? s3[1] : s3[2]);
                     qv[c] = (c < cnt[0] && d2 < k.H2 && s >= 0)
                                 ? vq[(int64_t)s * NB + lrow + (cx & 7)] : 0.f;
                 }
+                const float dyf = (float)dy, dzf = (float)dz;
 #pragma unroll
                 for (int c = 0; c < kBox; c++) {
                     if (qv[c] == 0.f) continue;
-                    const double d2 = D == 3 ? dadd(dadd(dx2[c], dy2), dz2)
-                                             : dadd(dx2[c], dy2);
-                    const KW kH = kernel_f32(d2, k.H2, k.Hf, k.inv_H, k.c0H, k.c1H);
-                    if (kH.r > 0.f) {
-                        // gradK_H(x_i - x_J) = -W'(r) (x_i - x_J) / r
-                        const float sc = qv[c] * (-kH.dw / kH.r);
-                        fx -= sc * (float)dxs[c];
-                        fy -= sc * (float)dy;
-                        if (D == 3) fz -= sc * (float)dz;
-                    }
+                    // gradK_H(x_i - x_J) = S (x_i - x_J), S = -W'(r) / r; at
+                    // r = 0 the difference is zero
+                    const float rinv = __builtin_amdgcn_rsqf(d2f[c]);
+                    const float r = d2f[c] > 0.f ? d2f[c] * rinv : 0.f;
+                    const float S = spline_s(r * k.inv_H, rinv, k.c1H, k.inv_H);
+                    const float sc = qv[c] * S;
+                    fx -= sc * dxf[c];
+                    fy -= sc * dyf;
+                    if (D == 3) fz -= sc * dzf;
                 }
             }
         }

@@ -58,7 +58,13 @@ struct KConst {
     float mass;
     float rho0;
     float kappa;
-    float pad;
+    // FP32 pre-test of |x_j - q|^2 < h^2 on brick-relative coordinates:
+    // below h2f_lo inside, at or above h2f_hi outside, in between the exact
+    // FP64 test decides (the band covers the FP32 rounding of the relative
+    // coordinates and of the squared distance)
+    float h2f_lo;
+    float h2f_hi;
+    float pad[3];
 };
 
 // ---- FP64 helpers with the reference's rounding ----------------------
@@ -136,6 +142,23 @@ __device__ __forceinline__ KW kernel_f32(double d2, double kr2, float kr,
         o.dw = -c1 * omx * omx;
     }
     return o;
+}
+
+// FP32 cubic spline from x = r / kr (valid for 0 <= x): W and
+// S = -W'(r) / r, the factor of gradK(d) = S d, without divisions
+// (SmoothedParticles.h:60-72: W' = c1 x (3x - 2) below x = 1/2, so
+// -W'/r = c1 (2 - 3x) / kr there; -c1 (1 - x)^2 above, so -W'/r =
+// c1 (1 - x)^2 / r).  rinv = 1 / r is only used for x >= 1/2.
+__device__ __forceinline__ float spline_w(float x, float c0) {
+    if (x < 0.5f) return c0 * (6.f * x * x * (x - 1.f) + 1.f);
+    const float omx = 1.f - x;
+    return x < 1.f ? 2.f * c0 * omx * omx * omx : 0.f;
+}
+__device__ __forceinline__ float spline_s(float x, float rinv, float c1,
+                                          float inv_kr) {
+    if (x < 0.5f) return c1 * inv_kr * (2.f - 3.f * x);
+    const float omx = 1.f - x;
+    return x < 1.f ? c1 * omx * omx * rinv : 0.f;
 }
 
 // ---- launch geometry ----------------------------------------------------

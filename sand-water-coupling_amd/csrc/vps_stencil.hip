@@ -59,8 +59,6 @@ namespace {
 
 constexpr int kT = 14;             // halo tile side (8 + 2 * 3)
 constexpr int kT2 = kT * kT;
-constexpr int kT3 = kT * kT * kT;
-constexpr int kHaloIters3 = (kT3 + 63) / 64;   // 43
 constexpr int kHaloIters2 = (kT2 + 63) / 64;   // 4
 constexpr int kUpdBlocks = 256;                // grid of the CG update
 constexpr int kUpdThreads = 1024;
@@ -146,7 +144,7 @@ __device__ __forceinline__ double block_sum(double v, double* sbuf) {
 }
 
 // Plane sums P_k, k = 0..3, at the LDS point pl (row stride kT).
-template <typename T, typename C>
+template <int RS, typename T, typename C>
 __device__ __forceinline__ void plane4(const T* pl, const C& cc, T P[4]) {
     P[0] = P[1] = P[2] = P[3] = T(0);
 #pragma unroll
@@ -156,11 +154,11 @@ __device__ __forceinline__ void plane4(const T* pl, const C& cc, T P[4]) {
             if (a * a + b * b > 15) continue;
             T u;
             if (a == 0 && b == 0) u = pl[0];
-            else if (a == 0) u = pl[b * kT] + pl[-b * kT];
+            else if (a == 0) u = pl[b * RS] + pl[-b * RS];
             else if (b == 0) u = pl[a] + pl[-a];
             else
-                u = (pl[a + b * kT] + pl[-a + b * kT]) +
-                    (pl[a - b * kT] + pl[-a - b * kT]);
+                u = (pl[a + b * RS] + pl[-a + b * RS]) +
+                    (pl[a - b * RS] + pl[-a - b * RS]);
 #pragma unroll
             for (int kk = 0; kk <= 3; kk++) {
                 const int k2 = a * a + b * b + kk * kk;
@@ -235,161 +233,206 @@ __device__ __forceinline__ T load_input(const StencilArgs& a, uint32_t g,
     }
 }
 
+// 3D: one workgroup of 4 waves per 8^3 brick.  The radius-3 halo is loaded
+// as 16-byte half rows of the neighbour bricks (x-fastest 8^3 layout): every
+// (y, z) line of the tile is the upper half row of the left brick, the
+// center row and the lower half row of the right brick (x = -4 .. 11), so
+// loads are float4 and contiguous within each brick.  The LDS tile keeps
+// rows of 24 floats (x = -8 .. 15, stride chosen so that the 8 x 8 columns
+// of a wave fall in distinct banks).  The 14 plane sums P_0..P_3 of every
+// (x, y) column are computed once (plane zp by wave zp mod 4) into LDS, and
+// wave w then forms the outputs of planes z = 2w, 2w + 1 as
+// P_3(z) + P_2(z+1) + P_1(z+2) + P_0(z+3) + P_1(z+4) + P_2(z+5) + P_3(z+6)
+// (tile planes, in that order).
+constexpr int kS3Threads = 256;
+constexpr int kRS = 24;                     // tile row stride (floats)
+constexpr int kPS = kRS * kT;               // tile plane stride
+constexpr int kUnits = kT * kT * 4;         // 16-byte loads per tile: 784
+constexpr int kUnitIters = (kUnits + kS3Threads - 1) / kS3Threads;   // 4
+
+// Input of the operator for 4 consecutive nodes at g (a multiple of 4):
+// the tile value (T) and the raw input p (for own[] and p_new).
+template <int KIND, int OP, typename T>
+__device__ __forceinline__ void load_input4(const StencilArgs& a, uint32_t g,
+                                            float beta, T v[4], float pn[4]) {
+    if constexpr (KIND == K_DIAG) {
+        const float4 V = *(const float4*)(a.vvol + g);
+        v[0] = (T)V.x; v[1] = (T)V.y; v[2] = (T)V.z; v[3] = (T)V.w;
+        pn[0] = pn[1] = pn[2] = pn[3] = 0.f;
+    } else {
+        float4 p;
+        if constexpr (KIND == K_SPMV_FUSED) {
+            const float4 r = *(const float4*)(a.in2 + g);
+            const float4 q = *(const float4*)(a.in + g);
+            p = make_float4(r.x + beta * q.x, r.y + beta * q.y, r.z + beta * q.z,
+                            r.w + beta * q.w);
+        } else {
+            p = *(const float4*)(a.in + g);
+        }
+        pn[0] = p.x; pn[1] = p.y; pn[2] = p.z; pn[3] = p.w;
+        if constexpr (OP == VPS_OP_ROW) {
+            const float4 V = *(const float4*)(a.vvol + g);
+            v[0] = (T)V.x * (T)p.x; v[1] = (T)V.y * (T)p.y;
+            v[2] = (T)V.z * (T)p.z; v[3] = (T)V.w * (T)p.w;
+        } else {
+            v[0] = (T)p.x; v[1] = (T)p.y; v[2] = (T)p.z; v[3] = (T)p.w;
+        }
+    }
+}
+
 template <int KIND, int OP, typename T, typename C>
-__global__ void __launch_bounds__(64) k_stencil3(StencilArgs a, C cc) {
-    __shared__ T tile[kHaloIters3 * 64];
-    __shared__ float own[512];   // raw own-brick input (p / x)
-    __shared__ uint32_t sbase[28];
+__global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
+    __shared__ T tile[kT * kPS];
+    __shared__ T psum[kT][4][64];   // plane sums P_k of plane zp, column l
+    __shared__ float own[512];      // raw own-brick input (p / x)
+    __shared__ uint32_t sbase[27];
+    __shared__ double sred[2][4];
     float beta = 0.f;
     if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_FUSED) {
-        if (!spmv_prologue(a, &beta)) return;
+        if (!spmv_prologue(a, &beta)) return;   // uniform over the grid
     }
     const int slot = xcd_remap(blockIdx.x, (int)a.nslots);
     const int brick = a.brick_of_slot[slot];
     const int t = threadIdx.x;
-    if (t < 28) {
+    const int l = t & 63, wv = t >> 6;
+    if (t < 27) {
         int64_t s = a.zero_slot;
-        if (t < 27) {
-            const int bx = brick % a.k.bdims[0];
-            const int by = (brick / a.k.bdims[0]) % a.k.bdims[1];
-            const int bz = brick / a.k.bdims[0] / a.k.bdims[1];
-            const int nx = bx + (t % 3) - 1, ny = by + (t / 3) % 3 - 1,
-                      nz = bz + t / 9 - 1;
-            if (nx >= 0 && ny >= 0 && nz >= 0 && nx < a.k.bdims[0] &&
-                ny < a.k.bdims[1] && nz < a.k.bdims[2]) {
-                const int sl = a.slot_of_brick[((int64_t)nz * a.k.bdims[1] + ny) *
-                                                   a.k.bdims[0] + nx];
-                if (sl >= 0) s = sl;
-            }
+        const int bx = brick % a.k.bdims[0];
+        const int by = (brick / a.k.bdims[0]) % a.k.bdims[1];
+        const int bz = brick / a.k.bdims[0] / a.k.bdims[1];
+        const int nx = bx + (t % 3) - 1, ny = by + (t / 3) % 3 - 1,
+                  nz = bz + t / 9 - 1;
+        if (nx >= 0 && ny >= 0 && nz >= 0 && nx < a.k.bdims[0] &&
+            ny < a.k.bdims[1] && nz < a.k.bdims[2]) {
+            const int sl = a.slot_of_brick[((int64_t)nz * a.k.bdims[1] + ny) *
+                                               a.k.bdims[0] + nx];
+            if (sl >= 0) s = sl;
         }
         sbase[t] = (uint32_t)(s * 512);
     }
-    const int x = t & 7, y = t >> 3;
+    const int x = l & 7, y = l >> 3;
     const int64_t base = (int64_t)slot * 512;
-    // epilogue operands of this lane's 8 outputs, fetched early
-    float Vz[8], Dz[8], Bz[8];
+    // epilogue operands of this lane's 2 outputs, fetched early
+    float Vz[2], Dz[2], Bz[2];
 #pragma unroll
-    for (int z = 0; z < 8; z++) {
-        const int64_t g = base + x + 8 * y + 64 * z;
-        Vz[z] = a.vvol[g];
-        if constexpr (KIND != K_DIAG && KIND != K_TRUE_RES) Dz[z] = a.vdiag[g];
-        else Dz[z] = 0.f;
-        if constexpr (KIND == K_CG_INIT || KIND == K_TRUE_RES) Bz[z] = a.vb[g];
-        else Bz[z] = 0.f;
+    for (int q = 0; q < 2; q++) {
+        const int64_t g = base + x + 8 * y + 64 * (2 * wv + q);
+        Vz[q] = a.vvol[g];
+        Dz[q] = (KIND != K_DIAG && KIND != K_TRUE_RES) ? a.vdiag[g] : 0.f;
+        Bz[q] = (KIND == K_CG_INIT || KIND == K_TRUE_RES) ? a.vb[g] : 0.f;
     }
     __syncthreads();
-    // Halo load: element e = t + 64 i of the 14^3 tile, decoded
-    // arithmetically; 4 chunks of 11 loads, each chunk in flight at once.
-    constexpr int kChunk = 11;
-#pragma unroll 1
-    for (int i0 = 0; i0 < kHaloIters3; i0 += kChunk) {
-        T val[kChunk];
-        float pnv[kChunk];
+    // Halo load: unit u = (line (yt, zt), part k): k = 0 the upper half row of
+    // the left brick (x = -4..-1), 1 and 2 the center row, 3 the lower half
+    // row of the right brick (x = 8..11).
+    {
+        T val[kUnitIters][4];
+        float pnv[kUnitIters][4];
 #pragma unroll
-        for (int j = 0; j < kChunk; j++) {
-            const int e = t + 64 * (i0 + j);
-            const int zz = e / kT2;
-            const int rem = e - zz * kT2;
-            const int yy = rem / kT;
-            const int xx = rem - yy * kT;
-            const int ux = xx + 5, uy = yy + 5, uz = zz + 5;
-            const int nb = e < kT3 ? (uz >> 3) * 9 + (uy >> 3) * 3 + (ux >> 3) : 27;
-            const uint32_t g =
-                sbase[nb] + (uint32_t)((ux & 7) + 8 * ((uy & 7) + 8 * (uz & 7)));
-            float pn = 0.f;
-            val[j] = i0 + j < kHaloIters3
-                         ? load_input<KIND, OP, T>(a, g, beta, &pn) : T(0);
-            pnv[j] = pn;
+        for (int i = 0; i < kUnitIters; i++) {
+            const int u = t + kS3Threads * i;
+            if (u < kUnits) {
+                const int k = u & 3, line = u >> 2;
+                const int yt = line % kT, zt = line / kT;
+                const int yy = yt + 5, zz = zt + 5;   // + 8 - 3: 0..23
+                const int bxo = k == 0 ? 0 : (k == 3 ? 2 : 1);
+                const int nb = (zz >> 3) * 9 + (yy >> 3) * 3 + bxo;
+                const int xl = (k == 0 || k == 2) ? 4 : 0;
+                const uint32_t g = sbase[nb] + (uint32_t)(xl + 8 * ((yy & 7) + 8 * (zz & 7)));
+                load_input4<KIND, OP, T>(a, g, beta, val[i], pnv[i]);
+            }
         }
 #pragma unroll
-        for (int j = 0; j < kChunk; j++) {
-            if (i0 + j >= kHaloIters3) continue;
-            const int e = t + 64 * (i0 + j);
-            tile[e] = val[j];
-            if constexpr (KIND != K_DIAG) {
-                const int zz = e / kT2;
-                const int rem = e - zz * kT2;
-                const int yy = rem / kT;
-                const int xx = rem - yy * kT;
-                if (xx >= 3 && xx < 11 && yy >= 3 && yy < 11 && zz >= 3 && zz < 11) {
-                    const int loc = (xx - 3) + 8 * ((yy - 3) + 8 * (zz - 3));
-                    own[loc] = pnv[j];
-                    if constexpr (KIND == K_SPMV_FUSED) a.out2[base + loc] = pnv[j];
+        for (int i = 0; i < kUnitIters; i++) {
+            const int u = t + kS3Threads * i;
+            if (u < kUnits) {
+                const int k = u & 3, line = u >> 2;
+                const int yt = line % kT, zt = line / kT;
+                T* dst = &tile[zt * kPS + yt * kRS + 4 + 4 * k];
+#pragma unroll
+                for (int c = 0; c < 4; c++) dst[c] = val[i][c];
+                if constexpr (KIND != K_DIAG) {
+                    if ((k == 1 || k == 2) && yt >= 3 && yt < 11 && zt >= 3 && zt < 11) {
+                        const int loc = 4 * (k - 1) + 8 * ((yt - 3) + 8 * (zt - 3));
+#pragma unroll
+                        for (int c = 0; c < 4; c++) own[loc + c] = pnv[i][c];
+                        if constexpr (KIND == K_SPMV_FUSED)
+                            *(float4*)(a.out2 + base + loc) =
+                                make_float4(pnv[i][0], pnv[i][1], pnv[i][2], pnv[i][3]);
+                    }
                 }
             }
         }
+    }
+    __syncthreads();
+    // plane sums, each plane once; column (x, y) sits at tile x = x + 8
+    for (int zp = wv; zp < kT; zp += 4) {
+        T P[4];
+        plane4<kRS, T>(&tile[zp * kPS + (y + 3) * kRS + (x + 8)], cc, P);
+#pragma unroll
+        for (int q = 0; q < 4; q++) psum[zp][q][l] = P[q];
     }
     __syncthreads();
     double red0 = 0, red1 = 0;
-    // Sliding window: after plane zp (z offset zz = zp - 3) the outputs
-    // z = zz - 3 .. zz + 3 have pending sums W[0..6]; z = zz - 3 is complete.
-    T W[7];
 #pragma unroll
-    for (int j = 0; j < 7; j++) W[j] = T(0);
-#pragma unroll 1
-    for (int zp = 0; zp < kT; zp++) {
-        T P[4];
-        plane4<T>(&tile[zp * kT2 + (y + 3) * kT + (x + 3)], cc, P);
-#pragma unroll
-        for (int j = 0; j < 7; j++) W[j] += P[j < 3 ? 3 - j : j - 3];
-        const int z = zp - 6;
-        if (z >= 0) {
-            const T acc = W[0];
-            const int loc = x + 8 * y + 64 * z;
-            const int64_t g = base + loc;
-            // z is uniform across the wave: select the prefetched operands
-            float V = 0.f, Dg = 0.f, Bg = 0.f;
-#pragma unroll
-            for (int q = 0; q < 8; q++)
-                if (q == z) {
-                    V = Vz[q];
-                    Dg = Dz[q];
-                    Bg = Bz[q];
-                }
-            const bool on = V > 0.f;
-            if constexpr (KIND == K_DIAG) {
-                const double sd = -(double)acc;
-                const double d = on ? (sd < a.alpha0 ? a.alpha0 : sd) : 0.0;
-                a.out_d[g] = d;
-                a.out_f[g] = (float)d;
-                a.out_s[g] = on ? (float)sd : 0.f;
-            } else if constexpr (KIND == K_TRUE_RES) {
-                const double xi = (double)own[loc];
-                const double st = OP == VPS_OP_ROW ? (double)acc
-                                                   : (double)V * (double)acc;
-                const double mx = a.vdiag64[g] * xi + st;
-                const double rt = on ? (double)Bg - mx : 0.0;
-                a.out[g] = (float)rt;
-                red0 += rt * rt;
+    for (int q = 0; q < 2; q++) {
+        const int z = 2 * wv + q;
+        T acc = T(0);
+        acc += psum[z][3][l];
+        acc += psum[z + 1][2][l];
+        acc += psum[z + 2][1][l];
+        acc += psum[z + 3][0][l];
+        acc += psum[z + 4][1][l];
+        acc += psum[z + 5][2][l];
+        acc += psum[z + 6][3][l];
+        const int loc = x + 8 * y + 64 * z;
+        const int64_t g = base + loc;
+        const float V = Vz[q], Dg = Dz[q], Bg = Bz[q];
+        const bool on = V > 0.f;
+        if constexpr (KIND == K_DIAG) {
+            const double sd = -(double)acc;
+            const double d = on ? (sd < a.alpha0 ? a.alpha0 : sd) : 0.0;
+            a.out_d[g] = d;
+            a.out_f[g] = (float)d;
+            a.out_s[g] = on ? (float)sd : 0.f;
+        } else if constexpr (KIND == K_TRUE_RES) {
+            const double xi = (double)own[loc];
+            const double st = OP == VPS_OP_ROW ? (double)acc : (double)V * (double)acc;
+            const double mx = a.vdiag64[g] * xi + st;
+            const double rt = on ? (double)Bg - mx : 0.0;
+            a.out[g] = (float)rt;
+            red0 += rt * rt;
+        } else {
+            const float pi = own[loc];
+            const float st = OP == VPS_OP_ROW ? (float)acc : V * (float)acc;
+            const float mp = on ? Dg * pi + st : 0.f;
+            if constexpr (KIND == K_APPLY) {
+                a.out[g] = mp;
+            } else if constexpr (KIND == K_CG_INIT) {
+                const float b = Bg;
+                const float r = on ? b - mp : 0.f;
+                a.out[g] = r;
+                a.out2[g] = r;
+                red0 += (double)b * (double)b;
+                red1 += (double)r * (double)r;
             } else {
-                const float pi = own[loc];
-                const float st = OP == VPS_OP_ROW ? (float)acc : V * (float)acc;
-                const float mp = on ? Dg * pi + st : 0.f;
-                if constexpr (KIND == K_APPLY) {
-                    a.out[g] = mp;
-                } else if constexpr (KIND == K_CG_INIT) {
-                    const float b = Bg;
-                    const float r = on ? b - mp : 0.f;
-                    a.out[g] = r;
-                    a.out2[g] = r;
-                    red0 += (double)b * (double)b;
-                    red1 += (double)r * (double)r;
-                } else {
-                    a.out[g] = mp;
-                    red0 += (double)pi * (double)mp;
-                }
+                a.out[g] = mp;
+                red0 += (double)pi * (double)mp;
             }
         }
-#pragma unroll
-        for (int j = 0; j < 6; j++) W[j] = W[j + 1];
-        W[6] = T(0);
     }
     if constexpr (KIND != K_DIAG && KIND != K_APPLY) {
         red0 = wave_sum(red0);
         if (KIND == K_CG_INIT) red1 = wave_sum(red1);
+        if (l == 0) {
+            sred[0][wv] = red0;
+            sred[1][wv] = red1;
+        }
+        __syncthreads();
         if (t == 0) {
-            a.part0[blockIdx.x] = red0;
-            if (KIND == K_CG_INIT) a.part1[blockIdx.x] = red1;
+            a.part0[blockIdx.x] = (sred[0][0] + sred[0][1]) + (sred[0][2] + sred[0][3]);
+            if (KIND == K_CG_INIT)
+                a.part1[blockIdx.x] = (sred[1][0] + sred[1][1]) + (sred[1][2] + sred[1][3]);
         }
     }
 }
@@ -704,13 +747,13 @@ static StencilArgs base_args(vps_context* c) {
 template <int KIND, int OP, typename T, typename C>
 static void launch(vps_context* c, const StencilArgs& a, const C& cc,
                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
-    const dim3 grid((unsigned)c->nslots), block(64);
+    const dim3 grid((unsigned)c->nslots), block(64), block3(kS3Threads);
     if (c->dim == 3) {
         if (e0)
-            hipExtLaunchKernelGGL((k_stencil3<KIND, OP, T, C>), grid, block, 0,
+            hipExtLaunchKernelGGL((k_stencil3<KIND, OP, T, C>), grid, block3, 0,
                                   c->stream, e0, e1, 0, a, cc);
         else
-            hipLaunchKernelGGL((k_stencil3<KIND, OP, T, C>), grid, block, 0,
+            hipLaunchKernelGGL((k_stencil3<KIND, OP, T, C>), grid, block3, 0,
                                c->stream, a, cc);
     } else {
         if (e0)

@@ -69,6 +69,167 @@ __global__ void __launch_bounds__(256) k_assign_slots(const uint32_t* flag,
     }
 }
 
+// Real-neighbour sums of one node: w, sum W_H, sum V^2 W, sum V W (the v
+// denominator), sum v W / rho and the divergence terms.  dx = q - x_j.
+struct RealAcc {
+    float w = 0.f, sH = 0.f, sV = 0.f, den = 0.f;
+    float nvx = 0.f, nvy = 0.f, nvz = 0.f;   // sum v_j W / rho_j
+    float gx = 0.f, gy = 0.f, gz = 0.f;      // sum V_j gradK
+    float dr = 0.f;                           // sum V_j v_j . gradK
+    template <int D>
+    __device__ __forceinline__ void add(const KConst& k, double d2, float dx,
+                                        float dy, float dz, float4 vj,
+                                        float inv_rhoj) {
+        const KW kh = kernel_f32(d2, k.h2, k.hf, k.inv_h, k.c0h, k.c1h);
+        const KW kH = kernel_f32(d2, k.H2, k.Hf, k.inv_H, k.c0H, k.c1H);
+        const float Vj = vj.w;
+        const float wr = kh.w * inv_rhoj;
+        w += kh.w;
+        sH += kH.w;
+        sV += Vj * Vj * kh.w;
+        den += Vj * kh.w;
+        nvx += vj.x * wr;
+        nvy += vj.y * wr;
+        nvz += vj.z * wr;
+        if (kh.r > 0.f) {
+            // gradK(q - x_j) = -W'(r) (q - x_j) / r
+            const float s = -kh.dw / kh.r;
+            const float ex = dx * s, ey = dy * s, ez = D == 3 ? dz * s : 0.f;
+            gx += Vj * ex;
+            gy += Vj * ey;
+            gz += Vj * ez;
+            dr += Vj * (vj.x * ex + vj.y * ey + vj.z * ez);
+        }
+    }
+    // The same sums from an FP32 squared distance d2f < h^2 (0 < d2f), with
+    // the division-free spline.
+    template <int D>
+    __device__ __forceinline__ void add_fast(const KConst& k, float d2f, float dx,
+                                             float dy, float dz, float4 vj,
+                                             float inv_rhoj) {
+        const float rinv = __builtin_amdgcn_rsqf(d2f);
+        const float r = d2f * rinv;
+        const float xh = r * k.inv_h, xH = r * k.inv_H;
+        const float wh = spline_w(xh, k.c0h);
+        const float sh = spline_s(xh, rinv, k.c1h, k.inv_h);
+        const float Vj = vj.w;
+        const float wr = wh * inv_rhoj;
+        w += wh;
+        sH += spline_w(xH, k.c0H);
+        sV += Vj * Vj * wh;
+        den += Vj * wh;
+        nvx += vj.x * wr;
+        nvy += vj.y * wr;
+        nvz += vj.z * wr;
+        const float ex = dx * sh, ey = dy * sh, ez = D == 3 ? dz * sh : 0.f;
+        gx += Vj * ex;
+        gy += Vj * ey;
+        gz += Vj * ez;
+        dr += Vj * (vj.x * ex + vj.y * ey + vj.z * ez);
+    }
+};
+
+// Everything after the real walk: the exact threshold decision, the boundary
+// walk, v_I, D_I and the outputs.
+// Boundary walk of one node from the global cell table.
+template <int D>
+struct BoundaryWalkGlobal {
+    const int32_t* bcstart;
+    const double4* bpos;
+    template <typename F>
+    __device__ __forceinline__ void operator()(const KConst& k, double qx,
+                                               double qy, double qz, F&& f) const {
+        for_each_candidate<D>(k, bcstart, bpos, qx, qy, qz, f);
+    }
+};
+
+template <int D, typename BW>
+__device__ __forceinline__ void finish_node(
+    const KConst& k, const RealAcc& a, double qx, double qy, double qz,
+    int cx, int cy, int cz, int64_t out, const double4* rpos,
+    const int32_t* rcstart, const BW& bwalk, const float4* bvel,
+    const float4* bnrm, float* vrho, float* vvol, float* vb, uint32_t* bits) {
+    bool virt = a.w > 1e-6f;
+    if (fabsf(a.w - 1e-6f) <= 1e-10f) {
+        // exact FP64 decision near the threshold (VirtualParticle.cpp:56)
+        double w64 = 0;
+        for_each_candidate<D>(k, rcstart, rpos, qx, qy, qz, [&](int j, const double4& pj) {
+            const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz)
+                                     : sq_dist2(pj.x, pj.y, qx, qy);
+            if (d2 < k.h2) w64 += kernel_w64(d2, k.inv_h64, k.c0h64);
+        });
+        virt = w64 > 1e-6;
+    }
+    if (!virt) {
+        vrho[out] = 0.f;
+        vvol[out] = 0.f;
+        vb[out] = 0.f;
+        return;
+    }
+    // boundary neighbours within h
+    float den = a.den;
+    float bnx = 0.f, bny = 0.f, bnz = 0.f;
+    float hx = 0.f, hy = 0.f, hz = 0.f, db = 0.f;
+    bwalk(k, qx, qy, qz, [&](int j, const double4& pj) {
+        const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz)
+                                 : sq_dist2(pj.x, pj.y, qx, qy);
+        if (d2 < k.h2) {
+            const KW kh = kernel_f32(d2, k.h2, k.hf, k.inv_h, k.c0h, k.c1h);
+            const float4 vj = bvel[j];
+            const float Vj = vj.w;
+            const float wr = kh.w / bnrm[j].w;
+            den += Vj * kh.w;
+            bnx += vj.x * wr;
+            bny += vj.y * wr;
+            bnz += vj.z * wr;
+            if (kh.r > 0.f) {
+                const float s = -kh.dw / kh.r;
+                const float ex = (float)(qx - pj.x) * s;
+                const float ey = (float)(qy - pj.y) * s;
+                const float ez = D == 3 ? (float)(qz - pj.z) * s : 0.f;
+                hx += Vj * ex;
+                hy += Vj * ey;
+                hz += Vj * ez;
+                db += Vj * (vj.x * ex + vj.y * ey + vj.z * ez);
+            }
+        }
+    });
+    // v_I = (m sum v_j W/rho_j + m_b sum v_b W/rho_b) / (sum V W + sum V_b W + 1e-6)
+    const float inv_den = 1.f / (den + 1e-6f);
+    const float ux = (k.mass * a.nvx + k.mass * bnx) * inv_den;
+    const float uy = (k.mass * a.nvy + k.mass * bny) * inv_den;
+    const float uz = (k.mass * a.nvz + k.mass * bnz) * inv_den;
+    const float rho = k.mass * a.sH;
+    const float drho = rho - k.rho0;
+    const float D_I = a.dr - (ux * a.gx + uy * a.gy + uz * a.gz) + db -
+                      (ux * hx + uy * hy + uz * hz) -
+                      k.kappa * (drho > 0.f ? drho : 0.f) / k.rho0;
+    vrho[out] = rho;
+    vvol[out] = a.sV;
+    vb[out] = D_I;
+    const int64_t lin = ((int64_t)cz * k.ndims[1] + cy) * k.ndims[0] + cx;
+    atomicOr(&bits[lin >> 5], 1u << (lin & 31));
+}
+
+// The real walk straight from global memory (2D, and 3D bricks whose
+// neighbourhood does not fit the LDS stage).
+template <int D>
+__device__ __forceinline__ void real_walk_global(const KConst& k,
+                                                 const int32_t* rcstart,
+                                                 const double4* rpos,
+                                                 const float4* rvel,
+                                                 const float* rrho, double qx,
+                                                 double qy, double qz,
+                                                 RealAcc& a) {
+    for_each_candidate<D>(k, rcstart, rpos, qx, qy, qz, [&](int j, const double4& pj) {
+        const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz)
+                                 : sq_dist2(pj.x, pj.y, qx, qy);
+        if (d2 < k.h2)
+            a.add<D>(k, d2, (float)(qx - pj.x), (float)(qy - pj.y),
+                     D == 3 ? (float)(qz - pj.z) : 0.f, rvel[j], 1.f / rrho[j]);
+    });
+}
+
 // Fused per-node pass.  One workgroup = one brick; thread = node.
 template <int D, int NB>
 __global__ void __launch_bounds__(NB) k_virtual_sums(
@@ -96,99 +257,239 @@ __global__ void __launch_bounds__(NB) k_virtual_sums(
     const double qx = node_coord(k.origin[0], cx, k.spacing);
     const double qy = node_coord(k.origin[1], cy, k.spacing);
     const double qz = D == 3 ? node_coord(k.origin[2], cz, k.spacing) : 0.0;
+    RealAcc a;
+    real_walk_global<D>(k, rcstart, rpos, rvel, rrho, qx, qy, qz, a);
+    finish_node<D>(k, a, qx, qy, qz, cx, cy, cz, out, rpos, rcstart,
+                   BoundaryWalkGlobal<D>{bcstart, bpos}, bvel, bnrm, vrho, vvol,
+                   vb, bits);
+}
 
-    float w = 0.f, sH = 0.f, sV = 0.f, den = 0.f;
-    float nvx = 0.f, nvy = 0.f, nvz = 0.f;  // sum v_j W / rho_j (real)
-    float gx = 0.f, gy = 0.f, gz = 0.f;     // sum V_j gradK (real)
-    float dr = 0.f;                          // sum V_j v_j . gradK (real)
-    for_each_candidate<D>(k, rcstart, rpos, qx, qy, qz, [&](int j, const double4& pj) {
-        const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz)
-                                 : sq_dist2(pj.x, pj.y, qx, qy);
-        if (d2 < k.h2) {
-            const KW kh = kernel_f32(d2, k.h2, k.hf, k.inv_h, k.c0h, k.c1h);
-            const KW kH = kernel_f32(d2, k.H2, k.Hf, k.inv_H, k.c0H, k.c1H);
-            const float4 vj = rvel[j];
-            const float Vj = vj.w;
-            const float wr = kh.w / rrho[j];
-            w += kh.w;
-            sH += kH.w;
-            sV += Vj * Vj * kh.w;
-            den += Vj * kh.w;
-            nvx += vj.x * wr;
-            nvy += vj.y * wr;
-            nvz += vj.z * wr;
-            if (kh.r > 0.f) {
-                // gradK(q - x_j) = -W'(r) (q - x_j) / r
-                const float s = -kh.dw / kh.r;
-                const float ex = (float)(qx - pj.x) * s;
-                const float ey = (float)(qy - pj.y) * s;
-                const float ez = D == 3 ? (float)(qz - pj.z) * s : 0.f;
-                gx += Vj * ex;
-                gy += Vj * ey;
-                gz += Vj * ez;
-                dr += Vj * (vj.x * ex + vj.y * ey + vj.z * ez);
+// 3D with the real neighbourhood of half a brick (8 x 8 x 4 nodes, one
+// workgroup of 256 threads) staged in LDS.  The cells within one cell of
+// those nodes form <= kStageRows x-rows of <= kStageCx cells; their particles
+// (contiguous per row in the cell-sorted order) are copied once -- the FP32
+// offset from their own cell corner, 1/rho, v and V -- and every node walks
+// its 3 x 3 x 3 cells from LDS in the order of the global walk.
+// q - x_j = (q - C_q) - (x_j - C_j) + (C_q - C_j) with C the FP64 cell
+// corners, so the FP32 difference is accurate to a few ulp of h.
+// Membership is pre-tested in FP32 with a band around h^2 in which the
+// exact FP64 test on the original positions decides (also for particles
+// clamped into a border cell), so the neighbour sets stay exact.  The
+// boundary cell table of the same rows is staged too, so nodes away from
+// the walls skip the boundary walk without touching global memory.
+constexpr int kStageCap = 2304;
+constexpr int kStageRows = 64;
+constexpr int kStageCx = 8;
+
+__device__ __forceinline__ double cell_corner(const KConst& k, int a, int c) {
+    return k.cell_lo[a] + (double)c / k.cell_inv;
+}
+
+struct BoundaryWalkStaged {
+    const int (*bcs)[kStageCx + 1];   // global starts of the staged rows
+    const int* meta;                  // clo x,y,z; ncx, ncy
+    const double4* bpos;
+    template <typename F>
+    __device__ __forceinline__ void operator()(const KConst& k, double qx,
+                                               double qy, double qz, F&& f) const {
+        const int cx = cell_coord(qx, k.cell_lo[0], k.cell_inv, k.cdims[0]);
+        const int cy = cell_coord(qy, k.cell_lo[1], k.cell_inv, k.cdims[1]);
+        const int cz = cell_coord(qz, k.cell_lo[2], k.cell_inv, k.cdims[2]);
+        const int i0 = max(cx - 1, 0) - meta[0];
+        const int i1 = min(cx + 1, k.cdims[0] - 1) - meta[0] + 1;
+        for (int z = max(cz - 1, 0); z <= min(cz + 1, k.cdims[2] - 1); z++)
+            for (int y = max(cy - 1, 0); y <= min(cy + 1, k.cdims[1] - 1); y++) {
+                const int r = (z - meta[2]) * meta[4] + (y - meta[1]);
+                const int b = bcs[r][i0], e = bcs[r][i1];
+                for (int j = b; j < e; j++) f(j, bpos[j]);
+            }
+    }
+};
+
+__global__ void __launch_bounds__(256) k_virtual_sums3s(
+    KConst k, const int32_t* brick_of_slot, int64_t nslots,
+    const double4* rpos, const float4* rvel, const float* rrho,
+    const int32_t* rcstart, const double4* bpos, const float4* bvel,
+    const float4* bnrm, const int32_t* bcstart, float* vrho, float* vvol,
+    float* vb, uint32_t* bits) {
+    __shared__ float4 s_p[kStageCap];   // (x_j - C_j, +-1/rho_j; < 0: exact only)
+    __shared__ float4 s_v[kStageCap];   // (v_j, V_j)
+    __shared__ int s_cs[kStageRows][kStageCx + 1];  // real cell starts (staged)
+    __shared__ int s_bcs[kStageRows][kStageCx + 1]; // boundary cell starts
+    __shared__ int s_rg[kStageRows];    // global - staged index
+    __shared__ int s_off[kStageRows + 1];
+    __shared__ int s_meta[8];   // clo x,y,z; ncx, ncy, ncz; total; ok
+    const int blk = xcd_remap(blockIdx.x, (int)(2 * nslots));
+    const int slot = blk >> 1, half = blk & 1;
+    const int brick = brick_of_slot[slot];
+    const int t = threadIdx.x;
+    const int lx = t & 7, ly = (t >> 3) & 7, lz = 4 * half + (t >> 6);
+    const int bx = brick % k.bdims[0];
+    const int by = (brick / k.bdims[0]) % k.bdims[1];
+    const int bz = brick / k.bdims[0] / k.bdims[1];
+    const int n0[3] = {bx * kBrick, by * kBrick, bz * kBrick + 4 * half};
+    const int span[3] = {kBrick, kBrick, 4};
+    if (t == 0) {
+        int ok = 1, nrows = 1;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const int first = min(n0[a], k.ndims[a] - 1);
+            const int last = min(n0[a] + span[a] - 1, k.ndims[a] - 1);
+            const int clo = max(cell_coord(node_coord(k.origin[a], first, k.spacing),
+                                           k.cell_lo[a], k.cell_inv, k.cdims[a]) - 1, 0);
+            const int chi = min(cell_coord(node_coord(k.origin[a], last, k.spacing),
+                                           k.cell_lo[a], k.cell_inv, k.cdims[a]) + 1,
+                                k.cdims[a] - 1);
+            s_meta[a] = clo;
+            s_meta[3 + a] = chi - clo + 1;
+            if (a > 0) nrows *= chi - clo + 1;
+        }
+        if (s_meta[3] > kStageCx || nrows > kStageRows) ok = 0;
+        s_meta[7] = ok;
+    }
+    __syncthreads();
+    const int ncx = s_meta[3], ncy = s_meta[4], ncz = s_meta[5];
+    const int nrows = ncy * ncz;
+    if (s_meta[7] && t < nrows) {
+        const int y = s_meta[1] + t % ncy, z = s_meta[2] + t / ncy;
+        const int64_t grow = ((int64_t)z * k.cdims[1] + y) * k.cdims[0] + s_meta[0];
+        for (int i = 0; i <= ncx; i++) {
+            s_cs[t][i] = rcstart[grow + i];
+            s_bcs[t][i] = bcstart[grow + i];
+        }
+    }
+    __syncthreads();
+    if (t == 0 && s_meta[7]) {
+        int acc = 0;
+        for (int r = 0; r < nrows; r++) {
+            s_off[r] = acc;
+            acc += s_cs[r][ncx] - s_cs[r][0];
+        }
+        s_off[nrows] = acc;
+        s_meta[6] = acc;
+        if (acc > kStageCap) s_meta[7] = 0;
+    }
+    __syncthreads();
+    const bool staged = s_meta[7] != 0;
+    if (staged) {
+        if (t < nrows) {
+            const int g0 = s_cs[t][0];
+            s_rg[t] = g0 - s_off[t];
+            for (int i = 0; i <= ncx; i++) s_cs[t][i] += s_off[t] - g0;
+        }
+        __syncthreads();
+        const int total = s_meta[6];
+        const float far = 2.f * (float)(1.0 / k.cell_inv);
+        for (int e0 = 0; e0 < total; e0 += 4 * 256) {
+            double4 pj[4];
+            float4 vj[4];
+            float rj[4];
+            int er[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = e0 + u * 256 + t;
+                er[u] = -1;
+                if (e < total) {
+                    // row of staged element e: last r with s_off[r] <= e
+                    int lo = 0, hi = nrows - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (s_off[mid] <= e) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    er[u] = e + s_rg[lo];
+                    pj[u] = rpos[er[u]];
+                    vj[u] = rvel[er[u]];
+                    rj[u] = rrho[er[u]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = e0 + u * 256 + t;
+                if (er[u] < 0) continue;
+                const double pc[3] = {pj[u].x, pj[u].y, pj[u].z};
+                float of[3];
+                bool is_far = false;
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    const int c = cell_coord(pc[a], k.cell_lo[a], k.cell_inv, k.cdims[a]);
+                    of[a] = (float)(pc[a] - cell_corner(k, a, c));
+                    is_far |= !(fabsf(of[a]) <= far);
+                }
+                const float ir = 1.f / rj[u];
+                s_p[e] = make_float4(of[0], of[1], of[2], is_far ? -ir : ir);
+                s_v[e] = vj[u];
             }
         }
-    });
-    bool virt = w > 1e-6f;
-    if (fabsf(w - 1e-6f) <= 1e-10f) {
-        // exact FP64 decision near the threshold (VirtualParticle.cpp:56)
-        double w64 = 0;
-        for_each_candidate<D>(k, rcstart, rpos, qx, qy, qz, [&](int j, const double4& pj) {
-            const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz)
-                                     : sq_dist2(pj.x, pj.y, qx, qy);
-            if (d2 < k.h2) w64 += kernel_w64(d2, k.inv_h64, k.c0h64);
-        });
-        virt = w64 > 1e-6;
+        __syncthreads();
     }
-    if (!virt) {
+    const int cx = n0[0] + lx, cy = n0[1] + ly, cz = bz * kBrick + lz;
+    const int64_t out = (int64_t)slot * 512 + lx + 8 * ly + 64 * lz;
+    if (cx >= k.ndims[0] || cy >= k.ndims[1] || cz >= k.ndims[2]) {
         vrho[out] = 0.f;
         vvol[out] = 0.f;
         vb[out] = 0.f;
         return;
     }
-    // boundary neighbours within h
-    float bnx = 0.f, bny = 0.f, bnz = 0.f;
-    float hx = 0.f, hy = 0.f, hz = 0.f, db = 0.f;
-    for_each_candidate<D>(k, bcstart, bpos, qx, qy, qz, [&](int j, const double4& pj) {
-        const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz)
-                                 : sq_dist2(pj.x, pj.y, qx, qy);
-        if (d2 < k.h2) {
-            const KW kh = kernel_f32(d2, k.h2, k.hf, k.inv_h, k.c0h, k.c1h);
-            const float4 vj = bvel[j];
-            const float Vj = vj.w;
-            const float wr = kh.w / bnrm[j].w;
-            den += Vj * kh.w;
-            bnx += vj.x * wr;
-            bny += vj.y * wr;
-            bnz += vj.z * wr;
-            if (kh.r > 0.f) {
-                const float s = -kh.dw / kh.r;
-                const float ex = (float)(qx - pj.x) * s;
-                const float ey = (float)(qy - pj.y) * s;
-                const float ez = D == 3 ? (float)(qz - pj.z) * s : 0.f;
-                hx += Vj * ex;
-                hy += Vj * ey;
-                hz += Vj * ez;
-                db += Vj * (vj.x * ex + vj.y * ey + vj.z * ez);
+    const double qx = node_coord(k.origin[0], cx, k.spacing);
+    const double qy = node_coord(k.origin[1], cy, k.spacing);
+    const double qz = node_coord(k.origin[2], cz, k.spacing);
+    RealAcc a;
+    if (!staged) {
+        real_walk_global<3>(k, rcstart, rpos, rvel, rrho, qx, qy, qz, a);
+        finish_node<3>(k, a, qx, qy, qz, cx, cy, cz, out, rpos, rcstart,
+                       BoundaryWalkGlobal<3>{bcstart, bpos}, bvel, bnrm, vrho,
+                       vvol, vb, bits);
+        return;
+    }
+    const double qc[3] = {qx, qy, qz};
+    int cc[3];
+    float qo[3], corner[3][3];   // corner[a][d + 1] = C_q - C_(q + d)
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        cc[ax] = cell_coord(qc[ax], k.cell_lo[ax], k.cell_inv, k.cdims[ax]);
+        const double Cq = cell_corner(k, ax, cc[ax]);
+        qo[ax] = (float)(qc[ax] - Cq);
+#pragma unroll
+        for (int d = -1; d <= 1; d++)
+            corner[ax][d + 1] = (float)(Cq - cell_corner(k, ax, cc[ax] + d));
+    }
+    const int xlo = max(cc[0] - 1, 0), xhi = min(cc[0] + 1, k.cdims[0] - 1);
+    const int zlo = max(cc[2] - 1, 0), zhi = min(cc[2] + 1, k.cdims[2] - 1);
+    const int ylo = max(cc[1] - 1, 0), yhi = min(cc[1] + 1, k.cdims[1] - 1);
+    for (int z = zlo; z <= zhi; z++) {
+        const float tz = qo[2] + corner[2][z - cc[2] + 1];
+        for (int y = ylo; y <= yhi; y++) {
+            const float ty = qo[1] + corner[1][y - cc[1] + 1];
+            const int r = (z - s_meta[2]) * ncy + (y - s_meta[1]);
+            const int rg = s_rg[r];
+            for (int x = xlo; x <= xhi; x++) {
+                const float tx = qo[0] + corner[0][x - cc[0] + 1];
+                const int i = x - s_meta[0];
+                const int jb = s_cs[r][i], je = s_cs[r][i + 1];
+                for (int j = jb; j < je; j++) {
+                    const float4 pr = s_p[j];
+                    const float dx = tx - pr.x, dy = ty - pr.y, dz = tz - pr.z;
+                    const float d2f = dx * dx + dy * dy + dz * dz;
+                    if (d2f >= k.h2f_hi && pr.w >= 0.f) continue;
+                    if (d2f < k.h2f_lo && pr.w >= 0.f && d2f > 0.f) {
+                        a.add_fast<3>(k, d2f, dx, dy, dz, s_v[j], pr.w);
+                    } else {
+                        // near h, at r = 0, or far from its cell: the
+                        // reference's FP64 test and the accurate spline
+                        const double4 pj = rpos[j + rg];
+                        const double d2 = sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz);
+                        if (!(d2 < k.h2)) continue;
+                        a.add<3>(k, d2, (float)(qx - pj.x), (float)(qy - pj.y),
+                                 (float)(qz - pj.z), s_v[j], fabsf(pr.w));
+                    }
+                }
             }
         }
-    });
-    // v_I = (m sum v_j W/rho_j + m_b sum v_b W/rho_b) / (sum V W + sum V_b W + 1e-6)
-    const float inv_den = 1.f / (den + 1e-6f);
-    const float ux = (k.mass * nvx + k.mass * bnx) * inv_den;
-    const float uy = (k.mass * nvy + k.mass * bny) * inv_den;
-    const float uz = (k.mass * nvz + k.mass * bnz) * inv_den;
-    const float rho = k.mass * sH;
-    const float drho = rho - k.rho0;
-    const float D_I = dr - (ux * gx + uy * gy + uz * gz) + db -
-                      (ux * hx + uy * hy + uz * hz) -
-                      k.kappa * (drho > 0.f ? drho : 0.f) / k.rho0;
-    vrho[out] = rho;
-    vvol[out] = sV;
-    vb[out] = D_I;
-    const int64_t lin = ((int64_t)cz * k.ndims[1] + cy) * k.ndims[0] + cx;
-    atomicOr(&bits[lin >> 5], 1u << (lin & 31));
+    }
+    finish_node<3>(k, a, qx, qy, qz, cx, cy, cz, out, rpos, rcstart,
+                   BoundaryWalkStaged{s_bcs, s_meta, bpos}, bvel, bnrm, vrho,
+                   vvol, vb, bits);
 }
 
 __global__ void __launch_bounds__(256) k_popc(const uint32_t* bits,
@@ -300,8 +601,8 @@ int virtual_generate(vps_context* c) {
     }
     const int rs = c->cur;
     if (D == 3)
-        hipLaunchKernelGGL((k_virtual_sums<3, 512>), dim3((unsigned)nslots),
-                           dim3(512), 0, c->stream, c->k, c->slot_brick[w].p,
+        hipLaunchKernelGGL(k_virtual_sums3s, dim3((unsigned)(2 * nslots)),
+                           dim3(256), 0, c->stream, c->k, c->slot_brick[w].p,
                            nslots, c->rpos[rs].p, c->rvel[rs].p, c->rrho.p,
                            c->cell_start.p, c->bpos.p, c->bvel.p, c->bnrm.p,
                            c->bcell_start.p, c->vrho.p, c->vvol.p, c->vb.p,

