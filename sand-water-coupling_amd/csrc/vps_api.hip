@@ -161,6 +161,7 @@ void free_all(vps_context* c) {
         dfree(c->rid[b]);
         dfree(c->brick_slot[b]);
         dfree(c->slot_brick[b]);
+        dfree(c->nbr_slot);
         dfree(c->vx[b]);
         dfree(c->vp[b]);
         dfree(c->p_by_rank[b]);
@@ -306,6 +307,7 @@ int vps_create(const vps_params* params, vps_context** out) {
     ALLOC(c->vt, cap);
     ALLOC(c->vq, cap);
     ALLOC(c->vrank, cap);
+    ALLOC(c->nbr_slot, (size_t)(c->nbricks + 1) * 27);
     ALLOC(c->bits, c->nwords + 1);
     ALLOC(c->word_prefix, c->nwords + 1);
     ALLOC(c->partials, partials_size(c->nbricks));

@@ -336,6 +336,9 @@ struct vps_context {
     vps::DBuf<uint32_t> brick_flag;
     vps::DBuf<int32_t> brick_slot[2];
     vps::DBuf<int32_t> slot_brick[2];
+    // slots of the 27 neighbour bricks of every slot of the current layout
+    // (the zero brick nbricks where absent), built after each generation
+    vps::DBuf<int32_t> nbr_slot;
     // per-node fields, slot-major
     vps::DBuf<float> vrho, vvol, vb, vdiag, vx[2], vr, vp[2], vt, vq, vrowsum;
     vps::DBuf<double> vdiag64;

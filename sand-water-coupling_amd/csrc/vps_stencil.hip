@@ -97,6 +97,7 @@ struct StencilArgs {
     KConst k;
     const int32_t* slot_of_brick;
     const int32_t* brick_of_slot;
+    const int32_t* nbr_slot;   // [slot][27] neighbour slots (3D)
     int64_t nslots;
     int64_t zero_slot;     // index of the all-zero brick
     const float* vvol;
@@ -250,36 +251,6 @@ constexpr int kPS = kRS * kT;               // tile plane stride
 constexpr int kUnits = kT * kT * 4;         // 16-byte loads per tile: 784
 constexpr int kUnitIters = (kUnits + kS3Threads - 1) / kS3Threads;   // 4
 
-// Input of the operator for 4 consecutive nodes at g (a multiple of 4):
-// the tile value (T) and the raw input p (for own[] and p_new).
-template <int KIND, int OP, typename T>
-__device__ __forceinline__ void load_input4(const StencilArgs& a, uint32_t g,
-                                            float beta, T v[4], float pn[4]) {
-    if constexpr (KIND == K_DIAG) {
-        const float4 V = *(const float4*)(a.vvol + g);
-        v[0] = (T)V.x; v[1] = (T)V.y; v[2] = (T)V.z; v[3] = (T)V.w;
-        pn[0] = pn[1] = pn[2] = pn[3] = 0.f;
-    } else {
-        float4 p;
-        if constexpr (KIND == K_SPMV_FUSED) {
-            const float4 r = *(const float4*)(a.in2 + g);
-            const float4 q = *(const float4*)(a.in + g);
-            p = make_float4(r.x + beta * q.x, r.y + beta * q.y, r.z + beta * q.z,
-                            r.w + beta * q.w);
-        } else {
-            p = *(const float4*)(a.in + g);
-        }
-        pn[0] = p.x; pn[1] = p.y; pn[2] = p.z; pn[3] = p.w;
-        if constexpr (OP == VPS_OP_ROW) {
-            const float4 V = *(const float4*)(a.vvol + g);
-            v[0] = (T)V.x * (T)p.x; v[1] = (T)V.y * (T)p.y;
-            v[2] = (T)V.z * (T)p.z; v[3] = (T)V.w * (T)p.w;
-        } else {
-            v[0] = (T)p.x; v[1] = (T)p.y; v[2] = (T)p.z; v[3] = (T)p.w;
-        }
-    }
-}
-
 template <int KIND, int OP, typename T, typename C>
 __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
     __shared__ T tile[kT * kPS];
@@ -287,28 +258,15 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
     __shared__ float own[512];      // raw own-brick input (p / x)
     __shared__ uint32_t sbase[27];
     __shared__ double sred[2][4];
-    float beta = 0.f;
-    if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_FUSED) {
-        if (!spmv_prologue(a, &beta)) return;   // uniform over the grid
-    }
+    __shared__ int s_done;
     const int slot = xcd_remap(blockIdx.x, (int)a.nslots);
-    const int brick = a.brick_of_slot[slot];
     const int t = threadIdx.x;
     const int l = t & 63, wv = t >> 6;
-    if (t < 27) {
-        int64_t s = a.zero_slot;
-        const int bx = brick % a.k.bdims[0];
-        const int by = (brick / a.k.bdims[0]) % a.k.bdims[1];
-        const int bz = brick / a.k.bdims[0] / a.k.bdims[1];
-        const int nx = bx + (t % 3) - 1, ny = by + (t / 3) % 3 - 1,
-                  nz = bz + t / 9 - 1;
-        if (nx >= 0 && ny >= 0 && nz >= 0 && nx < a.k.bdims[0] &&
-            ny < a.k.bdims[1] && nz < a.k.bdims[2]) {
-            const int sl = a.slot_of_brick[((int64_t)nz * a.k.bdims[1] + ny) *
-                                               a.k.bdims[0] + nx];
-            if (sl >= 0) s = sl;
-        }
-        sbase[t] = (uint32_t)(s * 512);
+    if (t < 27) sbase[t] = (uint32_t)a.nbr_slot[(int64_t)slot * 27 + t] * 512u;
+    if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_FUSED) {
+        // a solve that has ended skips the halo loads (launches past the
+        // convergence of a batch)
+        if (t == 32) s_done = a.st->done;
     }
     const int x = l & 7, y = l >> 3;
     const int64_t base = (int64_t)slot * 512;
@@ -322,44 +280,73 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
         Bz[q] = (KIND == K_CG_INIT || KIND == K_TRUE_RES) ? a.vb[g] : 0.f;
     }
     __syncthreads();
+    if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_FUSED) {
+        if (s_done) return;   // uniform over the block
+    }
     // Halo load: unit u = (line (yt, zt), part k): k = 0 the upper half row of
     // the left brick (x = -4..-1), 1 and 2 the center row, 3 the lower half
-    // row of the right brick (x = 8..11).
-    {
-        T val[kUnitIters][4];
-        float pnv[kUnitIters][4];
+    // row of the right brick (x = 8..11).  The raw loads do not depend on
+    // beta, so they are issued before the prologue's reductions.
+    float4 ra[kUnitIters], rb[kUnitIters], rv[kUnitIters];
 #pragma unroll
-        for (int i = 0; i < kUnitIters; i++) {
-            const int u = t + kS3Threads * i;
-            if (u < kUnits) {
-                const int k = u & 3, line = u >> 2;
-                const int yt = line % kT, zt = line / kT;
-                const int yy = yt + 5, zz = zt + 5;   // + 8 - 3: 0..23
-                const int bxo = k == 0 ? 0 : (k == 3 ? 2 : 1);
-                const int nb = (zz >> 3) * 9 + (yy >> 3) * 3 + bxo;
-                const int xl = (k == 0 || k == 2) ? 4 : 0;
-                const uint32_t g = sbase[nb] + (uint32_t)(xl + 8 * ((yy & 7) + 8 * (zz & 7)));
-                load_input4<KIND, OP, T>(a, g, beta, val[i], pnv[i]);
+    for (int i = 0; i < kUnitIters; i++) {
+        const int u = t + kS3Threads * i;
+        ra[i] = rb[i] = rv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (u < kUnits) {
+            const int k = u & 3, line = u >> 2;
+            const int yt = line % kT, zt = line / kT;
+            const int yy = yt + 5, zz = zt + 5;   // + 8 - 3: 0..23
+            const int bxo = k == 0 ? 0 : (k == 3 ? 2 : 1);
+            const int nb = (zz >> 3) * 9 + (yy >> 3) * 3 + bxo;
+            const int xl = (k == 0 || k == 2) ? 4 : 0;
+            const uint32_t g = sbase[nb] + (uint32_t)(xl + 8 * ((yy & 7) + 8 * (zz & 7)));
+            if constexpr (KIND == K_DIAG) {
+                rv[i] = *(const float4*)(a.vvol + g);
+            } else {
+                ra[i] = *(const float4*)(a.in + g);
+                if constexpr (KIND == K_SPMV_FUSED) rb[i] = *(const float4*)(a.in2 + g);
+                if constexpr (OP == VPS_OP_ROW) rv[i] = *(const float4*)(a.vvol + g);
             }
         }
+    }
+    float beta = 0.f;
+    if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_FUSED) {
+        if (!spmv_prologue(a, &beta)) return;   // uniform over the grid
+    }
 #pragma unroll
-        for (int i = 0; i < kUnitIters; i++) {
-            const int u = t + kS3Threads * i;
-            if (u < kUnits) {
-                const int k = u & 3, line = u >> 2;
-                const int yt = line % kT, zt = line / kT;
-                T* dst = &tile[zt * kPS + yt * kRS + 4 + 4 * k];
+    for (int i = 0; i < kUnitIters; i++) {
+        const int u = t + kS3Threads * i;
+        if (u < kUnits) {
+            const int k = u & 3, line = u >> 2;
+            const int yt = line % kT, zt = line / kT;
+            T v[4];
+            float pn[4];
+            if constexpr (KIND == K_DIAG) {
+                v[0] = (T)rv[i].x; v[1] = (T)rv[i].y; v[2] = (T)rv[i].z; v[3] = (T)rv[i].w;
+                pn[0] = pn[1] = pn[2] = pn[3] = 0.f;
+            } else {
+                float4 p = ra[i];
+                if constexpr (KIND == K_SPMV_FUSED)
+                    p = make_float4(rb[i].x + beta * ra[i].x, rb[i].y + beta * ra[i].y,
+                                    rb[i].z + beta * ra[i].z, rb[i].w + beta * ra[i].w);
+                pn[0] = p.x; pn[1] = p.y; pn[2] = p.z; pn[3] = p.w;
+                if constexpr (OP == VPS_OP_ROW) {
+                    v[0] = (T)rv[i].x * (T)p.x; v[1] = (T)rv[i].y * (T)p.y;
+                    v[2] = (T)rv[i].z * (T)p.z; v[3] = (T)rv[i].w * (T)p.w;
+                } else {
+                    v[0] = (T)p.x; v[1] = (T)p.y; v[2] = (T)p.z; v[3] = (T)p.w;
+                }
+            }
+            T* dst = &tile[zt * kPS + yt * kRS + 4 + 4 * k];
 #pragma unroll
-                for (int c = 0; c < 4; c++) dst[c] = val[i][c];
-                if constexpr (KIND != K_DIAG) {
-                    if ((k == 1 || k == 2) && yt >= 3 && yt < 11 && zt >= 3 && zt < 11) {
-                        const int loc = 4 * (k - 1) + 8 * ((yt - 3) + 8 * (zt - 3));
+            for (int c = 0; c < 4; c++) dst[c] = v[c];
+            if constexpr (KIND != K_DIAG) {
+                if ((k == 1 || k == 2) && yt >= 3 && yt < 11 && zt >= 3 && zt < 11) {
+                    const int loc = 4 * (k - 1) + 8 * ((yt - 3) + 8 * (zt - 3));
 #pragma unroll
-                        for (int c = 0; c < 4; c++) own[loc + c] = pnv[i][c];
-                        if constexpr (KIND == K_SPMV_FUSED)
-                            *(float4*)(a.out2 + base + loc) =
-                                make_float4(pnv[i][0], pnv[i][1], pnv[i][2], pnv[i][3]);
-                    }
+                    for (int c = 0; c < 4; c++) own[loc + c] = pn[c];
+                    if constexpr (KIND == K_SPMV_FUSED)
+                        *(float4*)(a.out2 + base + loc) = make_float4(pn[0], pn[1], pn[2], pn[3]);
                 }
             }
         }
@@ -534,17 +521,29 @@ __global__ void __launch_bounds__(kUpdThreads) k_cg_update(
     CgState* st, const double* pt_part, int npt, const double* rr_prev,
     int kk, double* rr_out) {
     __shared__ double sbuf[kUpdThreads / 64];
-    if (st->done) return;
+    __shared__ int s_done;
+    // every load that does not depend on alpha is issued first: the state
+    // flag (read once per block, so the exit below is uniform), both
+    // partial sums and this thread's vector elements
+    if (threadIdx.x == 0) s_done = st->done;
+    const double abs0 = st->abs0;
     double pt = 0, rp = 0;
     for (int i = threadIdx.x; i < npt; i += kUpdThreads) pt += pt_part[i];
-    pt = block_sum<kUpdThreads>(pt, sbuf);
-    if (kk == 0) {
-        rp = st->abs0;
-    } else {
-        double s = 0;
-        for (int i = threadIdx.x; i < kUpdBlocks; i += kUpdThreads) s += rr_prev[i];
-        rp = block_sum<kUpdThreads>(s, sbuf);
+    if (kk != 0)
+        for (int i = threadIdx.x; i < kUpdBlocks; i += kUpdThreads) rp += rr_prev[i];
+    const int64_t i0 = (int64_t)blockIdx.x * kUpdThreads + threadIdx.x;
+    const int64_t stride = (int64_t)kUpdBlocks * kUpdThreads;
+    float4 xv0{}, rv0{}, pv0{}, tv0{};
+    if (i0 < n4) {
+        xv0 = x[i0];
+        rv0 = r[i0];
+        pv0 = p[i0];
+        tv0 = tt[i0];
     }
+    __syncthreads();
+    if (s_done) return;
+    pt = block_sum<kUpdThreads>(pt, sbuf);
+    rp = kk == 0 ? abs0 : block_sum<kUpdThreads>(rp, sbuf);
     const double ald = rp / pt;
     if (!isfinite(ald)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 3;
@@ -552,10 +551,13 @@ __global__ void __launch_bounds__(kUpdThreads) k_cg_update(
     }
     const float al = (float)ald;
     double rr = 0;
-    for (int64_t i = (int64_t)blockIdx.x * kUpdThreads + threadIdx.x; i < n4;
-         i += (int64_t)kUpdBlocks * kUpdThreads) {
-        float4 xv = x[i], rv = r[i];
-        const float4 pv = p[i], tv = tt[i];
+    for (int64_t i = i0; i < n4; i += stride) {
+        float4 xv, rv, pv, tv;
+        if (i == i0) {
+            xv = xv0; rv = rv0; pv = pv0; tv = tv0;
+        } else {
+            xv = x[i]; rv = r[i]; pv = p[i]; tv = tt[i];
+        }
         xv.x += al * pv.x;
         xv.y += al * pv.y;
         xv.z += al * pv.z;
@@ -729,6 +731,7 @@ static StencilArgs base_args(vps_context* c) {
     a.k = c->k;
     a.slot_of_brick = c->brick_slot[c->which].p;
     a.brick_of_slot = c->slot_brick[c->which].p;
+    a.nbr_slot = c->nbr_slot.p;
     a.nslots = c->nslots;
     a.zero_slot = c->nbricks;
     a.vvol = c->vvol.p;

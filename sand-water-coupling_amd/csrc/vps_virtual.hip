@@ -264,20 +264,20 @@ __global__ void __launch_bounds__(NB) k_virtual_sums(
                    vb, bits);
 }
 
-// 3D with the real neighbourhood of half a brick (8 x 8 x 4 nodes, one
-// workgroup of 256 threads) staged in LDS.  The cells within one cell of
-// those nodes form <= kStageRows x-rows of <= kStageCx cells; their particles
-// (contiguous per row in the cell-sorted order) are copied once -- the FP32
-// offset from their own cell corner, 1/rho, v and V -- and every node walks
-// its 3 x 3 x 3 cells from LDS in the order of the global walk.
+// 3D with the brick's real neighbourhood staged in LDS (one workgroup of
+// 512 threads per brick, thread = node).  The cells within one cell of the
+// brick's nodes form <= kStageRows x-rows of <= kStageCx cells; their
+// particles (contiguous per row in the cell-sorted order) are copied once,
+// each as its FP32 offset from its own cell corner with 1/rho_j, and every
+// node walks its 3 x 3 x 3 cells from LDS in the order of the global walk.
 // q - x_j = (q - C_q) - (x_j - C_j) + (C_q - C_j) with C the FP64 cell
 // corners, so the FP32 difference is accurate to a few ulp of h.
 // Membership is pre-tested in FP32 with a band around h^2 in which the
-// exact FP64 test on the original positions decides (also for particles
-// clamped into a border cell), so the neighbour sets stay exact.  The
-// boundary cell table of the same rows is staged too, so nodes away from
-// the walls skip the boundary walk without touching global memory.
-constexpr int kStageCap = 2304;
+// exact FP64 test on the original positions decides (also at r = 0 and for
+// particles clamped into a border cell), so the neighbour sets stay exact.
+// The boundary cell table of the same rows is staged too, so nodes away
+// from the walls skip the boundary walk without touching global memory.
+constexpr int kStageCap = 4096;
 constexpr int kStageRows = 64;
 constexpr int kStageCx = 8;
 
@@ -306,36 +306,32 @@ struct BoundaryWalkStaged {
     }
 };
 
-__global__ void __launch_bounds__(256) k_virtual_sums3s(
+__global__ void __launch_bounds__(512) k_virtual_sums3s(
     KConst k, const int32_t* brick_of_slot, int64_t nslots,
     const double4* rpos, const float4* rvel, const float* rrho,
     const int32_t* rcstart, const double4* bpos, const float4* bvel,
     const float4* bnrm, const int32_t* bcstart, float* vrho, float* vvol,
     float* vb, uint32_t* bits) {
     __shared__ float4 s_p[kStageCap];   // (x_j - C_j, +-1/rho_j; < 0: exact only)
-    __shared__ float4 s_v[kStageCap];   // (v_j, V_j)
     __shared__ int s_cs[kStageRows][kStageCx + 1];  // real cell starts (staged)
     __shared__ int s_bcs[kStageRows][kStageCx + 1]; // boundary cell starts
     __shared__ int s_rg[kStageRows];    // global - staged index
     __shared__ int s_off[kStageRows + 1];
     __shared__ int s_meta[8];   // clo x,y,z; ncx, ncy, ncz; total; ok
-    const int blk = xcd_remap(blockIdx.x, (int)(2 * nslots));
-    const int slot = blk >> 1, half = blk & 1;
+    const int slot = xcd_remap(blockIdx.x, (int)nslots);
     const int brick = brick_of_slot[slot];
     const int t = threadIdx.x;
-    const int lx = t & 7, ly = (t >> 3) & 7, lz = 4 * half + (t >> 6);
+    const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
     const int bx = brick % k.bdims[0];
     const int by = (brick / k.bdims[0]) % k.bdims[1];
     const int bz = brick / k.bdims[0] / k.bdims[1];
-    const int n0[3] = {bx * kBrick, by * kBrick, bz * kBrick + 4 * half};
-    const int span[3] = {kBrick, kBrick, 4};
+    const int n0[3] = {bx * kBrick, by * kBrick, bz * kBrick};
     if (t == 0) {
         int ok = 1, nrows = 1;
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            const int first = min(n0[a], k.ndims[a] - 1);
-            const int last = min(n0[a] + span[a] - 1, k.ndims[a] - 1);
-            const int clo = max(cell_coord(node_coord(k.origin[a], first, k.spacing),
+            const int last = min(n0[a] + kBrick - 1, k.ndims[a] - 1);
+            const int clo = max(cell_coord(node_coord(k.origin[a], n0[a], k.spacing),
                                            k.cell_lo[a], k.cell_inv, k.cdims[a]) - 1, 0);
             const int chi = min(cell_coord(node_coord(k.origin[a], last, k.spacing),
                                            k.cell_lo[a], k.cell_inv, k.cdims[a]) + 1,
@@ -380,14 +376,13 @@ __global__ void __launch_bounds__(256) k_virtual_sums3s(
         __syncthreads();
         const int total = s_meta[6];
         const float far = 2.f * (float)(1.0 / k.cell_inv);
-        for (int e0 = 0; e0 < total; e0 += 4 * 256) {
+        for (int e0 = 0; e0 < total; e0 += 4 * 512) {
             double4 pj[4];
-            float4 vj[4];
             float rj[4];
             int er[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const int e = e0 + u * 256 + t;
+                const int e = e0 + u * 512 + t;
                 er[u] = -1;
                 if (e < total) {
                     // row of staged element e: last r with s_off[r] <= e
@@ -399,13 +394,12 @@ __global__ void __launch_bounds__(256) k_virtual_sums3s(
                     }
                     er[u] = e + s_rg[lo];
                     pj[u] = rpos[er[u]];
-                    vj[u] = rvel[er[u]];
                     rj[u] = rrho[er[u]];
                 }
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const int e = e0 + u * 256 + t;
+                const int e = e0 + u * 512 + t;
                 if (er[u] < 0) continue;
                 const double pc[3] = {pj[u].x, pj[u].y, pj[u].z};
                 float of[3];
@@ -418,13 +412,12 @@ __global__ void __launch_bounds__(256) k_virtual_sums3s(
                 }
                 const float ir = 1.f / rj[u];
                 s_p[e] = make_float4(of[0], of[1], of[2], is_far ? -ir : ir);
-                s_v[e] = vj[u];
             }
         }
         __syncthreads();
     }
-    const int cx = n0[0] + lx, cy = n0[1] + ly, cz = bz * kBrick + lz;
-    const int64_t out = (int64_t)slot * 512 + lx + 8 * ly + 64 * lz;
+    const int cx = n0[0] + lx, cy = n0[1] + ly, cz = n0[2] + lz;
+    const int64_t out = (int64_t)slot * 512 + t;
     if (cx >= k.ndims[0] || cy >= k.ndims[1] || cz >= k.ndims[2]) {
         vrho[out] = 0.f;
         vvol[out] = 0.f;
@@ -472,16 +465,17 @@ __global__ void __launch_bounds__(256) k_virtual_sums3s(
                     const float dx = tx - pr.x, dy = ty - pr.y, dz = tz - pr.z;
                     const float d2f = dx * dx + dy * dy + dz * dz;
                     if (d2f >= k.h2f_hi && pr.w >= 0.f) continue;
+                    const int g = j + rg;
                     if (d2f < k.h2f_lo && pr.w >= 0.f && d2f > 0.f) {
-                        a.add_fast<3>(k, d2f, dx, dy, dz, s_v[j], pr.w);
+                        a.add_fast<3>(k, d2f, dx, dy, dz, rvel[g], pr.w);
                     } else {
                         // near h, at r = 0, or far from its cell: the
                         // reference's FP64 test and the accurate spline
-                        const double4 pj = rpos[j + rg];
+                        const double4 pj = rpos[g];
                         const double d2 = sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz);
                         if (!(d2 < k.h2)) continue;
                         a.add<3>(k, d2, (float)(qx - pj.x), (float)(qy - pj.y),
-                                 (float)(qz - pj.z), s_v[j], fabsf(pr.w));
+                                 (float)(qz - pj.z), rvel[g], fabsf(pr.w));
                     }
                 }
             }
@@ -490,6 +484,33 @@ __global__ void __launch_bounds__(256) k_virtual_sums3s(
     finish_node<3>(k, a, qx, qy, qz, cx, cy, cz, out, rpos, rcstart,
                    BoundaryWalkStaged{s_bcs, s_meta, bpos}, bvel, bnrm, vrho,
                    vvol, vb, bits);
+}
+
+// Slots of the 3^D neighbour bricks of every slot (zero brick if absent).
+template <int D>
+__global__ void __launch_bounds__(256) k_nbr_slots(KConst k,
+                                                   const int32_t* brick_of_slot,
+                                                   const int32_t* slot_of_brick,
+                                                   int64_t nslots, int32_t zero,
+                                                   int32_t* nbr) {
+    constexpr int NN = D == 3 ? 27 : 9;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nslots * NN) return;
+    const int64_t slot = i / NN;
+    const int t = (int)(i - slot * NN);
+    const int brick = brick_of_slot[slot];
+    const int bx = brick % k.bdims[0];
+    const int by = (brick / k.bdims[0]) % k.bdims[1];
+    const int bz = D == 3 ? brick / k.bdims[0] / k.bdims[1] : 0;
+    const int nx = bx + (t % 3) - 1, ny = by + (t / 3) % 3 - 1,
+              nz = D == 3 ? bz + t / 9 - 1 : 0;
+    int s = zero;
+    if (nx >= 0 && ny >= 0 && nz >= 0 && nx < k.bdims[0] && ny < k.bdims[1] &&
+        nz < k.bdims[2]) {
+        const int sl = slot_of_brick[((int64_t)nz * k.bdims[1] + ny) * k.bdims[0] + nx];
+        if (sl >= 0) s = sl;
+    }
+    nbr[slot * 27 + t] = s;
 }
 
 __global__ void __launch_bounds__(256) k_popc(const uint32_t* bits,
@@ -601,8 +622,8 @@ int virtual_generate(vps_context* c) {
     }
     const int rs = c->cur;
     if (D == 3)
-        hipLaunchKernelGGL(k_virtual_sums3s, dim3((unsigned)(2 * nslots)),
-                           dim3(256), 0, c->stream, c->k, c->slot_brick[w].p,
+        hipLaunchKernelGGL(k_virtual_sums3s, dim3((unsigned)nslots),
+                           dim3(512), 0, c->stream, c->k, c->slot_brick[w].p,
                            nslots, c->rpos[rs].p, c->rvel[rs].p, c->rrho.p,
                            c->cell_start.p, c->bpos.p, c->bvel.p, c->bnrm.p,
                            c->bcell_start.p, c->vrho.p, c->vvol.p, c->vb.p,
@@ -614,6 +635,14 @@ int virtual_generate(vps_context* c) {
                            c->cell_start.p, c->bpos.p, c->bvel.p, c->bnrm.p,
                            c->bcell_start.p, c->vrho.p, c->vvol.p, c->vb.p,
                            c->bits.p);
+    if (D == 3)
+        hipLaunchKernelGGL(k_nbr_slots<3>, dim3(blocks(nslots * 27, 256)), dim3(256),
+                           0, c->stream, c->k, c->slot_brick[w].p, c->brick_slot[w].p,
+                           nslots, (int32_t)c->nbricks, c->nbr_slot.p);
+    else
+        hipLaunchKernelGGL(k_nbr_slots<2>, dim3(blocks(nslots * 9, 256)), dim3(256),
+                           0, c->stream, c->k, c->slot_brick[w].p, c->brick_slot[w].p,
+                           nslots, (int32_t)c->nbricks, c->nbr_slot.p);
     hipLaunchKernelGGL(k_popc, dim3(blocks(c->nwords, 256)), dim3(256), 0,
                        c->stream, c->bits.p, c->nwords,
                        (int32_t*)c->word_prefix.p);
