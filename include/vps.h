@@ -141,7 +141,9 @@ typedef struct vps_stats {
     double ms_spmv_avg;        /* average duration of one CG SpMV launch */
     double ms_update_avg;      /* average duration of one CG update launch */
     int64_t spmv_launches;
-    int64_t reserved[4];
+    int64_t reserved[4];       /* [0] residual replacements of the last solve,
+                                  [1] host microseconds spent launching its CG
+                                  kernels */
 } vps_stats;
 
 typedef struct vps_context vps_context;

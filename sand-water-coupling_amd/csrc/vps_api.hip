@@ -121,6 +121,7 @@ void fill_kconst(vps_context* c) {
     k.spacing = p.lattice_spacing;
     const double cs = k.h * (1 + 1e-6);
     k.cell_inv = 1 / cs;
+    k.cell_size = cs;
     for (int a = 0; a < 3; a++) {
         if (a < p.dim) {
             k.cell_lo[a] = p.domain_lo[a];
@@ -167,6 +168,7 @@ void free_all(vps_context* c) {
         dfree(c->p_by_rank[b]);
     }
     dfree(c->rrho);
+    dfree(c->rofs);
     dfree(c->cell_of);
     dfree(c->slot_of);
     dfree(c->perm);
@@ -203,6 +205,7 @@ int ensure_particle_capacity(vps_context* c, int64_t n) {
         VPS_TRY(dalloc(c, c->rid[b], (size_t)n));
     }
     VPS_TRY(dalloc(c, c->rrho, (size_t)n));
+    VPS_TRY(dalloc(c, c->rofs, (size_t)n));
     const int64_t m = std::max(n, c->nb);
     VPS_TRY(dalloc(c, c->cell_of, (size_t)m));
     VPS_TRY(dalloc(c, c->slot_of, (size_t)m));

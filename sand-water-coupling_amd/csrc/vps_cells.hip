@@ -127,6 +127,27 @@ __global__ void __launch_bounds__(256) k_density(const double4* pos,
     if (set_volume) vel[i].w = k.mass / r;
 }
 
+// FP32 offset of every (sorted) particle from its cell corner; w = +1, or
+// -1 when the particle is clamped into a border cell (|offset| > 2 cells).
+__global__ void __launch_bounds__(256) k_cell_offsets3(const double4* pos,
+                                                       int64_t n, KConst k,
+                                                       float4* ofs) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double4 p = pos[i];
+    const double pc[3] = {p.x, p.y, p.z};
+    const float far = 2.f * (float)k.cell_size;
+    float of[3];
+    bool is_far = false;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int c = cell_coord(pc[a], k.cell_lo[a], k.cell_inv, k.cdims[a]);
+        of[a] = (float)(pc[a] - cell_corner(k, a, c));
+        is_far |= !(fabsf(of[a]) <= far);
+    }
+    ofs[i] = make_float4(of[0], of[1], of[2], is_far ? -1.f : 1.f);
+}
+
 // Boundary density from boundary-only neighbours (the boundary set has its
 // own searcher, DualParticleSphLiquid.cpp:96-97); V_b = m / rho_b.
 template <int D>
@@ -226,11 +247,13 @@ int cells_sort_boundary(vps_context* c) {
 int real_density(vps_context* c, bool set_volume) {
     const int64_t n = c->n;
     const int s = c->cur;
-    if (c->dim == 3)
+    if (c->dim == 3) {
+        hipLaunchKernelGGL(k_cell_offsets3, dim3(blocks(n, 256)), dim3(256), 0,
+                           c->stream, c->rpos[s].p, n, c->k, c->rofs.p);
         hipLaunchKernelGGL(k_density<3>, dim3(blocks(n, 256)), dim3(256), 0,
                            c->stream, c->rpos[s].p, c->cell_start.p, n, c->k,
                            c->rrho.p, c->rvel[s].p, set_volume ? 1 : 0);
-    else
+    } else
         hipLaunchKernelGGL(k_density<2>, dim3(blocks(n, 256)), dim3(256), 0,
                            c->stream, c->rpos[s].p, c->cell_start.p, n, c->k,
                            c->rrho.p, c->rvel[s].p, set_volume ? 1 : 0);

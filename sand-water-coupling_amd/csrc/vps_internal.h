@@ -53,6 +53,7 @@ struct KConst {
     // cell grid
     double cell_lo[3];
     double cell_inv;
+    double cell_size;                      // 1 / cell_inv (cell corners)
     int cdims[3];
     // physics
     float mass;
@@ -175,6 +176,13 @@ __device__ __forceinline__ int cell_coord(double p, double lo, double inv,
                                           int n) {
     int c = (int)floor((p - lo) * inv);
     return c < 0 ? 0 : (c >= n ? n - 1 : c);
+}
+
+// Corner C_c = lo + c * size of cell c along one axis, in FP64.  Any fixed
+// value per cell serves: staged gathers form q - x_j as
+// (q - C_q) - (x_j - C_j) + (C_q - C_j).
+__device__ __forceinline__ double cell_corner(const KConst& k, int a, int c) {
+    return k.cell_lo[a] + (double)c * k.cell_size;
 }
 
 // Walks the candidates of a radius-h query: the 3^(D-1) x-rows of 3 cells
@@ -314,6 +322,9 @@ struct vps_context {
     vps::DBuf<int32_t> rid[2];
     int cur = 0;
     vps::DBuf<float> rrho;
+    // sorted order: FP32 offset of each particle from its cell corner, w =
+    // +-1/rho (negative: clamped into a border cell, exact test only)
+    vps::DBuf<float4> rofs;
     vps::DBuf<int32_t> cell_of, slot_of, perm;
     int64_t ncells = 0;
     vps::DBuf<int32_t> cell_count, cell_start;

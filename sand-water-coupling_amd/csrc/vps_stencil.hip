@@ -44,6 +44,8 @@
 #include <cfloat>
 #include <cmath>
 
+#include <chrono>
+
 #include "vps_internal.h"
 
 namespace vps {
@@ -56,6 +58,14 @@ struct CoefD {
 };
 
 namespace {
+
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+// Streaming (nontemporal) 16-byte store.
+__device__ __forceinline__ void store_nt4(float* p, float a, float b, float c,
+                                          float d) {
+    f32x4_t v = {a, b, c, d};
+    __builtin_nontemporal_store(v, (f32x4_t*)p);
+}
 
 constexpr int kT = 14;             // halo tile side (8 + 2 * 3)
 constexpr int kT2 = kT * kT;
@@ -346,7 +356,7 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
 #pragma unroll
                     for (int c = 0; c < 4; c++) own[loc + c] = pn[c];
                     if constexpr (KIND == K_SPMV_FUSED)
-                        *(float4*)(a.out2 + base + loc) = make_float4(pn[0], pn[1], pn[2], pn[3]);
+                        store_nt4(a.out2 + base + loc, pn[0], pn[1], pn[2], pn[3]);
                 }
             }
         }
@@ -403,7 +413,8 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
                 red0 += (double)b * (double)b;
                 red1 += (double)r * (double)r;
             } else {
-                a.out[g] = mp;
+                // t = M p is read by the next kernel only: streaming store
+                __builtin_nontemporal_store(mp, &a.out[g]);
                 red0 += (double)pi * (double)mp;
             }
         }
@@ -566,8 +577,8 @@ __global__ void __launch_bounds__(kUpdThreads) k_cg_update(
         rv.y -= al * tv.y;
         rv.z -= al * tv.z;
         rv.w -= al * tv.w;
-        x[i] = xv;
-        r[i] = rv;
+        store_nt4((float*)&x[i], xv.x, xv.y, xv.z, xv.w);
+        store_nt4((float*)&r[i], rv.x, rv.y, rv.z, rv.w);
         rr += ((double)rv.x * rv.x + (double)rv.y * rv.y) +
               ((double)rv.z * rv.z + (double)rv.w * rv.w);
     }
@@ -588,6 +599,7 @@ __global__ void __launch_bounds__(1024) k_cg_start(const double* pbb,
     const double bb = block_sum<1024>(a, sbuf);
     const double rr = block_sum<1024>(b, sbuf);
     if (threadIdx.x != 0) return;
+    *st = CgState{};   // the whole state starts here (no host upload)
     st->bb = bb;
     st->rr = rr;
     st->abs0 = rr;
@@ -820,15 +832,6 @@ int apply_operator_f32(vps_context* c, const float* in, float* out,
     return VPS_OK;
 }
 
-// Host -> device copy of the CG scalars through the pinned staging buffer.
-static int upload_state(vps_context* c, const CgState& s) {
-    *c->host_cg = s;
-    VPS_HIP(c, hipMemcpyAsync(c->cg.p, c->host_cg, sizeof(CgState),
-                              hipMemcpyHostToDevice, c->stream));
-    VPS_HIP(c, hipStreamSynchronize(c->stream));
-    return VPS_OK;
-}
-
 static int read_state(vps_context* c, CgState* out) {
     VPS_HIP(c, hipMemcpyAsync(c->host_cg, c->cg.p, sizeof(CgState),
                               hipMemcpyDeviceToHost, c->stream));
@@ -867,8 +870,6 @@ int cg_solve(vps_context* c, vps_stats* stats) {
     float* p1 = c->vp[1].p;
     const int nslots = (int)c->nslots;
 
-    CgState init{};
-    VPS_TRY(upload_state(c, init));
     {
         StencilArgs a = base_args(c);
         a.in = x;
@@ -882,20 +883,19 @@ int cg_solve(vps_context* c, vps_stats* stats) {
                            c->cg.p);
         VPS_HIP(c, hipGetLastError());
     }
-    CgState hs;
-    VPS_TRY(read_state(c, &hs));
-    if (hs.done == 3) {
-        c->err = "non-finite right-hand side or initial residual";
-        return VPS_ERR_NONFINITE;
+    // The start state is read back together with the first batch: its
+    // kernels return at once when k_cg_start has already ended the solve
+    // (converged, zero or non-finite right-hand side).  Only a zero
+    // iteration limit needs the state before launching.
+    CgState hs{};
+    if (max_iter == 0) {
+        VPS_TRY(read_state(c, &hs));
+        if (hs.done == 0) {
+            c->err = "CG reached the iteration limit without converging";
+            return VPS_ERR_NO_CONVERGENCE;
+        }
     }
-    if (hs.done == 4) {
-        VPS_HIP(c, hipMemsetAsync(x, 0, (size_t)nn * 4, c->stream));
-        hs.done = 1;
-    }
-    if (hs.done == 0 && max_iter == 0) {
-        c->err = "CG reached the iteration limit without converging";
-        return VPS_ERR_NO_CONVERGENCE;
-    }
+    bool start_checked = max_iter == 0;
     const bool prof = c->kernel_timing;
     const int64_t max_timed = (int64_t)c->kev.size() / 4;
     int64_t timed = 0, spmv_launches = 0;
@@ -904,11 +904,13 @@ int cg_solve(vps_context* c, vps_stats* stats) {
     int replacements = 0;
     int predicted = c->prev_iters > 0 ? c->prev_iters + 1 : 8;
     std::vector<int> timed_iter;
+    double host_launch_us = 0;
     while (true) {
         if (hs.done == 0) {
             int batch = iter_base + kk + 2 < predicted
                             ? predicted - (iter_base + kk) : 4;
             if (batch > 64) batch = 64;
+            const auto h0 = std::chrono::steady_clock::now();
             for (int j = 0; j < batch; j++) {
                 StencilArgs s = base_args(c);
                 s.kk = kk;
@@ -954,7 +956,24 @@ int cg_solve(vps_context* c, vps_stats* stats) {
                 kk++;
             }
             VPS_HIP(c, hipGetLastError());
+            host_launch_us += std::chrono::duration<double, std::micro>(
+                                  std::chrono::steady_clock::now() - h0).count();
             VPS_TRY(read_state(c, &hs));
+            if (!start_checked) {
+                start_checked = true;
+                if (hs.done == 3) {
+                    c->err = "non-finite right-hand side or initial residual";
+                    return VPS_ERR_NONFINITE;
+                }
+                if (hs.done == 4) {   // b = 0: x = 0, zero iterations
+                    VPS_HIP(c, hipMemsetAsync(x, 0, (size_t)nn * 4, c->stream));
+                    hs.done = 1;
+                    hs.iter = 0;
+                    hs.rr = 0;
+                    hs.true_rr = 0;
+                    break;
+                }
+            }
             if (hs.done == 0) {
                 if (iter_base + kk > max_iter + 8) {
                     c->err = "CG loop did not terminate";
@@ -1004,14 +1023,16 @@ int cg_solve(vps_context* c, vps_stats* stats) {
     }
     if (stats) {
         stats->cg_iterations = hs.iter;
-        stats->cg_rel_residual = std::sqrt(hs.rr / hs.bb);
+        // b = 0 solves exactly with x = 0 (relative residual 0)
+        stats->cg_rel_residual = hs.bb > 0 ? std::sqrt(hs.rr / hs.bb) : 0.0;
         stats->cg_true_rel_residual =
-            hs.true_rr >= 0 ? std::sqrt(hs.true_rr / hs.bb) : -1.0;
+            hs.true_rr >= 0 ? (hs.bb > 0 ? std::sqrt(hs.true_rr / hs.bb) : 0.0) : -1.0;
         stats->rhs_norm = std::sqrt(hs.bb);
         stats->spmv_launches = spmv_launches;
         stats->ms_spmv_avg = counted ? spmv_ms / counted : 0;
         stats->ms_update_avg = counted ? upd_ms / counted : 0;
         stats->reserved[0] = replacements;
+        stats->reserved[1] = (int64_t)host_launch_us;   // host time spent launching
     }
     if (hs.done == 2) {
         c->err = "CG reached the iteration limit without converging";

@@ -281,10 +281,6 @@ constexpr int kStageCap = 4096;
 constexpr int kStageRows = 64;
 constexpr int kStageCx = 8;
 
-__device__ __forceinline__ double cell_corner(const KConst& k, int a, int c) {
-    return k.cell_lo[a] + (double)c / k.cell_inv;
-}
-
 struct BoundaryWalkStaged {
     const int (*bcs)[kStageCx + 1];   // global starts of the staged rows
     const int* meta;                  // clo x,y,z; ncx, ncy
@@ -309,9 +305,9 @@ struct BoundaryWalkStaged {
 __global__ void __launch_bounds__(512) k_virtual_sums3s(
     KConst k, const int32_t* brick_of_slot, int64_t nslots,
     const double4* rpos, const float4* rvel, const float* rrho,
-    const int32_t* rcstart, const double4* bpos, const float4* bvel,
-    const float4* bnrm, const int32_t* bcstart, float* vrho, float* vvol,
-    float* vb, uint32_t* bits) {
+    const float4* rofs, const int32_t* rcstart, const double4* bpos,
+    const float4* bvel, const float4* bnrm, const int32_t* bcstart,
+    float* vrho, float* vvol, float* vb, uint32_t* bits) {
     __shared__ float4 s_p[kStageCap];   // (x_j - C_j, +-1/rho_j; < 0: exact only)
     __shared__ int s_cs[kStageRows][kStageCx + 1];  // real cell starts (staged)
     __shared__ int s_bcs[kStageRows][kStageCx + 1]; // boundary cell starts
@@ -375,15 +371,12 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
         }
         __syncthreads();
         const int total = s_meta[6];
-        const float far = 2.f * (float)(1.0 / k.cell_inv);
         for (int e0 = 0; e0 < total; e0 += 4 * 512) {
-            double4 pj[4];
+            float4 pj[4];
             float rj[4];
-            int er[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int e = e0 + u * 512 + t;
-                er[u] = -1;
                 if (e < total) {
                     // row of staged element e: last r with s_off[r] <= e
                     int lo = 0, hi = nrows - 1;
@@ -392,26 +385,17 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
                         if (s_off[mid] <= e) lo = mid;
                         else hi = mid - 1;
                     }
-                    er[u] = e + s_rg[lo];
-                    pj[u] = rpos[er[u]];
-                    rj[u] = rrho[er[u]];
+                    pj[u] = rofs[e + s_rg[lo]];
+                    rj[u] = rrho[e + s_rg[lo]];
                 }
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int e = e0 + u * 512 + t;
-                if (er[u] < 0) continue;
-                const double pc[3] = {pj[u].x, pj[u].y, pj[u].z};
-                float of[3];
-                bool is_far = false;
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    const int c = cell_coord(pc[a], k.cell_lo[a], k.cell_inv, k.cdims[a]);
-                    of[a] = (float)(pc[a] - cell_corner(k, a, c));
-                    is_far |= !(fabsf(of[a]) <= far);
+                if (e < total) {
+                    const float ir = 1.f / rj[u];
+                    s_p[e] = make_float4(pj[u].x, pj[u].y, pj[u].z, pj[u].w < 0.f ? -ir : ir);
                 }
-                const float ir = 1.f / rj[u];
-                s_p[e] = make_float4(of[0], of[1], of[2], is_far ? -ir : ir);
             }
         }
         __syncthreads();
@@ -624,7 +608,7 @@ int virtual_generate(vps_context* c) {
     if (D == 3)
         hipLaunchKernelGGL(k_virtual_sums3s, dim3((unsigned)nslots),
                            dim3(512), 0, c->stream, c->k, c->slot_brick[w].p,
-                           nslots, c->rpos[rs].p, c->rvel[rs].p, c->rrho.p,
+                           nslots, c->rpos[rs].p, c->rvel[rs].p, c->rrho.p, c->rofs.p,
                            c->cell_start.p, c->bpos.p, c->bvel.p, c->bnrm.p,
                            c->bcell_start.p, c->vrho.p, c->vvol.p, c->vb.p,
                            c->bits.p);

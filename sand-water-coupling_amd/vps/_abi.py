@@ -94,6 +94,10 @@ class Stats(C.Structure):
             if name == "reserved":
                 continue
             out[name] = getattr(self, name)
+        # residual replacements of the last solve, host time spent launching
+        # its CG kernels (microseconds)
+        out["cg_replacements"] = self.reserved[0]
+        out["cg_host_launch_us"] = self.reserved[1]
         return out
 
 

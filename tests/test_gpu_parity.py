@@ -227,3 +227,22 @@ def test_error_paths():
     st = g.project(allow_fail=True)
     assert st["status"] == _abi.VPS_ERR_NO_CONVERGENCE
     g.close()
+
+
+def test_zero_right_hand_side():
+    """A fluid at rest below its rest density: D = 0, so CG returns x = 0
+    with zero iterations (Eigen's b = 0 rule) and the velocities stay 0."""
+    sc = scene("3d")
+    sc = dict(sc, v=np.zeros_like(sc["v"]))
+    o = O.Oracle(sc, **INIT)
+    o.initialize()
+    g = Projection(sc, **INIT)
+    g.initialize()
+    so = o.projection_step()
+    sg = g.projection_step()
+    assert so["cg_iterations"] == 0 and sg["cg_iterations"] == 0
+    assert np.all(g.virtual()["p"] == 0)
+    _, v = g.particles()
+    assert np.all(v == 0)
+    g.close()
+    o.close()
