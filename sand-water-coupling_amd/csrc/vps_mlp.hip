@@ -38,6 +38,7 @@
 //   per-wave gradient partials are reduced in a fixed order, then Adam.
 // MFMA lane layouts (16x16x16): A lane l holds A[l%16][4(l/16)+j], B lane l
 // holds B[4(l/16)+j][l%16], C lane l holds C[4(l/16)+i][l%16].
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -68,10 +69,9 @@ constexpr int kInferBlocks = 1024;
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// FP32 -> bf16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950).
 __device__ __forceinline__ short f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
-    return (short)(u >> 16);
+    return __builtin_bit_cast(short, (__bf16)f);
 }
 __device__ __forceinline__ float bf2f(short b) {
     return __uint_as_float(((uint32_t)(uint16_t)b) << 16);
@@ -265,6 +265,90 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const float* P, FeatSrc fs,
                             : 0.f;
             }
         }
+    }
+}
+
+// Raw per-lane inputs of one 16-node tile (loaded one tile ahead): lane
+// (r, q) holds the 4 feature sources of node r for q < 2, and every lane the
+// node's V and history depth (which decide whether it takes the network).
+struct RawTile {
+    float v[4];
+    float vol;
+    int h;
+    int64_t g;
+};
+
+__device__ __forceinline__ void load_raw(const FeatSrc& s, int64_t tile, int64_t nn,
+                                         RawTile& R) {
+    const int lane = threadIdx.x & 63, q = lane >> 4;
+    R.g = tile * 16 + (lane & 15);
+    R.v[0] = R.v[1] = R.v[2] = R.v[3] = 0.f;
+    R.vol = 0.f;
+    R.h = 0;
+    if (R.g >= nn) return;
+    R.vol = s.vvol[R.g];
+    R.h = s.hh[R.g];
+    if (q == 0) {
+        R.v[0] = s.p1[R.g];
+        R.v[1] = s.dp1[R.g];
+        R.v[2] = s.dp2[R.g];
+        R.v[3] = s.vb[R.g];
+    } else if (q == 1) {
+        R.v[0] = s.vdiag[R.g];
+        R.v[1] = s.vrow[R.g];
+    }
+}
+
+// Inference over the slot nodes with the next tile's loads in flight while
+// the current one runs through the network.
+__global__ void __launch_bounds__(256) k_mlp_infer_pipe(const float* P, FeatSrc fs,
+                                                        int64_t nn, float* out) {
+    // the block reads the parameters once (coalesced) into LDS; each wave
+    // builds its register tiles from there
+    __shared__ float sP[kNParams];
+    for (int i = threadIdx.x; i < kNParams; i += 256) sP[i] = P[i];
+    __syncthreads();
+    Wregs W;
+    load_weights<true>(sP, W);
+    const int lane = threadIdx.x & 63, q = lane >> 4;
+    const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t nwaves = (int64_t)gridDim.x * 4;
+    const int64_t ntiles = (nn + 15) / 16;
+    const float inv_ps = fs.scale[SC_INV_PS], inv_dps = fs.scale[SC_INV_DPS];
+    const float inv_ds = fs.scale[SC_INV_DS], tau = fs.scale[SC_TAU];
+    RawTile cur, nxt;
+    if (wave < ntiles) load_raw(fs, wave, nn, cur);
+    for (int64_t tile = wave; tile < ntiles; tile += nwaves) {
+        if (tile + nwaves < ntiles) load_raw(fs, tile + nwaves, nn, nxt);
+        const bool ok = cur.g < nn && cur.vol > 0.f && cur.h >= 1;
+        if (__any(ok)) {
+            float f[4];
+            if (q == 0) {
+                f[0] = cur.v[0] * inv_ps;
+                f[1] = cur.v[1] * inv_dps;
+                f[2] = cur.v[2] * inv_dps;
+                f[3] = cur.v[3] * inv_ds;
+            } else if (q == 1) {
+                f[0] = cur.v[0] * fs.inv_a0 - 1.f;
+                f[1] = cur.v[1] * fs.inv_a0 - 1.f;
+                f[2] = cur.vol * fs.inv_vs - 1.f;
+                f[3] = cur.h >= 3 ? 1.f : (cur.h == 2 ? 0.5f : 0.f);
+            } else {
+                f[0] = f[1] = f[2] = f[3] = 0.f;
+            }
+            bf16x4 xb;
+#pragma unroll
+            for (int j = 0; j < 4; j++) xb[j] = f2bf(ok ? f[j] : 0.f);
+            Fwd F;
+            forward<false>(W, nullptr, xb, F);
+            // lanes 0..15 hold node r's output and (q = 0) its raw history
+            if (lane < 16 && cur.g < nn)
+                out[cur.g] = ok ? cur.v[0] + prior(cur.h, cur.v[1], cur.v[2]) + tau * F.out
+                                : 0.f;
+        } else if (lane < 16 && cur.g < nn) {
+            out[cur.g] = 0.f;
+        }
+        cur = nxt;
     }
 }
 
@@ -493,20 +577,31 @@ __global__ void __launch_bounds__(256) k_mlp_train(
     }
 }
 
-// Sum of the per-wave partials (fixed order), optional Adam step.
+// Sum of the per-block partials (fixed order), optional Adam step.  Block =
+// 64 parameters x 4 row groups; each thread sums rows g, g + 4, ... of its
+// parameter, the 4 groups are added in order.
 __global__ void __launch_bounds__(256) k_mlp_reduce(
-    const float* grad_part, int nwaves, const double* loss_part, float* grad,
+    const float* grad_part, int nrows, const double* loss_part, float* grad,
     double* loss, float* P, float* m, float* v, int adam, float lr, float b1t,
     float b2t) {
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p == 0) {
+    __shared__ float s_g[4][64];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + tx;
+    if (blockIdx.x == 0 && threadIdx.x < 64) {
         double s = 0;
-        for (int w = 0; w < nwaves; w++) s += loss_part[w];
-        *loss = s;
+        for (int w = threadIdx.x; w < nrows; w += 64) s += loss_part[w];
+        s = wave_sum(s);
+        if (threadIdx.x == 0) *loss = s;
     }
-    if (p >= kNParams) return;
     float g = 0.f;
-    for (int w = 0; w < nwaves; w++) g += grad_part[(int64_t)w * kNParams + p];
+    if (p < kNParams) {
+#pragma unroll 8
+        for (int w = ty; w < nrows; w += 4) g += grad_part[(int64_t)w * kNParams + p];
+    }
+    s_g[ty][tx] = g;
+    __syncthreads();
+    if (ty != 0 || p >= kNParams) return;
+    g = (s_g[0][tx] + s_g[1][tx]) + (s_g[2][tx] + s_g[3][tx]);
     grad[p] = g;
     if (adam) {
         const float mm = 0.9f * m[p] + 0.1f * g;
@@ -794,8 +889,10 @@ int mlp_warm_start(vps_context* c, int mode, float* x0) {
                        m->hp1.p, m->hdp1.p, m->hdp2.p, m->hh.p);
     VPS_HIP(c, hipGetLastError());
     if (mode == VPS_WARM_LEARNED) {
-        hipLaunchKernelGGL(k_mlp_infer<false>, dim3(kInferBlocks), dim3(256), 0,
-                           c->stream, m->P.p, feat_src(c, m->step_scale), nullptr, nn, x0);
+        // about 8 tiles of 16 nodes per wave
+        const int64_t nblk = std::min<int64_t>(std::max<int64_t>((nn + 511) / 512, 1), 8192);
+        hipLaunchKernelGGL(k_mlp_infer_pipe, dim3((unsigned)nblk), dim3(256), 0,
+                           c->stream, m->P.p, feat_src(c, m->step_scale), nn, x0);
     } else if (mode == VPS_WARM_EXTRAPOLATE) {
         hipLaunchKernelGGL(k_extrapolate, dim3(blocks(nn, 256)), dim3(256), 0,
                            c->stream, feat_src(c, m->step_scale), nn, x0);
@@ -834,7 +931,7 @@ int mlp_after_solve(vps_context* c, const vps_stats* st) {
                                m->slot_of_rank.p, c->nvirt, nullptr, nullptr, ns,
                                tpw, m->seed ^ (uint64_t)m->adam_t * 0x100000001ull,
                                m->grad_part.p, m->loss_part.p);
-            hipLaunchKernelGGL(k_mlp_reduce, dim3(blocks(kNParams, 256)), dim3(256),
+            hipLaunchKernelGGL(k_mlp_reduce, dim3(blocks(kNParams, 64)), dim3(256),
                                0, c->stream, m->grad_part.p, kTrainBlocks,
                                m->loss_part.p, m->grad.p, m->loss.p, m->P.p, m->m.p,
                                m->v.p, 1, (float)m->lr, b1t, b2t);
@@ -916,7 +1013,7 @@ int vps_mlp_loss_and_grad(vps_context* c, int64_t n, const float* features,
     hipLaunchKernelGGL(k_mlp_train<true>, dim3(kTrainBlocks), dim3(256), 0,
                        c->stream, m->P.p, none, nullptr, nullptr, n, df.p, dt.p, n,
                        tpw, 0ull, m->grad_part.p, m->loss_part.p);
-    hipLaunchKernelGGL(k_mlp_reduce, dim3(blocks(kNParams, 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(blocks(kNParams, 64)), dim3(256), 0,
                        c->stream, m->grad_part.p, kTrainBlocks, m->loss_part.p,
                        m->grad.p, dl.p, m->P.p, m->m.p, m->v.p, 0, 0.f, 0.f, 0.f);
     VPS_HIP(c, hipGetLastError());
