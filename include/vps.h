@@ -168,7 +168,10 @@ const char* vps_last_error(const vps_context* ctx);
 /* ---- state ------------------------------------------------------------ */
 
 /* Boundary (solid wall) particles: positions, inward unit normals and
- * velocities, each n*dim doubles.  Replaces BoundaryParticles::addSurface
+ * velocities, each n*dim doubles.  A zero normal marks a grain (a moving
+ * sphere, e.g. a sand grain): its normal in the pressure-gradient boundary
+ * term is the unit vector from the grain to the fluid particle (the
+ * reference defines no grain normal).  Replaces BoundaryParticles::addSurface
  * and the boundary velocity field (DualParticleSphLiquid.cpp:79-84): the
  * caller samples the walls.  May be called again to move the walls (their
  * densities are recomputed at the next vps_initialize or
@@ -226,6 +229,33 @@ int vps_set_kernel_timing(vps_context* ctx, int32_t enable);
 
 /* Blocks until all work on the context's stream is done. */
 int vps_synchronize(vps_context* ctx);
+
+/* ---- sand grains (DEM) --------------------------------------------- */
+
+/* DEMParticle<Dim> material (Cores/Structures/DEMParticle.cpp:13-28):
+ * K_norm = young * r, K_tang = K_norm * poisson, Coulomb limit
+ * tan(friction_angle).  Reference defaults: 1e9, 0.3, 0.5 rad. */
+typedef struct vps_dem_params {
+    double young;
+    double poisson;
+    double friction_angle;
+    double reserved;
+} vps_dem_params;
+
+/* One DEMParticleSand<Dim>::advance (DEMParticleSand.cpp:89-152) on a context
+ * whose real particles are the grains (radius = vps_params.particle_radius)
+ * and whose boundary particles are the walls: x += v dt; v += dt * sum of
+ * grain contacts within 2r + wall normal / friction forces within 4r;
+ * v.y -= gravity * dt.  The lattice of such a context is unused. */
+int vps_dem_step(vps_context* grains, double dt, double gravity,
+                 const vps_dem_params* params);
+
+/* Sets the boundary of the water context to its walls (vps_set_boundary)
+ * followed by the current grains of a DEM context (zero normals, the grains'
+ * velocities), then recomputes the boundary densities and volumes: the
+ * grains act on the water as moving boundary particles.  Both contexts must
+ * be on the same device. */
+int vps_couple_grains(vps_context* water, vps_context* grains);
 
 /* ---- read-back (tests, output) --------------------------------------- */
 

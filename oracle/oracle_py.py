@@ -77,6 +77,9 @@ def lib():
     pr("or_project", C.c_int, [P, C.POINTER(OrStats)])
     pr("or_projection_step", C.c_int, [P, C.POINTER(OrStats)])
     pr("or_advect", C.c_int, [P, C.c_double, C.c_double])
+    pr("or_refresh_boundary", C.c_int, [P])
+    pr("or_dem_step", C.c_int, [P, C.c_double, C.c_double, C.c_double,
+                                C.c_double, C.c_double])
     pr("or_virtual_count", C.c_int64, [P])
     pr("or_get_virtual", C.c_int, [P, I64, D, D, D, D, D, D])
     pr("or_set_pressure", C.c_int, [P, D])
@@ -180,6 +183,28 @@ class Oracle:
 
     def advect(self, dt, gravity):
         _check(self.L.or_advect(self.h, dt, gravity), "or_advect")
+
+    def set_boundary(self, bx, bn, bv):
+        bx = np.ascontiguousarray(bx, dtype=np.float64)
+        bn = np.ascontiguousarray(bn, dtype=np.float64)
+        bv = np.ascontiguousarray(bv, dtype=np.float64)
+        _check(self.L.or_set_boundary(self.h, len(bx), _d(bx), _d(bn), _d(bv)),
+               "or_set_boundary")
+        _check(self.L.or_refresh_boundary(self.h), "or_refresh_boundary")
+        self.nb = len(bx)
+
+    def set_particles(self, x, v):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        _check(self.L.or_set_particles(self.h, len(x), _d(x), _d(v)),
+               "or_set_particles")
+        self.n = len(x)
+
+    def dem_step(self, dt, gravity=9.8, young=1e9, poisson=0.3,
+                 friction_angle=0.5):
+        """DEMParticleSand::advance on the particles as grains."""
+        _check(self.L.or_dem_step(self.h, dt, gravity, young, poisson,
+                                  friction_angle), "or_dem_step")
 
     def particles(self):
         x = np.zeros((self.n, self.dim))

@@ -505,6 +505,15 @@ int or_set_boundary(or_sim* s, int64_t n, const double* x, const double* nrm,
     return 0;
 }
 
+/* BoundaryParticles::computeInfo after the boundary set changed
+ * (DualParticleSphLiquid.cpp:88-89 runs it once for static walls) */
+int or_refresh_boundary(or_sim* s) {
+    set_reset(&s->bnd);
+    set_compute_densities(&s->bnd);
+    set_compute_volumes(&s->bnd);
+    return 0;
+}
+
 int or_set_particles(or_sim* s, int64_t n, const double* x, const double* v) {
     if (n <= 0 || !x) return VPS_ERR_INVALID_ARG;
     if (set_resize(&s->real, n)) return VPS_ERR_OOM;
@@ -873,11 +882,20 @@ static void cb_grad_boundary(void* c, int32_t b, const double* pb) {
     double d[3] = {t->q[0] - pb[0], t->q[1] - pb[1], t->q[2] - pb[2]};
     double dv[3] = {t->vi[0] - s->bv[3 * b], t->vi[1] - s->bv[3 * b + 1],
                     t->vi[2] - s->bv[3 * b + 2]};
-    const double dn = dot3(s->dim, s->bn + 3 * b, dv);
+    const double* nb = s->bn + 3 * b;
+    double ng[3];
+    if (nb[0] == 0. && nb[1] == 0. && nb[2] == 0.) {
+        /* a grain (zero normal): the unit vector from the grain to the
+         * particle; the reference has no grain normal */
+        const double r = sqrt(dot3(s->dim, d, d));
+        for (int a = 0; a < 3; a++) ng[a] = r > 0. ? d[a] / r : 0.;
+        nb = ng;
+    }
+    const double dn = dot3(s->dim, nb, dv);
     const double m = 0. < dn ? 0. : dn; /* std::min(dn, 0.) */
     const double sc = s->bnd.vol[b] * m;
     const double w = kern_Wv(&s->bnd.k, d);
-    for (int a = 0; a < s->dim; a++) t->f[a] -= sc * s->bn[3 * b + a] * w;
+    for (int a = 0; a < s->dim; a++) t->f[a] -= sc * nb[a] * w;
 }
 
 /* applyPressureGradient (VirtualParticle.cpp:110-134) */
@@ -1046,6 +1064,140 @@ int or_advect(or_sim* s, double dt, double gravity) {
         for (int a = 0; a < s->dim; a++) s->real.x[3 * i + a] += s->v[3 * i + a] * dt;
         s->v[3 * i + 1] -= gravity * dt;
     }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * DEM sand step.  DEMParticleSand::advance (DEMParticleSand.cpp:90-94):
+ *   moveParticles (:98-109)       x += v dt, rebuild the grains' searcher
+ *   applyPressureForce (:118-152) v_i += dt * getForceSum(i) + wall terms
+ *   applyExternalForces (:111-115) v_i.y -= g dt
+ * The grains' searcher has kernel radius 2r (kernel_rate 2,
+ * DEMParticle.cpp:17); the walls keep 4r.  Contact force ComputeDemForces
+ * (DEMParticle.cpp:69-96) with K_norm = Young r, K_tang = K_norm Poisson
+ * (:45-55) and tan of the friction angle (:58-60).  The capillary force
+ * (:98-124) needs 1e-6 < |d| - 2r, which no neighbour within 2r has, so it
+ * is zero on every pair the searcher yields and is left out.  The reference
+ * adds each force to velocities[i] while other threads read velocities[j]
+ * (a data race); here every force reads the velocities of the start of the
+ * pass.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int dim;
+    const double* q;
+    const double* vi;
+    const double* v;      /* start-of-pass velocities, 3 per grain */
+    double two_r, min_dist, k_norm, k_tang, tan_phi;
+    double f[3];
+} dem_ctx;
+
+static void cb_dem_contact(void* c, int32_t j, const double* xj) {
+    dem_ctx* t = (dem_ctx*)c;
+    double d[3], w[3];
+    for (int a = 0; a < 3; a++) {
+        d[a] = xj[a] - t->q[a];            /* positions[j] - positions[i] */
+        w[a] = t->v[3 * j + a] - t->vi[a]; /* velocities[j] - velocities[i] */
+    }
+    double dist = sqrt(sqnorm(t->dim, d));
+    const double pen = t->two_r - dist;
+    if (!(pen > 0.)) return;
+    if (dist <= t->min_dist) dist = t->min_dist;
+    double n[3], vt[3], fn[3], fs[3];
+    for (int a = 0; a < 3; a++) n[a] = d[a] * (1 / dist);
+    const double vn = dot3(t->dim, w, n);
+    for (int a = 0; a < 3; a++) {
+        vt[a] = w[a] - vn * n[a];
+        fn[a] = t->k_norm * pen * n[a];
+        fs[a] = -t->k_tang * vt[a];
+    }
+    const double max_fs = sqrt(sqnorm(t->dim, fn)) * t->tan_phi;
+    const double fs_norm = sqrt(sqnorm(t->dim, fs));
+    if (fs_norm > max_fs)
+        for (int a = 0; a < 3; a++) fs[a] = fs[a] * max_fs / fs_norm;
+    for (int a = 0; a < t->dim; a++) t->f[a] += -fn[a] - fs[a];
+}
+
+typedef struct {
+    const or_sim* s;
+    const double* q;
+    const double* vi;
+    double k_tang, tan_phi;
+    double f[3];
+} dem_wall_ctx;
+
+/* the boundary lambda of DEMParticleSand::applyPressureForce (:138-149) */
+static void cb_dem_wall(void* c, int32_t js, const double* xj) {
+    dem_wall_ctx* t = (dem_wall_ctx*)c;
+    const int dim = t->s->dim;
+    const double* nb = t->s->bn + 3 * js;
+    double dv[3], d[3], Fn[3], Ft[3];
+    for (int a = 0; a < 3; a++) {
+        dv[a] = t->vi[a] - t->s->bv[3 * js + a];
+        d[a] = t->q[a] - xj[a];
+    }
+    const double un = dot3(dim, nb, dv);
+    const double w = kern_Wv(&t->s->bnd.k, d);
+    const double m = un < 0. ? un : 0.;
+    for (int a = 0; a < 3; a++) {
+        Fn[a] = -t->s->bnd.vol[js] * m * nb[a] * w;
+        Ft[a] = -t->k_tang * (dv[a] - un * nb[a]);
+    }
+    const double tn = sqrt(sqnorm(dim, Ft));
+    if (tn <= 0.000001) {
+        for (int a = 0; a < dim; a++) t->f[a] += Fn[a] + Ft[a];
+    } else {
+        const double lim = sqrt(sqnorm(dim, Fn)) * t->tan_phi;
+        for (int a = 0; a < dim; a++)
+            t->f[a] += Fn[a] + (lim > tn ? Ft[a] : (lim / tn) * Ft[a]);
+    }
+}
+
+int or_dem_step(or_sim* s, double dt, double gravity, double young,
+                double poisson, double friction_angle) {
+    const int dim = s->dim;
+    const int64_t n = s->real.n;
+    if (n == 0) return VPS_ERR_STATE;
+    int64_t i;
+    /* moveParticles */
+    for (i = 0; i < n; i++)
+        for (int a = 0; a < dim; a++) s->real.x[3 * i + a] += s->v[3 * i + a] * dt;
+    or_set grains = s->real; /* the same positions behind a 2r searcher */
+    memset(&grains.hash, 0, sizeof(grains.hash));
+    grains.k.kr = s->p.particle_radius * 2;
+    grains.k.kr2 = grains.k.kr * grains.k.kr;
+    set_reset(&grains);
+    /* wall volumes: computeInfo of the boundary set (DEMParticleSand.cpp:84-85) */
+    if (s->bnd.n > 0) {
+        set_reset(&s->bnd);
+        set_compute_densities(&s->bnd);
+        set_compute_volumes(&s->bnd);
+    }
+    double* v0 = (double*)malloc((size_t)n * 3 * sizeof(double));
+    if (!v0) {
+        hash_free(&grains.hash);
+        return VPS_ERR_OOM;
+    }
+    memcpy(v0, s->v, (size_t)n * 3 * sizeof(double));
+    const double r = s->p.particle_radius;
+    const double k_norm = young * r, k_tang = k_norm * poisson;
+    const double tan_phi = tan(friction_angle);
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < n; i++) {
+        const double* q = grains.x + 3 * i;
+        dem_ctx c = {dim, q, v0 + 3 * i, v0, 2 * r, 0.0001 * r, k_norm, k_tang,
+                     tan_phi, {0, 0, 0}};
+        set_for_each(&grains, q, cb_dem_contact, &c);
+        /* force = dt * getForceSum(i), then each wall term added to it */
+        dem_wall_ctx cw = {s, q, v0 + 3 * i, k_tang, tan_phi,
+                           {c.f[0] * dt, c.f[1] * dt, c.f[2] * dt}};
+        set_for_each(&s->bnd, q, cb_dem_wall, &cw);
+        for (int a = 0; a < dim; a++) s->v[3 * i + a] = v0[3 * i + a] + cw.f[a];
+        /* applyExternalForces */
+        s->v[3 * i + 1] -= gravity * dt;
+    }
+    free(v0);
+    hash_free(&grains.hash);
+    set_reset(&s->real);
     return 0;
 }
 

@@ -52,6 +52,12 @@ int or_generate(or_sim* s);
 int or_project(or_sim* s, or_stats* st);
 int or_projection_step(or_sim* s, or_stats* st);
 int or_advect(or_sim* s, double dt, double gravity);
+/* boundary densities and volumes of the current boundary set */
+int or_refresh_boundary(or_sim* s);
+/* one DEMParticleSand::advance of the grains held as real particles, the
+ * walls as boundary particles (velocities of the start of the pass) */
+int or_dem_step(or_sim* s, double dt, double gravity, double young,
+                double poisson, double friction_angle);
 
 int64_t or_virtual_count(or_sim* s);
 int or_get_virtual(or_sim* s, int64_t* node, double* rho, double* vol,

@@ -169,6 +169,8 @@ void free_all(vps_context* c) {
     }
     dfree(c->rrho);
     dfree(c->rofs);
+    dfree(c->dem_vnew);
+    dfree(c->couple_stage);
     dfree(c->cell_of);
     dfree(c->slot_of);
     dfree(c->perm);
@@ -178,6 +180,10 @@ void free_all(vps_context* c) {
     dfree(c->bvel);
     dfree(c->bnrm);
     dfree(c->bid);
+    dfree(c->bpos_spare);
+    dfree(c->bvel_spare);
+    dfree(c->bnrm_spare);
+    dfree(c->bid_spare);
     dfree(c->bcell_start);
     dfree(c->brick_flag);
     dfree(c->vrho);
@@ -230,6 +236,34 @@ void fill_common_stats(vps_context* c, vps_stats* st) {
 }
 
 }  // namespace
+
+namespace vps {
+// Boundary set from device FP64 AoS arrays (n x dim each): upload into the
+// cell-sorted layout, densities rho_b and volumes V_b = m / rho_b.
+int upload_boundary_device(vps_context* c, int64_t n, const double* dx,
+                           const double* dn, const double* dv) {
+    const int D = c->dim;
+    if (c->bpos.n < (size_t)n) {
+        dfree(c->bpos);
+        dfree(c->bvel);
+        dfree(c->bnrm);
+        dfree(c->bid);
+        VPS_TRY(dalloc(c, c->bpos, (size_t)n));
+        VPS_TRY(dalloc(c, c->bvel, (size_t)n));
+        VPS_TRY(dalloc(c, c->bnrm, (size_t)n));
+        VPS_TRY(dalloc(c, c->bid, (size_t)n));
+    }
+    if (!c->bcell_start.p) VPS_TRY(dalloc(c, c->bcell_start, (size_t)c->ncells + 1));
+    c->nb = n;
+    hipLaunchKernelGGL(k_upload_boundary, dim3(blocks(n, 256)), dim3(256), 0,
+                       c->stream, n, D, dx, dn, dv, c->bpos.p, c->bvel.p,
+                       c->bnrm.p, c->bid.p);
+    VPS_HIP(c, hipGetLastError());
+    VPS_TRY(cells_sort_boundary(c));
+    VPS_TRY(boundary_density(c));
+    return VPS_OK;
+}
+}  // namespace vps
 
 extern "C" {
 
@@ -388,6 +422,7 @@ int vps_set_boundary(vps_context* c, int64_t n, const double* x,
         return fail(c, VPS_ERR_NONFINITE, "boundary: non-finite input");
     c->nb = n;
     c->host_bx.assign(x, x + n * D);
+    c->couple_walls = -1;   // staged walls of vps_couple_grains are stale
     c->host_bn.assign(normals, normals + n * D);
     if (v) c->host_bv.assign(v, v + n * D);
     else c->host_bv.assign((size_t)(n * D), 0.0);
@@ -402,7 +437,7 @@ int vps_set_boundary(vps_context* c, int64_t n, const double* x,
 
 int vps_refresh_boundary(vps_context* c) {
     if (!c) return VPS_ERR_INVALID_ARG;
-    const int64_t n = c->nb;
+    const int64_t n = (int64_t)(c->host_bx.size() / c->dim);
     if (n == 0) return VPS_OK;
     const int D = c->dim;
     DBuf<double> dx, dn, dv;
@@ -415,18 +450,14 @@ int vps_refresh_boundary(vps_context* c) {
                               hipMemcpyHostToDevice, c->stream));
     VPS_HIP(c, hipMemcpyAsync(dv.p, c->host_bv.data(), (size_t)n * D * 8,
                               hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_upload_boundary, dim3(blocks(n, 256)), dim3(256), 0,
-                       c->stream, n, D, dx.p, dn.p, dv.p, c->bpos.p, c->bvel.p,
-                       c->bnrm.p, c->bid.p);
-    VPS_HIP(c, hipGetLastError());
-    VPS_TRY(cells_sort_boundary(c));
-    VPS_TRY(boundary_density(c));
+    VPS_TRY(upload_boundary_device(c, n, dx.p, dn.p, dv.p));
     VPS_HIP(c, hipStreamSynchronize(c->stream));
     dfree(dx);
     dfree(dn);
     dfree(dv);
     return VPS_OK;
 }
+
 
 int vps_set_particles(vps_context* c, int64_t n, const double* x,
                       const double* v) {

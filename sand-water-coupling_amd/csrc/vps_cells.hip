@@ -217,30 +217,23 @@ int cells_sort_real(vps_context* c) {
 }
 
 int cells_sort_boundary(vps_context* c) {
-    // sort in place through temporaries
+    // sort into the spare buffers, then swap them in (both stay allocated:
+    // a moving boundary is re-sorted every step)
     const int64_t nb = c->nb;
-    DBuf<double4> p2;
-    DBuf<float4> v2, n2;
-    DBuf<int32_t> i2;
-    VPS_TRY(dalloc(c, p2, nb));
-    VPS_TRY(dalloc(c, v2, nb));
-    VPS_TRY(dalloc(c, n2, nb));
-    VPS_TRY(dalloc(c, i2, nb));
+    VPS_TRY(dalloc(c, c->bpos_spare, nb));
+    VPS_TRY(dalloc(c, c->bvel_spare, nb));
+    VPS_TRY(dalloc(c, c->bnrm_spare, nb));
+    VPS_TRY(dalloc(c, c->bid_spare, nb));
     VPS_TRY(dalloc(c, c->cell_of, nb > c->n ? nb : c->n));
     VPS_TRY(dalloc(c, c->slot_of, nb > c->n ? nb : c->n));
     VPS_TRY(dalloc(c, c->perm, nb > c->n ? nb : c->n));
-    VPS_TRY((sort_set<float4, float4>(c, nb, c->bpos.p, p2.p, c->bvel.p, v2.p,
-                                      c->bnrm.p, n2.p, c->bid.p, i2.p,
-                                      c->bcell_start.p)));
-    std::swap(c->bpos, p2);
-    std::swap(c->bvel, v2);
-    std::swap(c->bnrm, n2);
-    std::swap(c->bid, i2);
-    VPS_HIP(c, hipStreamSynchronize(c->stream));
-    dfree(p2);
-    dfree(v2);
-    dfree(n2);
-    dfree(i2);
+    VPS_TRY((sort_set<float4, float4>(c, nb, c->bpos.p, c->bpos_spare.p, c->bvel.p,
+                                      c->bvel_spare.p, c->bnrm.p, c->bnrm_spare.p,
+                                      c->bid.p, c->bid_spare.p, c->bcell_start.p)));
+    std::swap(c->bpos, c->bpos_spare);
+    std::swap(c->bvel, c->bvel_spare);
+    std::swap(c->bnrm, c->bnrm_spare);
+    std::swap(c->bid, c->bid_spare);
     return VPS_OK;
 }
 

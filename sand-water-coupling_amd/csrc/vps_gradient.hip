@@ -125,7 +125,15 @@ __global__ void __launch_bounds__(256) k_gradient(
         if (d2 < k.h2) {
             const KW kh = kernel_f32(d2, k.h2, k.hf, k.inv_h, k.c0h, k.c1h);
             const float4 vb = bvel[j];
-            const float4 nb = bnrm[j];
+            float4 nb = bnrm[j];
+            if (nb.x == 0.f && nb.y == 0.f && (D == 2 || nb.z == 0.f)) {
+                // a grain: the unit vector from the grain to the particle
+                // (none when they coincide)
+                const float inv = kh.r > 0.f ? 1.f / kh.r : 0.f;
+                nb.x = (float)(q.x - pj.x) * inv;
+                nb.y = (float)(q.y - pj.y) * inv;
+                nb.z = D == 3 ? (float)(q.z - pj.z) * inv : 0.f;
+            }
             const float dn = nb.x * (vi.x - vb.x) + nb.y * (vi.y - vb.y) +
                              (D == 3 ? nb.z * (vi.z - vb.z) : 0.f);
             const float m = dn < 0.f ? dn : 0.f;

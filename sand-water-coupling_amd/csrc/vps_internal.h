@@ -325,6 +325,9 @@ struct vps_context {
     // sorted order: FP32 offset of each particle from its cell corner, w =
     // +-1/rho (negative: clamped into a border cell, exact test only)
     vps::DBuf<float4> rofs;
+    vps::DBuf<float4> dem_vnew;         // DEM step: velocities after the forces
+    vps::DBuf<double> couple_stage;     // grain coupling: boundary x, n, v
+    int64_t couple_walls = -1;          // walls already staged there (-1: none)
     vps::DBuf<int32_t> cell_of, slot_of, perm;
     int64_t ncells = 0;
     vps::DBuf<int32_t> cell_count, cell_start;
@@ -336,6 +339,9 @@ struct vps_context {
     vps::DBuf<float4> bnrm;   // (n, rho_b)
     vps::DBuf<int32_t> bid;
     vps::DBuf<int32_t> bcell_start;
+    vps::DBuf<double4> bpos_spare;   // sort targets of the boundary arrays
+    vps::DBuf<float4> bvel_spare, bnrm_spare;
+    vps::DBuf<int32_t> bid_spare;
     std::vector<double> host_bx, host_bn, host_bv;
 
     // lattice / bricks
@@ -427,6 +433,8 @@ int scan_exclusive_i32(vps_context* c, const int32_t* in, int32_t* out,
                        int64_t n, int32_t* total_dev);
 int cells_sort_real(vps_context* c);
 int cells_sort_boundary(vps_context* c);
+int upload_boundary_device(vps_context* c, int64_t n, const double* dx,
+                           const double* dn, const double* dv);
 int real_density(vps_context* c, bool set_volume);
 int boundary_density(vps_context* c);
 int virtual_generate(vps_context* c);

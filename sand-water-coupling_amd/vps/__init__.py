@@ -5,5 +5,5 @@ is a thin ctypes layer used by tests and the benchmark.  ``Projection``
 mirrors the reference's DualParticleSphLiquid / VirtualParticle call order
 (Cores/Physics/DualParticleSphLiquid.cpp:78-126).
 """
-from .core import Projection, VpsError, load_library  # noqa: F401
+from .core import DemSand, Projection, VpsError, load_library  # noqa: F401
 from . import scenes  # noqa: F401

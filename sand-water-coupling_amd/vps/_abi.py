@@ -139,6 +139,23 @@ _F = C.POINTER(C.c_float)
 _I64 = C.POINTER(C.c_int64)
 
 
+class DemParams(C.Structure):
+    """Contact constants of DEMParticle<Dim> (Cores/Structures/DEMParticle.cpp:
+    15-60): Young's modulus (K_norm = young r), Poisson ratio
+    (K_tang = K_norm poisson) and the friction angle in radians."""
+    _fields_ = [
+        ("young", C.c_double),
+        ("poisson", C.c_double),
+        ("friction_angle", C.c_double),
+        ("reserved", C.c_double),
+    ]
+
+
+def dem_params(young=1e9, poisson=0.3, friction_angle=0.5):
+    """The reference's defaults (DEMParticle.cpp:18-21)."""
+    return DemParams(young, poisson, friction_angle, 0.0)
+
+
 def declare(lib):
     """Declares every exported symbol of libvps on a CDLL handle."""
     _proto(lib, "vps_abi_version", C.c_int32, [])
@@ -175,6 +192,9 @@ def declare(lib):
     _proto(lib, "vps_mlp_forward", C.c_int, [_P, C.c_int64, _F, _F])
     _proto(lib, "vps_mlp_get_features", C.c_int, [_P, _F])
     _proto(lib, "vps_mlp_training_state", C.c_int, [_P, _D, _I64])
+    _proto(lib, "vps_dem_step", C.c_int,
+           [_P, C.c_double, C.c_double, C.POINTER(DemParams)])
+    _proto(lib, "vps_couple_grains", C.c_int, [_P, _P])
     _proto(lib, "vps_debug_apply_operator", C.c_int, [_P, _D, _D])
     return lib
 
@@ -191,7 +211,7 @@ EXPORTED = [
     "vps_query_neighbors", "vps_mlp_set_training", "vps_mlp_parameter_count",
     "vps_mlp_get_parameters", "vps_mlp_set_parameters",
     "vps_mlp_loss_and_grad", "vps_mlp_forward", "vps_mlp_get_features",
-    "vps_mlp_training_state",
+    "vps_mlp_training_state", "vps_dem_step", "vps_couple_grains",
     "vps_debug_apply_operator",
 ]
 

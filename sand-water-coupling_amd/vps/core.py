@@ -246,3 +246,27 @@ class Projection:
         loss, steps = C.c_double(), C.c_int64()
         self._check(self.L.vps_mlp_training_state(self.h, C.byref(loss), C.byref(steps)))
         return loss.value, steps.value
+
+    # -- grains as moving boundary particles
+    def couple_grains(self, grains):
+        """Boundary = this context's walls followed by the grains of a
+        ``DemSand`` context (zero normals: the pressure-gradient boundary term
+        takes the unit vector from the grain to the fluid particle)."""
+        self._check(self.L.vps_couple_grains(self.h, grains.h))
+        self.nb = len(self.scene["bx"]) + grains.n
+
+
+class DemSand(Projection):
+    """The grains of DEMParticleSand<Dim> (Cores/Physics/DEMParticleSand.h:13-58)
+    on the GPU: the scene's particles are the grains, its boundary particles
+    the walls.  ``advance(dt)`` is DEMParticleSand::advance
+    (DEMParticleSand.cpp:90-94): move, contact and wall forces, gravity."""
+
+    def __init__(self, scene, young=1e9, poisson=0.3, friction_angle=0.5,
+                 gravity=9.8, **overrides):
+        super().__init__(scene, **overrides)
+        self.dem = _abi.dem_params(young, poisson, friction_angle)
+        self.gravity = gravity
+
+    def advance(self, dt):
+        self._check(self.L.vps_dem_step(self.h, dt, self.gravity, C.byref(self.dem)))

@@ -150,3 +150,115 @@ def small_3d(nx=12, ny=8, nz=6, seed=DEFAULT_SEED, jitter=0.1):
     box = (2 * nx * dx, (ny + 4) * dx, (nz + 4) * dx)
     return dam_break(3, dx, box, block, seed, jitter=jitter,
                      name="small_3d_%d" % (nx * ny * nz))
+
+
+def _box_walls(dim, dx, box, wall_layers):
+    """Cell-centred wall lattice around the box [0, box] with the normals of
+    the box's complement (pointing inwards) and zero velocity."""
+    nbox = np.rint(np.asarray(box[:dim]) / dx).astype(np.int64)
+    waxes = [(np.arange(-wall_layers, nbox[a] + wall_layers) + 0.5) * dx
+             for a in range(dim)]
+    wgrid = np.meshgrid(*waxes, indexing="ij")
+    w = np.stack([g.transpose().reshape(-1) for g in wgrid], axis=1)
+    outside = np.any((w < 0) | (w > np.asarray(box[:dim])), axis=1)
+    w = np.ascontiguousarray(w[outside])
+    return w, box_complement_normals(w, np.zeros(dim), np.asarray(box[:dim])), np.zeros_like(w)
+
+
+def grain_scene(x, v, particle_radius, bx, bn, bv, domain_lo, domain_hi,
+                mass=1.0, name="grains"):
+    """Scene of a DEM context: the grains are the particles, the walls the
+    boundary particles.  The virtual lattice is a single node (a DEM context
+    never generates virtual particles).  Grain-grain contacts reach 2r; the
+    context's cells are sized by its kernel radius 4r."""
+    dim = x.shape[1]
+    lo = np.zeros(3)
+    hi = np.zeros(3)
+    lo[:dim] = domain_lo[:dim]
+    hi[:dim] = domain_hi[:dim]
+    return {
+        "name": name,
+        "dim": dim,
+        "dx": 2 * particle_radius,
+        "x": np.ascontiguousarray(x, dtype=np.float64),
+        "v": np.ascontiguousarray(v, dtype=np.float64),
+        "bx": np.ascontiguousarray(bx, dtype=np.float64).reshape(-1, dim),
+        "bn": np.ascontiguousarray(bn, dtype=np.float64).reshape(-1, dim),
+        "bv": np.ascontiguousarray(bv, dtype=np.float64).reshape(-1, dim),
+        "particle_radius": particle_radius,
+        "lattice_spacing": 2 * particle_radius,
+        "lattice_origin": lo.copy(),
+        "lattice_dims": np.ones(3, dtype=np.int64),
+        "mass": mass,
+        "rest_density": 1000.0,
+        "kappa": mass / 1000.0,
+        "domain_lo": lo,
+        "domain_hi": hi,
+    }
+
+
+def sand_water(dim=3, dx=1.0 / 150, box=(1.0, 1.2, 1.0), bed=0.3, water=0.6,
+               seed=DEFAULT_SEED, jitter=0.0, wall_layers=3, rho0=1000.0,
+               grain_speed=0.0):
+    """Sand bed under a water column, both on the cell-centred lattice of
+    spacing dx, in the box [0, box]: grains of radius r = dx / 2 (spacing 2r)
+    fill 0 <= y < bed, water fills bed <= y < bed + water over the whole
+    footprint.  dx = 1/150 in the 1 x 1.2 x 1 box gives 150 x 45 x 150 =
+    1,012,500 grains and 150 x 90 x 150 = 2,025,000 water particles.
+
+    Returns (water scene, grain scene): the water's boundary is the walls (the
+    grains are added each step by coupling), the grains' boundary the same
+    walls.  ``jitter`` displaces grains by a uniform value in
+    [-jitter dx, jitter dx] per coordinate (overlapping grains push apart);
+    ``grain_speed`` gives every grain a random velocity of that size."""
+    box = np.asarray(box, dtype=np.float64)[:dim]
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = np.rint(box / dx).astype(np.int64)
+    nbed = int(round(bed / dx))
+    nwat = int(round(water / dx))
+
+    def layer_block(j0, j1):
+        axes = [(np.arange(n[a]) + 0.5) * dx for a in range(dim)]
+        axes[1] = (np.arange(j0, j1) + 0.5) * dx
+        grid = np.meshgrid(*axes, indexing="ij")
+        return np.stack([g.transpose().reshape(-1) for g in grid], axis=1)
+
+    g = layer_block(0, nbed)
+    if jitter:
+        g = g + rng.uniform(-jitter * dx, jitter * dx, size=g.shape)
+    gv = np.zeros_like(g)
+    if grain_speed:
+        d = rng.normal(size=g.shape)
+        gv = grain_speed * d / np.linalg.norm(d, axis=1, keepdims=True)
+    x = layer_block(nbed, nbed + nwat)
+    w, wn, wv = _box_walls(dim, dx, box, wall_layers)
+    origin, dims = staggered_node_grid(np.zeros(dim), box, dx, dim, 2)
+    mass = rho0 * dx ** dim
+    h = 2 * dx
+    lo = np.zeros(3)
+    hi = np.zeros(3)
+    lo[:dim] = -(wall_layers + 1) * dx - h
+    hi[:dim] = box + (wall_layers + 1) * dx + h
+    water_scene = {
+        "name": "sand_water_%dd" % dim,
+        "dim": dim,
+        "dx": dx,
+        "x": np.ascontiguousarray(x),
+        "v": np.zeros_like(x),
+        "bx": w,
+        "bn": wn,
+        "bv": wv,
+        "particle_radius": dx / 2,
+        "lattice_spacing": dx,
+        "lattice_origin": origin,
+        "lattice_dims": dims,
+        "mass": mass,
+        "rest_density": rho0,
+        "kappa": mass / rho0,
+        "domain_lo": lo,
+        "domain_hi": hi,
+        "seed": seed,
+    }
+    grains = grain_scene(g, gv, dx / 2, w, wn, wv, lo, hi, mass=mass,
+                         name="sand_bed_%dd" % dim)
+    return water_scene, grains
