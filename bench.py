@@ -57,6 +57,14 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--workload", default="dam_break", choices=["dam_break", "sand_water"],
+                    help="dam_break: the 3D dam break (the headline line); "
+                         "sand_water: water column falling on a DEM sand bed, "
+                         "grains as moving boundary particles, learned warm start")
+    ap.add_argument("--sand-jitter", type=float, default=0.02,
+                    help="grain packing jitter in units of dx")
+    ap.add_argument("--cpu-sand-dx", type=float, default=1.0 / 40,
+                    help="spacing of the CPU sample of the sand-water step")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB "
                          "(3 times) and exit: the reference kernel that "
@@ -189,6 +197,161 @@ def run(args, sc, warm, train=False, kernel_timing=False):
                 train_iters=train_iters, timed_launch=timed_launch)
 
 
+def sand_water_scenes(dx, seed, jitter):
+    """2M water over a 1M-grain bed at dx = 1/150: a 1 x 0.3 x 1 bed of
+    grains of radius dx/2, a 0.6 deep water column 0.05 above it, in a
+    1 x 1.2 x 1 box."""
+    from vps import scenes
+    return scenes.sand_water(3, dx, (1.0, 1.2, 1.0), bed=0.3, water=0.6, gap=0.05,
+                             seed=seed, jitter=jitter)
+
+
+def sand_water_step(grains, water, dt):
+    """DEM step of the grains, grains into the water's boundary, advection,
+    projection; returns the projection stats and the host-timed phases."""
+    t0 = time.perf_counter()
+    grains.advance(dt)
+    t1 = time.perf_counter()
+    water.couple_grains(grains)
+    t2 = time.perf_counter()
+    water.advect(dt, 9.8)
+    st = water.projection_step()
+    st["ms_dem"] = 1e3 * (t1 - t0)
+    st["ms_couple"] = 1e3 * (t2 - t1)
+    return st
+
+
+def cpu_baseline_sand_water(args):
+    """The same coupled step on the FP64 oracle (DEM step, boundary =
+    walls + grains, advection, projection) at a coarser spacing."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    from vps import _abi
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    ws, gs = sand_water_scenes(args.cpu_sand_dx, args.seed, args.sand_jitter)
+    og = oracle_py.Oracle(gs, threads=threads)
+    w2 = dict(ws)
+    w2["bx"] = np.concatenate([ws["bx"], gs["x"]])
+    w2["bn"] = np.concatenate([ws["bn"], np.zeros_like(gs["x"])])
+    w2["bv"] = np.concatenate([ws["bv"], gs["v"]])
+    ow = oracle_py.Oracle(w2, threads=threads, warm_start=_abi.WARM_EXTRAPOLATE,
+                          rest_density_mode=_abi.FROM_PARAMS, kappa_mode=_abi.FROM_PARAMS)
+    ow.initialize()
+    times = []
+    for _ in range(args.cpu_steps + 1):
+        t0 = time.perf_counter()
+        og.dem_step(args.dt, 9.8)
+        xg, vg = og.particles()
+        ow.set_boundary(np.concatenate([ws["bx"], xg]),
+                        np.concatenate([ws["bn"], np.zeros_like(xg)]),
+                        np.concatenate([ws["bv"], vg]))
+        ow.advect(args.dt, 9.8)
+        ow.projection_step()
+        times.append(time.perf_counter() - t0)
+    times = times[1:]
+    med = float(np.median(times))
+    n = len(ws["x"])
+    og.close()
+    ow.close()
+    return {
+        "value": n / med,
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "FP64 oracle coupled step (DEM step, walls + grains boundary "
+                  "refresh, advection, projection with the extrapolated warm "
+                  "start) on the same scene at dx=%.5g (%d water, %d grains), "
+                  "median of %d steps after 1 warm-up, %d OpenMP threads"
+                  % (args.cpu_sand_dx, n, len(gs["x"]), len(times), threads),
+    }
+
+
+def main_sand_water(args):
+    from vps import DemSand, Projection, _abi
+    ws, gs = sand_water_scenes(args.dx, args.seed, args.sand_jitter)
+    n, ng = len(ws["x"]), len(gs["x"])
+    grains = DemSand(gs)
+    water = Projection(ws, warm_start=WARM["learned"],
+                       rest_density_mode=_abi.FROM_PARAMS, kappa_mode=_abi.FROM_PARAMS)
+    t0 = time.perf_counter()
+    water.couple_grains(grains)
+    water.initialize()
+    log("sand_water: init %.3f s, water %d, grains %d, boundary %d"
+        % (time.perf_counter() - t0, n, ng, water.nb))
+    water.mlp_set_training(True, batch=args.batch, lr=args.lr,
+                           steps_per_solve=args.train_per_solve, seed=args.seed)
+    train_iters = []
+    for k in range(args.train_steps):
+        train_iters.append(sand_water_step(grains, water, args.dt)["cg_iterations"])
+    water.mlp_set_training(False)
+    for k in range(args.warmup):
+        st = sand_water_step(grains, water, args.dt)
+        log("sand_water warmup %d: nv=%d iters=%d" % (k, st["n_virtual"], st["cg_iterations"]))
+    stats = []
+    water.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        stats.append(sand_water_step(grains, water, args.dt))
+    water.synchronize()
+    elapsed = time.perf_counter() - t0
+    kt = []
+    if not args.no_kernel_timing:
+        water.set_kernel_timing(True)
+        for k in range(5):
+            kt.append(sand_water_step(grains, water, args.dt))
+        water.set_kernel_timing(False)
+    _, vg = grains.particles()
+    grains.close()
+    water.close()
+    ms_step = 1e3 * elapsed / args.steps
+    iters = [s["cg_iterations"] for s in stats]
+    spmv = [s["ms_spmv_avg"] for s in kt if s["ms_spmv_avg"] > 0]
+    roof = None
+    if spmv:
+        t_launch = float(np.mean(spmv)) * 1e-3
+        bytes_launch = 24.0 * float(np.mean([s["n_virtual"] for s in kt]))
+        ach = bytes_launch / t_launch / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_stencil3<SPMV_FUSED, TRANSPOSE>",
+                "avg_launch_us": round(t_launch * 1e6, 2),
+                "bytes_per_launch": bytes_launch}
+    out = {
+        "metric": METRIC,
+        "value": round(n * args.steps / elapsed, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 sums/SpMV/DEM forces, f64 dots+membership+positions",
+        "data": "synthetic",
+        "config": {"workload": "sand_water_3d_%d_%d" % (n, ng), "n_water": n,
+                   "n_grains": ng, "n_boundary": len(ws["bx"]) + ng, "dx": args.dx,
+                   "dt": args.dt, "seed": args.seed, "warm_start": "learned",
+                   "grain_jitter_dx": args.sand_jitter, "tolerance": 1e-6,
+                   "untimed_steps": args.train_steps + args.warmup,
+                   "parallelism": "single GPU"},
+        "cg_iters_mean": round(float(np.mean(iters)), 2),
+        "cg_iters": iters,
+        "cg_iters_while_training": train_iters,
+        "true_rel_residual_max": max(s["cg_true_rel_residual"] for s in stats),
+        "phase_ms_mean": {k: round(float(np.mean([s[k] for s in stats])), 4)
+                          for k in ("ms_dem", "ms_couple", "ms_sort", "ms_generate",
+                                    "ms_diag", "ms_cg", "ms_gradient", "ms_total")},
+        "grain_speed_max": float(np.abs(vg).max()),
+        "roofline": roof,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_sand_water(args)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def summary(r, n, steps):
     st = r["stats"]
     return {
@@ -208,6 +371,8 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU "
               "path; run with --gpus 1", file=sys.stderr)
         return 2
+    if args.workload == "sand_water":
+        return main_sand_water(args)
     from vps import scenes
     sc = scenes.config2(seed=args.seed, dx=args.dx)
     n = len(sc["x"])

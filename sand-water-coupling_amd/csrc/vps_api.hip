@@ -170,6 +170,8 @@ void free_all(vps_context* c) {
     dfree(c->rrho);
     dfree(c->rofs);
     dfree(c->dem_vnew);
+    dfree(c->grad_chunks);
+    dfree(c->grad_over);
     dfree(c->couple_stage);
     dfree(c->cell_of);
     dfree(c->slot_of);
@@ -368,6 +370,9 @@ int vps_create(const vps_params* params, vps_context** out) {
         CgState z{};
         if (hipMemset(c->cg.p, 0, sizeof(CgState)) != hipSuccess ||
             hipMemset(c->counters.p, 0, 64 * sizeof(int64_t)) != hipSuccess ||
+            // counting arrays start at zero; their consumers clear them again
+            hipMemset(c->cell_count.p, 0, (size_t)(c->ncells + 1) * 4) != hipSuccess ||
+            hipMemset(c->brick_flag.p, 0, (size_t)c->nbricks * 4) != hipSuccess ||
             hipMemset(c->vq.p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->vrho.p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->vvol.p, 0, cap * 4) != hipSuccess ||
@@ -387,6 +392,7 @@ int vps_create(const vps_params* params, vps_context** out) {
         (void)z;
     }
     if (rc == VPS_OK) rc = mlp_create(c);
+    if (rc == VPS_OK) rc = gradient_setup(c);
     if (rc != VPS_OK) {
         const std::string msg = c->err.empty() ? "allocation failed" : c->err;
         vps_destroy(c);

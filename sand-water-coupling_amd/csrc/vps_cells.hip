@@ -184,7 +184,7 @@ static int sort_set(vps_context* c, int64_t n, const double4* pin,
                     B* bout, const int32_t* idin, int32_t* idout,
                     int32_t* cstart) {
     const int64_t C = c->ncells;
-    VPS_HIP(c, hipMemsetAsync(c->cell_count.p, 0, (size_t)C * 4, c->stream));
+    // cell_count is zero on entry: cleared at creation and by every scan of it
     if (c->dim == 3)
         hipLaunchKernelGGL(k_cell_count<3>, dim3(blocks(n, 256)), dim3(256), 0,
                            c->stream, pin, n, c->k, c->cell_count.p,
@@ -193,7 +193,8 @@ static int sort_set(vps_context* c, int64_t n, const double4* pin,
         hipLaunchKernelGGL(k_cell_count<2>, dim3(blocks(n, 256)), dim3(256), 0,
                            c->stream, pin, n, c->k, c->cell_count.p,
                            c->cell_of.p, c->slot_of.p);
-    VPS_TRY(scan_exclusive_i32(c, c->cell_count.p, cstart, C, cstart + C));
+    VPS_TRY(scan_exclusive_i32(c, c->cell_count.p, cstart, C, cstart + C,
+                               c->cell_count.p));
     hipLaunchKernelGGL(k_cell_scatter, dim3(blocks(n, 256)), dim3(256), 0,
                        c->stream, c->cell_of.p, c->slot_of.p, cstart, n,
                        c->perm.p);

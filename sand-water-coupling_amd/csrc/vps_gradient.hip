@@ -12,6 +12,8 @@
 // Grid::dataPosition rounds them) and the exact FP64 test
 // ((dx^2 + dy^2) + dz^2) < H^2 decides membership.  The brick map gives
 // q_J = V_J p_J (zero on non-virtual nodes).
+#include <cmath>
+
 #include "vps_internal.h"
 
 namespace vps {
@@ -21,18 +23,48 @@ inline unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs)
 
 constexpr int kBox = 10;   // max node candidates per axis (2H/s + 2)
 
+// Boundary part of the update: -sum_b V_b min(n_b.(v_i - v_b), 0) n_b W_h.
 template <int D>
-__global__ void __launch_bounds__(256) k_gradient(
-    KConst k, int64_t n, const double4* rpos, float4* rvel,
-    const int32_t* slot_of_brick, const float* vq, const double4* bpos,
-    const float4* bvel, const float4* bnrm, const int32_t* bcstart) {
-    const int nblk = (int)((n + 255) / 256);
-    const int64_t i = (int64_t)xcd_remap(blockIdx.x, nblk) * 256 + threadIdx.x;
-    if (i >= n) return;
-    const double4 q = rpos[i];
+__device__ __forceinline__ void boundary_term(const KConst& k, const double4& q,
+                                              const float4& vi, const double4* bpos,
+                                              const float4* bvel, const float4* bnrm,
+                                              const int32_t* bcstart, float& fx,
+                                              float& fy, float& fz) {
+    for_each_candidate<D>(k, bcstart, bpos, q.x, q.y, q.z, [&](int j, const double4& pj) {
+        const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, q.x, q.y, q.z)
+                                 : sq_dist2(pj.x, pj.y, q.x, q.y);
+        if (d2 < k.h2) {
+            const KW kh = kernel_f32(d2, k.h2, k.hf, k.inv_h, k.c0h, k.c1h);
+            const float4 vb = bvel[j];
+            float4 nb = bnrm[j];
+            if (nb.x == 0.f && nb.y == 0.f && (D == 2 || nb.z == 0.f)) {
+                // a grain: the unit vector from the grain to the particle
+                // (none when they coincide)
+                const float inv = kh.r > 0.f ? 1.f / kh.r : 0.f;
+                nb.x = (float)(q.x - pj.x) * inv;
+                nb.y = (float)(q.y - pj.y) * inv;
+                nb.z = D == 3 ? (float)(q.z - pj.z) * inv : 0.f;
+            }
+            const float dn = nb.x * (vi.x - vb.x) + nb.y * (vi.y - vb.y) +
+                             (D == 3 ? nb.z * (vi.z - vb.z) : 0.f);
+            const float m = dn < 0.f ? dn : 0.f;
+            const float sc = vb.w * m * kh.w;
+            fx -= sc * nb.x;
+            fy -= sc * nb.y;
+            if (D == 3) fz -= sc * nb.z;
+        }
+    });
+}
+
+
+// Virtual part of the update from global memory: the node box around x_i,
+// exact FP64 membership, brick map lookups.
+template <int D>
+__device__ __forceinline__ void virtual_term_global(const KConst& k, const double4& q,
+                                                    const int32_t* slot_of_brick,
+                                                    const float* vq, float& fx,
+                                                    float& fy, float& fz) {
     const double qc[3] = {q.x, q.y, q.z};
-    float4 vi = rvel[i];
-    float fx = 0.f, fy = 0.f, fz = 0.f;
     const double inv = 1.0 / k.spacing;
     int lo[3] = {0, 0, 0}, cnt[3] = {1, 1, 1};
     bool empty = false;
@@ -118,35 +150,256 @@ __global__ void __launch_bounds__(256) k_gradient(
             }
         }
     }
-    // boundary term
-    for_each_candidate<D>(k, bcstart, bpos, q.x, q.y, q.z, [&](int j, const double4& pj) {
-        const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, q.x, q.y, q.z)
-                                 : sq_dist2(pj.x, pj.y, q.x, q.y);
-        if (d2 < k.h2) {
-            const KW kh = kernel_f32(d2, k.h2, k.hf, k.inv_h, k.c0h, k.c1h);
-            const float4 vb = bvel[j];
-            float4 nb = bnrm[j];
-            if (nb.x == 0.f && nb.y == 0.f && (D == 2 || nb.z == 0.f)) {
-                // a grain: the unit vector from the grain to the particle
-                // (none when they coincide)
-                const float inv = kh.r > 0.f ? 1.f / kh.r : 0.f;
-                nb.x = (float)(q.x - pj.x) * inv;
-                nb.y = (float)(q.y - pj.y) * inv;
-                nb.z = D == 3 ? (float)(q.z - pj.z) * inv : 0.f;
-            }
-            const float dn = nb.x * (vi.x - vb.x) + nb.y * (vi.y - vb.y) +
-                             (D == 3 ? nb.z * (vi.z - vb.z) : 0.f);
-            const float m = dn < 0.f ? dn : 0.f;
-            const float sc = vb.w * m * kh.w;
-            fx -= sc * nb.x;
-            fy -= sc * nb.y;
-            if (D == 3) fz -= sc * nb.z;
-        }
-    });
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_gradient(
+    KConst k, int64_t n, const double4* rpos, float4* rvel,
+    const int32_t* slot_of_brick, const float* vq, const double4* bpos,
+    const float4* bvel, const float4* bnrm, const int32_t* bcstart) {
+    const int nblk = (int)((n + 255) / 256);
+    const int64_t i = (int64_t)xcd_remap(blockIdx.x, nblk) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double4 q = rpos[i];
+    float4 vi = rvel[i];
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    virtual_term_global<D>(k, q, slot_of_brick, vq, fx, fy, fz);
+    boundary_term<D>(k, q, vi, bpos, bvel, bnrm, bcstart, fx, fy, fz);
     vi.x += fx;
     vi.y += fy;
     if (D == 3) vi.z += fz;
     rvel[i] = vi;
+}
+
+// ---- LDS-staged 3D gradient ---------------------------------------------
+// Work unit: a span of up to span_cells consecutive cells of one cell row
+// (the particles of a span are contiguous in the sorted order).  The block
+// stages q = V p of the node box that holds every particle's candidates
+// (nodes f-3 .. f+4 per axis around f = floor((x - origin) / s): the nodes
+// within H = 4s of x), zero off the virtual set and off the lattice, then
+// each thread sums its particle's 8 x 8 x 8 candidates from LDS.  Nodes at
+// or beyond H get S = 0 from the spline itself; the kernel gradient is
+// continuous and zero at H, so the FP32 distance decides membership (a
+// node within rounding of H contributes nothing measurable).  A span whose
+// box exceeds the stage (particles far outside their cells) takes the
+// global per-particle path.
+constexpr int kGradThreads = 256;
+constexpr int kGradCap = 7168;   // staged nodes (28 KB: five blocks per CU)
+
+__global__ void __launch_bounds__(256) k_spans(KConst k, int span_cells, int nspan,
+                                               const int32_t* cstart, int4* chunks,
+                                               int32_t* count) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t nrows = (int64_t)k.cdims[1] * k.cdims[2];
+    if (i >= nrows * nspan) return;
+    const int64_t row = i / nspan;
+    const int sp = (int)(i - row * nspan);
+    const int64_t c0 = row * k.cdims[0] + (int64_t)sp * span_cells;
+    const int e = (sp + 1) * span_cells < k.cdims[0] ? (sp + 1) * span_cells : k.cdims[0];
+    const int64_t c1 = row * k.cdims[0] + e;
+    const int a = cstart[c0], b = cstart[c1];
+    if (b > a) chunks[atomicAdd(count, 1)] = make_int4(a, b, (int)c0, (int)c1);
+}
+
+__device__ __forceinline__ int node_floor(double x, double o, double inv) {
+    double f = floor((x - o) * inv);
+    f = f < -1.0e9 ? -1.0e9 : (f > 1.0e9 ? 1.0e9 : f);
+    return (int)f;
+}
+
+// Spans whose box exceeds the stage, from the overflow list that
+// k_gradient3s fills: the global-memory path, one thread per particle.
+__global__ void __launch_bounds__(kGradThreads) k_gradient_overflow(
+    KConst k, const int4* chunks, const int32_t* over, const int32_t* nover,
+    const double4* rpos, float4* rvel, const int32_t* slot_of_brick, const float* vq,
+    const double4* bpos, const float4* bvel, const float4* bnrm, const int32_t* bcstart) {
+    const int no = *nover;
+    for (int o = blockIdx.x; o < no; o += gridDim.x) {
+        const int4 ch = chunks[over[o]];
+        for (int p = ch.x + threadIdx.x; p < ch.y; p += kGradThreads) {
+            const double4 q = rpos[p];
+            const float4 vi = rvel[p];
+            float fx = 0.f, fy = 0.f, fz = 0.f;
+            virtual_term_global<3>(k, q, slot_of_brick, vq, fx, fy, fz);
+            boundary_term<3>(k, q, vi, bpos, bvel, bnrm, bcstart, fx, fy, fz);
+            float4 v = vi;
+            v.x += fx;
+            v.y += fy;
+            v.z += fz;
+            rvel[p] = v;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kGradThreads, 4) k_gradient3s(
+    KConst k, const int4* chunks, int nchunks, const double4* rpos, float4* rvel,
+    const int32_t* slot_of_brick, const float* vq, const double4* bpos,
+    const float4* bvel, const float4* bnrm, const int32_t* bcstart, int32_t* over,
+    int32_t* nover) {
+    __shared__ float s_q[kGradCap];
+    __shared__ int s_box[6];
+    __shared__ int s_walls;   // boundary particles in the cells around the span
+    const int chunk = xcd_remap(blockIdx.x, nchunks);
+    const int4 ch = chunks[chunk];
+    const int t = threadIdx.x;
+    if (t < 3) {
+        s_box[t] = 0x7fffffff;
+        s_box[t + 3] = -0x7fffffff;
+    }
+    if (t == 3) s_walls = 0;
+    __syncthreads();
+    if (t < 9) {
+        // the boundary cells that the particles' walks would visit: the
+        // span's cells +-1 along x in the 3 x 3 neighbour rows
+        const int cdx = k.cdims[0], cdy = k.cdims[1], cdz = k.cdims[2];
+        const int row = ch.z / cdx;
+        const int cy = row % cdy, cz = row / cdy;
+        const int x0 = ch.z - row * cdx, x1 = ch.w - 1 - row * cdx;
+        const int yy = cy + t % 3 - 1, zz = cz + t / 3 - 1;
+        if (yy >= 0 && zz >= 0 && yy < cdy && zz < cdz) {
+            const int64_t rb = ((int64_t)zz * cdy + yy) * cdx;
+            const int lo = x0 > 0 ? x0 - 1 : 0, hi = x1 + 1 < cdx ? x1 + 1 : cdx - 1;
+            if (bcstart[rb + hi + 1] > bcstart[rb + lo]) s_walls = 1;
+        }
+    }
+    const double inv = 1.0 / k.spacing;
+    {
+        int mn[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
+        int mx[3] = {-0x7fffffff, -0x7fffffff, -0x7fffffff};
+        for (int p = ch.x + t; p < ch.y; p += kGradThreads) {
+            const double4 q = rpos[p];
+            const double qc[3] = {q.x, q.y, q.z};
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const int f = node_floor(qc[a], k.origin[a], inv);
+                mn[a] = min(mn[a], f);
+                mx[a] = max(mx[a], f);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            mn[a] = -wave_max<int>(-mn[a]);
+            mx[a] = wave_max<int>(mx[a]);
+        }
+        if ((t & 63) == 0) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                atomicMin(&s_box[a], mn[a]);
+                atomicMax(&s_box[a + 3], mx[a]);
+            }
+        }
+    }
+    __syncthreads();
+    const int lo0 = s_box[0] - 3, lo1 = s_box[1] - 3, lo2 = s_box[2] - 3;
+    const int64_t e0 = (int64_t)s_box[3] - s_box[0] + 8;
+    const int64_t e1 = (int64_t)s_box[4] - s_box[1] + 8;
+    const int64_t e2 = (int64_t)s_box[5] - s_box[2] + 8;
+    if (e0 * e1 * e2 > kGradCap) {
+        // particles far outside their cells: left to k_gradient_overflow
+        if (t == 0) over[atomicAdd(nover, 1)] = chunk;
+        return;
+    }
+    {
+        // one wave per box row (iy, iz), lanes along x
+        const int n0 = (int)e0, n1 = (int)e1, nrow = (int)(e1 * e2);
+        const int lane = t & 63;
+        for (int rw = t >> 6; rw < nrow; rw += kGradThreads / 64) {
+            const int iy = rw % n1, iz = rw / n1;
+            const int gy = lo1 + iy, gz = lo2 + iz;
+            const bool yz_in = gy >= 0 && gz >= 0 && gy < k.ndims[1] && gz < k.ndims[2];
+            const int64_t brow = ((int64_t)(gz >> 3) * k.bdims[1] + (gy >> 3)) * k.bdims[0];
+            const int lyz = ((gz & 7) << 6) + ((gy & 7) << 3);
+            float* dst = s_q + rw * n0;
+            for (int ix = lane; ix < n0; ix += 64) {
+                const int gx = lo0 + ix;
+                float v = 0.f;
+                if (yz_in && gx >= 0 && gx < k.ndims[0]) {
+                    const int sl = slot_of_brick[brow + (gx >> 3)];
+                    if (sl >= 0) v = vq[(int64_t)sl * 512 + lyz + (gx & 7)];
+                }
+                dst[ix] = v;
+            }
+        }
+    }
+    __syncthreads();
+    // spline of H as a function of d2 (S = -W'(r) / r):
+    //   r < H/2:     S = c1 / H (2 - 3 r / H)
+    //   H/2 <= r < H: S = c1 (1 - r/H)^2 / r = rinv (c1 + c1 d2 / H^2) - 2 c1 / H
+    const float c1 = k.c1H, iH = k.inv_H;
+    const float kIn0 = 2.f * c1 * iH, kIn1 = 3.f * c1 * iH * iH;
+    const float kOut1 = c1 * iH * iH, kOut0 = -2.f * c1 * iH;
+    const float H2 = (float)k.H2, H2q = 0.25f * (float)k.H2;
+    for (int p0 = ch.x; p0 < ch.y; p0 += kGradThreads) {
+        const int p = p0 + t;
+        if (p >= ch.y) break;
+        const double4 q = rpos[p];
+        const float4 vi = rvel[p];
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        {
+            const int f0 = node_floor(q.x, k.origin[0], inv);
+            const int f1 = node_floor(q.y, k.origin[1], inv);
+            const int f2 = node_floor(q.z, k.origin[2], inv);
+            float dxf[8], dx2[8];
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                dxf[c] = (float)dsub(node_coord(k.origin[0], f0 - 3 + c, k.spacing), q.x);
+                // + 1e-30: rsq stays finite at a coincident node (whose
+                // direction factors are all zero)
+                dx2[c] = dxf[c] * dxf[c] + 1e-30f;
+            }
+            // y and z offsets of candidate 0; candidate c adds c s (FP32)
+            const float dy0 = (float)dsub(node_coord(k.origin[1], f1 - 3, k.spacing), q.y);
+            const float dz0 = (float)dsub(node_coord(k.origin[2], f2 - 3, k.spacing), q.z);
+            const float sf = (float)k.spacing;
+            const int n0 = (int)e0, n1 = (int)e1;
+            const float* base = s_q + ((f2 - 3 - lo2) * n1 + (f1 - 3 - lo1)) * n0 + (f0 - 3 - lo0);
+            // Candidate c of row (cy, cz) is node f - 3 + c: its offset from x
+            // lies in (c - 4, c - 3] s.  With d(c) = the smallest |offset| of
+            // index c (3, 2, 1, 0, 0, 1, 2, 3) a row holds members only if
+            // d(cy)^2 + d(cz)^2 < 16, and only at |c - 3.5| < sqrt(R2) + 0.5,
+            // R2 = 16 - d(cy)^2 - d(cz)^2: all 8 for R2 >= 11, c = 1..6 for
+            // R2 = 6..8, c = 2..5 for R2 = 3 (the uniform bounds below).
+#pragma unroll 1
+            for (int cz = 0; cz < 8; cz++) {
+                const float dzfc = fmaf((float)cz, sf, dz0), dz2c = dzfc * dzfc;
+                const int dzm = cz <= 3 ? 3 - cz : cz - 4;
+                const float* plane = base + cz * n1 * n0;
+#pragma unroll
+                for (int cy = 0; cy < 8; cy++) {
+                    const int dym = cy <= 3 ? 3 - cy : cy - 4;
+                    const int R2 = 16 - dym * dym - dzm * dzm;
+                    if (R2 <= 0) continue;   // uniform: no member in the row
+                    const int cut = R2 >= 11 ? 0 : (R2 >= 6 ? 1 : 2);
+                    const float dyc = fmaf((float)cy, sf, dy0);
+                    const float dyz = fmaf(dyc, dyc, dz2c);
+                    if (!(dyz < H2)) continue;   // this particle: none within H
+                    const float* row = plane + cy * n0;
+                    float sx = 0.f, ss = 0.f;
+#pragma unroll
+                    for (int c = 0; c < 8; c++) {
+                        if (c < cut || c > 7 - cut) continue;   // uniform
+                        const float d2 = dx2[c] + dyz;
+                        const float rinv = __builtin_amdgcn_rsqf(d2);
+                        const float so = fmaf(fmaf(d2, kOut1, c1), rinv, kOut0);
+                        const float si = fmaf(-kIn1, d2 * rinv, kIn0);
+                        const float S = d2 < H2q ? si : (d2 < H2 ? so : 0.f);
+                        const float sc = row[c] * S;
+                        sx = fmaf(sc, dxf[c], sx);
+                        ss += sc;
+                    }
+                    fx -= sx;
+                    fy -= ss * dyc;
+                    fz -= ss * dzfc;
+                }
+            }
+        }
+        if (s_walls) boundary_term<3>(k, q, vi, bpos, bvel, bnrm, bcstart, fx, fy, fz);
+        float4 v = vi;
+        v.x += fx;
+        v.y += fy;
+        v.z += fz;
+        rvel[p] = v;
+    }
 }
 
 template <int D>
@@ -222,11 +475,55 @@ __global__ void __launch_bounds__(128) k_query(
 
 }  // namespace
 
+int gradient_setup(vps_context* c) {
+    c->grad_span_cells = 0;
+    if (c->dim != 3) return VPS_OK;
+    const KConst& k = c->k;
+    // candidates f-3 .. f+4 per axis assume H = 4 s
+    if (!(std::fabs(k.H - 4.0 * k.spacing) <= 1e-12 * k.H)) return VPS_OK;
+    const double ratio = k.cell_size / k.spacing;
+    const int64_t eyz = (int64_t)std::floor(ratio) + 9;   // box extent along y, z
+    const double room = (double)kGradCap / (double)(eyz * eyz) - 9.0;
+    int span = (int)std::floor(room / ratio);
+    if (span < 1) return VPS_OK;
+    if (span > k.cdims[0]) span = k.cdims[0];
+    c->grad_span_cells = span;
+    c->grad_nspan = (k.cdims[0] + span - 1) / span;
+    const int64_t nrows = (int64_t)k.cdims[1] * k.cdims[2];
+    VPS_TRY(dalloc(c, c->grad_chunks, (size_t)(nrows * c->grad_nspan)));
+    VPS_TRY(dalloc(c, c->grad_over, (size_t)(nrows * c->grad_nspan)));
+    return VPS_OK;
+}
+
+int gradient_spans(vps_context* c) {
+    if (c->grad_span_cells <= 0) return VPS_OK;
+    const int64_t total = (int64_t)c->k.cdims[1] * c->k.cdims[2] * c->grad_nspan;
+    hipLaunchKernelGGL(k_spans, dim3(blocks(total, 256)), dim3(256), 0, c->stream, c->k,
+                       c->grad_span_cells, c->grad_nspan, c->cell_start.p,
+                       c->grad_chunks.p, (int32_t*)(c->counters.p + 2));
+    VPS_HIP(c, hipGetLastError());
+    return VPS_OK;
+}
+
 int pressure_gradient(vps_context* c) {
     const int64_t n = c->n;
     const int s = c->cur;
     const int w = c->which;
-    if (c->dim == 3)
+    if (c->dim == 3 && c->grad_span_cells > 0) {
+        if (c->grad_nchunks > 0) {
+            int32_t* nover = (int32_t*)(c->counters.p + 3);   // zeroed with the spans
+            int32_t* over = c->grad_over.p;
+            hipLaunchKernelGGL(k_gradient3s, dim3((unsigned)c->grad_nchunks),
+                               dim3(kGradThreads), 0, c->stream, c->k,
+                               c->grad_chunks.p, (int)c->grad_nchunks, c->rpos[s].p,
+                               c->rvel[s].p, c->brick_slot[w].p, c->vq.p, c->bpos.p,
+                               c->bvel.p, c->bnrm.p, c->bcell_start.p, over, nover);
+            hipLaunchKernelGGL(k_gradient_overflow, dim3(64), dim3(kGradThreads), 0,
+                               c->stream, c->k, c->grad_chunks.p, over, nover,
+                               c->rpos[s].p, c->rvel[s].p, c->brick_slot[w].p, c->vq.p,
+                               c->bpos.p, c->bvel.p, c->bnrm.p, c->bcell_start.p);
+        }
+    } else if (c->dim == 3)
         hipLaunchKernelGGL(k_gradient<3>, dim3(blocks(n, 256)), dim3(256), 0,
                            c->stream, c->k, n, c->rpos[s].p, c->rvel[s].p,
                            c->brick_slot[w].p, c->vq.p, c->bpos.p, c->bvel.p,

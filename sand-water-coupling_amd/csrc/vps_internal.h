@@ -325,6 +325,11 @@ struct vps_context {
     // sorted order: FP32 offset of each particle from its cell corner, w =
     // +-1/rho (negative: clamped into a border cell, exact test only)
     vps::DBuf<float4> rofs;
+    vps::DBuf<int4> grad_chunks;        // 3D gradient: particle and cell ranges of spans
+    vps::DBuf<int32_t> grad_over;       // spans left to the global-memory path
+    int grad_span_cells = 0;            // cells per span (0: global-memory gradient)
+    int grad_nspan = 0;                 // spans per cell row
+    int64_t grad_nchunks = 0;           // non-empty spans of the last generation
     vps::DBuf<float4> dem_vnew;         // DEM step: velocities after the forces
     vps::DBuf<double> couple_stage;     // grain coupling: boundary x, n, v
     int64_t couple_walls = -1;          // walls already staged there (-1: none)
@@ -429,8 +434,14 @@ void dfree(DBuf<T>& b) {
 }
 
 // launchers (defined in the .hip files)
+// 3D gradient set-up at creation: span length and the span list buffer
+int gradient_setup(vps_context* c);
+// appends the non-empty cell spans of the current sort (count in counters[2])
+int gradient_spans(vps_context* c);
+// exclusive scan of in into out (out may alias in); clear_in (== in, not
+// aliased with out) zeroes the input once read
 int scan_exclusive_i32(vps_context* c, const int32_t* in, int32_t* out,
-                       int64_t n, int32_t* total_dev);
+                       int64_t n, int32_t* total_dev, int32_t* clear_in = nullptr);
 int cells_sort_real(vps_context* c);
 int cells_sort_boundary(vps_context* c);
 int upload_boundary_device(vps_context* c, int64_t n, const double* dx,

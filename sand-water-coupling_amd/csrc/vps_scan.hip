@@ -91,10 +91,12 @@ __global__ void __launch_bounds__(1024) k_scan_sums(int32_t* sums, int nt,
     if (threadIdx.x == 0 && total) *total = tot;
 }
 
+// clear != nullptr (the input itself, not aliased with out): the input is
+// zeroed once read, ready for the next counting pass.
 __global__ void __launch_bounds__(kScanBS) k_tile_scan(const int32_t* in,
                                                        int64_t n,
                                                        const int32_t* sums,
-                                                       int32_t* out) {
+                                                       int32_t* out, int32_t* clear) {
     const int64_t base = (int64_t)blockIdx.x * kScanTile;
     // each thread owns kScanIPT consecutive elements
     int v[kScanIPT];
@@ -104,6 +106,13 @@ __global__ void __launch_bounds__(kScanBS) k_tile_scan(const int32_t* in,
         const int64_t i = base + (int64_t)threadIdx.x * kScanIPT + k;
         v[k] = i < n ? in[i] : 0;
         s += v[k];
+    }
+    if (clear) {
+#pragma unroll
+        for (int k = 0; k < kScanIPT; k++) {
+            const int64_t i = base + (int64_t)threadIdx.x * kScanIPT + k;
+            if (i < n) clear[i] = 0;
+        }
     }
     int tot;
     int run = block_exclusive_scan(s, &tot) + sums[blockIdx.x];
@@ -152,7 +161,7 @@ __global__ void __launch_bounds__(256) k_max_f32(const float* in, int64_t n,
 }  // namespace
 
 int scan_exclusive_i32(vps_context* c, const int32_t* in, int32_t* out,
-                       int64_t n, int32_t* total_dev) {
+                       int64_t n, int32_t* total_dev, int32_t* clear_in) {
     if (n <= 0) {
         if (total_dev) VPS_HIP(c, hipMemsetAsync(total_dev, 0, 4, c->stream));
         return VPS_OK;
@@ -165,7 +174,7 @@ int scan_exclusive_i32(vps_context* c, const int32_t* in, int32_t* out,
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, c->stream, sums,
                        (int)nt, total_dev);
     hipLaunchKernelGGL(k_tile_scan, dim3((unsigned)nt), dim3(kScanBS), 0,
-                       c->stream, in, n, sums, out);
+                       c->stream, in, n, sums, out, clear_in);
     VPS_HIP(c, hipGetLastError());
     return VPS_OK;
 }

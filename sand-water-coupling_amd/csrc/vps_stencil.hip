@@ -130,6 +130,7 @@ struct StencilArgs {
     int kk;                // iteration index since the (re)start
     int iter_base;         // Eigen iteration count at the (re)start
     int64_t max_iter;
+    int spec;              // K_TRUE_RES: run only if the solve has converged
 };
 
 // Sum of n per-block partials, identical in every block and lane: lane l
@@ -273,9 +274,10 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
     const int t = threadIdx.x;
     const int l = t & 63, wv = t >> 6;
     if (t < 27) sbase[t] = (uint32_t)a.nbr_slot[(int64_t)slot * 27 + t] * 512u;
-    if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_FUSED) {
+    if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_FUSED || KIND == K_TRUE_RES) {
         // a solve that has ended skips the halo loads (launches past the
-        // convergence of a batch)
+        // convergence of a batch); a speculative true-residual check runs
+        // only once the recursive residual has converged
         if (t == 32) s_done = a.st->done;
     }
     const int x = l & 7, y = l >> 3;
@@ -292,6 +294,9 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
     __syncthreads();
     if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_FUSED) {
         if (s_done) return;   // uniform over the block
+    }
+    if constexpr (KIND == K_TRUE_RES) {
+        if (a.spec && s_done != 1) return;   // uniform over the block
     }
     // Halo load: unit u = (line (yt, zt), part k): k = 0 the upper half row of
     // the left brick (x = -4..-1), 1 and 2 the center row, 3 the lower half
@@ -444,6 +449,9 @@ __global__ void __launch_bounds__(64) k_stencil2(StencilArgs a, C cc) {
     float beta = 0.f;
     if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_FUSED) {
         if (!spmv_prologue(a, &beta)) return;
+    }
+    if constexpr (KIND == K_TRUE_RES) {
+        if (a.spec && a.st->done != 1) return;   // same value in every lane
     }
     const int slot = xcd_remap(blockIdx.x, (int)a.nslots);
     const int brick = a.brick_of_slot[slot];
@@ -617,9 +625,12 @@ __global__ void __launch_bounds__(1024) k_cg_start(const double* pbb,
 }
 
 // Deterministic one-block sum of n partials into *out.
+// st != nullptr: only once the recursive residual has converged (the
+// speculative check launched after every batch)
 __global__ void __launch_bounds__(1024) k_sum_partials(const double* p, int n,
-                                                       double* out) {
+                                                       double* out, const CgState* st) {
     __shared__ double sbuf[16];
+    if (st && st->done != 1) return;   // same value in every thread
     double a = 0;
     for (int i = threadIdx.x; i < n; i += 1024) a += p[i];
     a = block_sum<1024>(a, sbuf);
@@ -907,8 +918,7 @@ int cg_solve(vps_context* c, vps_stats* stats) {
     double host_launch_us = 0;
     while (true) {
         if (hs.done == 0) {
-            int batch = iter_base + kk + 2 < predicted
-                            ? predicted - (iter_base + kk) : 4;
+            int batch = iter_base + kk < predicted ? predicted - (iter_base + kk) : 4;
             if (batch > 64) batch = 64;
             const auto h0 = std::chrono::steady_clock::now();
             for (int j = 0; j < batch; j++) {
@@ -955,6 +965,18 @@ int cg_solve(vps_context* c, vps_stats* stats) {
                 spmv_launches++;
                 kk++;
             }
+            if (c->check_true) {
+                // the true-residual check rides on the same read-back: it
+                // runs on the device only if this batch converged
+                StencilArgs tr = base_args(c);
+                tr.in = x;
+                tr.out = t;   // r_true
+                tr.part0 = part_true(c);
+                tr.spec = 1;
+                launch_op<K_TRUE_RES, double, CoefD>(c, tr, cd);
+                hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, c->stream,
+                                   part_true(c), nslots, &c->cg.p->true_rr, c->cg.p);
+            }
             VPS_HIP(c, hipGetLastError());
             host_launch_us += std::chrono::duration<double, std::micro>(
                                   std::chrono::steady_clock::now() - h0).count();
@@ -979,6 +1001,17 @@ int cg_solve(vps_context* c, vps_stats* stats) {
                     c->err = "CG loop did not terminate";
                     return VPS_ERR_NO_CONVERGENCE;
                 }
+                // size the next batch from the mean residual decay so far
+                const int done_it = hs.iter - iter_base;
+                if (done_it > 0 && hs.abs0 > 0 && hs.rr > 0 && hs.rr < hs.abs0 &&
+                    hs.thr > 0) {
+                    const double rate = std::log(hs.rr / hs.abs0) / done_it;
+                    const double left = std::log(hs.thr / hs.rr) / rate;
+                    int more = left > 0 ? (int)std::ceil(1.1 * left) : 1;
+                    if (more < 1) more = 1;
+                    if (more > 64) more = 64;
+                    predicted = iter_base + kk + more;
+                }
                 continue;
             }
         }
@@ -987,17 +1020,9 @@ int cg_solve(vps_context* c, vps_stats* stats) {
             return VPS_ERR_NONFINITE;
         }
         if (hs.done == 2) break;
-        // converged on the recursive residual: check the true residual
+        // converged on the recursive residual; the true residual of this
+        // state was computed right after the batch
         if (!c->check_true) break;
-        StencilArgs s = base_args(c);
-        s.in = x;
-        s.out = t;   // r_true
-        s.part0 = part_true(c);
-        launch_op<K_TRUE_RES, double, CoefD>(c, s, cd);
-        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, c->stream,
-                           part_true(c), nslots, &c->cg.p->true_rr);
-        VPS_HIP(c, hipGetLastError());
-        VPS_TRY(read_state(c, &hs));
         if (hs.true_rr < hs.thr || replacements >= 4) break;
         // restart from the true residual
         replacements++;

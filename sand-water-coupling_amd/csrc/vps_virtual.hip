@@ -28,11 +28,19 @@ namespace {
 
 inline unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// Marks the bricks within h of each particle and clears the lattice bitmask
+// of the previous generation (grid-stride over its words).
 template <int D>
 __global__ void __launch_bounds__(256) k_mark_bricks(const double4* pos,
                                                      int64_t n, KConst k,
-                                                     uint32_t* flag) {
+                                                     uint32_t* flag, uint32_t* bits,
+                                                     int64_t nwords, int32_t* span_count) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    for (int64_t w = i; w < nwords; w += (int64_t)gridDim.x * 256) bits[w] = 0u;
+    if (i == 0) {   // the gradient's span and overflow lists are rebuilt next
+        span_count[0] = 0;
+        span_count[2] = 0;
+    }
     if (i >= n) return;
     const double4 p = pos[i];
     const double pc[3] = {p.x, p.y, p.z};
@@ -54,16 +62,18 @@ __global__ void __launch_bounds__(256) k_mark_bricks(const double4* pos,
                 flag[((int64_t)bz * k.bdims[1] + by) * k.bdims[0] + bx] = 1u;
 }
 
-__global__ void __launch_bounds__(256) k_assign_slots(const uint32_t* flag,
+__global__ void __launch_bounds__(256) k_assign_slots(uint32_t* flag,
                                                       int64_t nbricks,
                                                       int32_t* slot_of_brick,
                                                       int32_t* brick_of_slot) {
     const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (b >= nbricks) return;
-    // slot_of_brick holds the exclusive prefix of flag on entry
+    // slot_of_brick holds the exclusive prefix of flag on entry; the flag is
+    // cleared for the next generation
     const int s = slot_of_brick[b];
     if (flag[b]) {
         brick_of_slot[s] = (int32_t)b;
+        flag[b] = 0u;
     } else {
         slot_of_brick[b] = -1;
     }
@@ -233,12 +243,14 @@ __device__ __forceinline__ void real_walk_global(const KConst& k,
 // Fused per-node pass.  One workgroup = one brick; thread = node.
 template <int D, int NB>
 __global__ void __launch_bounds__(NB) k_virtual_sums(
-    KConst k, const int32_t* brick_of_slot, int64_t nslots,
+    KConst k, const int32_t* brick_of_slot, const int32_t* nslots_p,
     const double4* rpos, const float4* rvel, const float* rrho,
     const int32_t* rcstart, const double4* bpos, const float4* bvel,
     const float4* bnrm, const int32_t* bcstart, float* vrho, float* vvol,
     float* vb, uint32_t* bits) {
-    const int slot = xcd_remap(blockIdx.x, (int)nslots);
+    const int nslots = *nslots_p;   // the grid may exceed the slot count
+    if ((int)blockIdx.x >= nslots) return;
+    const int slot = xcd_remap(blockIdx.x, nslots);
     const int brick = brick_of_slot[slot];
     const int t = threadIdx.x;
     const int lx = t & 7, ly = (t >> 3) & 7, lz = D == 3 ? t >> 6 : 0;
@@ -303,7 +315,7 @@ struct BoundaryWalkStaged {
 };
 
 __global__ void __launch_bounds__(512) k_virtual_sums3s(
-    KConst k, const int32_t* brick_of_slot, int64_t nslots,
+    KConst k, const int32_t* brick_of_slot, const int32_t* nslots_p,
     const double4* rpos, const float4* rvel, const float* rrho,
     const float4* rofs, const int32_t* rcstart, const double4* bpos,
     const float4* bvel, const float4* bnrm, const int32_t* bcstart,
@@ -314,7 +326,9 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
     __shared__ int s_rg[kStageRows];    // global - staged index
     __shared__ int s_off[kStageRows + 1];
     __shared__ int s_meta[8];   // clo x,y,z; ncx, ncy, ncz; total; ok
-    const int slot = xcd_remap(blockIdx.x, (int)nslots);
+    const int nslots = *nslots_p;   // the grid may exceed the slot count
+    if ((int)blockIdx.x >= nslots) return;
+    const int slot = xcd_remap(blockIdx.x, nslots);
     const int brick = brick_of_slot[slot];
     const int t = threadIdx.x;
     const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
@@ -475,11 +489,11 @@ template <int D>
 __global__ void __launch_bounds__(256) k_nbr_slots(KConst k,
                                                    const int32_t* brick_of_slot,
                                                    const int32_t* slot_of_brick,
-                                                   int64_t nslots, int32_t zero,
+                                                   const int32_t* nslots_p, int32_t zero,
                                                    int32_t* nbr) {
     constexpr int NN = D == 3 ? 27 : 9;
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= nslots * NN) return;
+    if (i >= (int64_t)*nslots_p * NN) return;
     const int64_t slot = i / NN;
     const int t = (int)(i - slot * NN);
     const int brick = brick_of_slot[slot];
@@ -506,12 +520,13 @@ __global__ void __launch_bounds__(256) k_popc(const uint32_t* bits,
 template <int D, int NB>
 __global__ void __launch_bounds__(NB) k_node_rank(KConst k,
                                                   const int32_t* brick_of_slot,
-                                                  int64_t nslots,
+                                                  const int32_t* nslots_p,
                                                   const float* vvol,
                                                   const uint32_t* bits,
                                                   const int32_t* prefix,
                                                   int32_t* rank) {
     const int slot = blockIdx.x;
+    if (slot >= *nslots_p) return;
     const int64_t out = (int64_t)slot * NB + threadIdx.x;
     if (!(vvol[out] > 0.f)) {
         rank[out] = -1;
@@ -570,63 +585,35 @@ __global__ void __launch_bounds__(256) k_after_solve(int64_t nnode,
 
 }  // namespace
 
-int virtual_generate(vps_context* c) {
-    // The new slot map goes into the buffer that does NOT hold the last solved
-    // pressures, so both warm starts can still read them.
-    const int w = c->last_solve_buf >= 0 ? 1 - c->last_solve_buf : 0;
+// Slot-indexed passes of one generation on a grid of `grid` blocks (slots);
+// blocks at or beyond the device-side slot count return at once.
+static int generate_slots(vps_context* c, int w, int64_t grid) {
     const int D = c->dim;
-    VPS_HIP(c, hipMemsetAsync(c->brick_flag.p, 0, (size_t)c->nbricks * 4,
-                              c->stream));
-    const int s = c->cur;
-    if (D == 3)
-        hipLaunchKernelGGL(k_mark_bricks<3>, dim3(blocks(c->n, 256)), dim3(256),
-                           0, c->stream, c->rpos[s].p, c->n, c->k,
-                           c->brick_flag.p);
-    else
-        hipLaunchKernelGGL(k_mark_bricks<2>, dim3(blocks(c->n, 256)), dim3(256),
-                           0, c->stream, c->rpos[s].p, c->n, c->k,
-                           c->brick_flag.p);
-    int64_t* dcnt = c->counters.p;
-    VPS_TRY(scan_exclusive_i32(c, (const int32_t*)c->brick_flag.p,
-                               c->brick_slot[w].p, c->nbricks,
-                               (int32_t*)dcnt));
-    hipLaunchKernelGGL(k_assign_slots, dim3(blocks(c->nbricks, 256)), dim3(256),
-                       0, c->stream, c->brick_flag.p, c->nbricks,
-                       c->brick_slot[w].p, c->slot_brick[w].p);
-    VPS_HIP(c, hipMemcpyAsync(c->host_counters, dcnt, 4, hipMemcpyDeviceToHost,
-                              c->stream));
-    VPS_HIP(c, hipStreamSynchronize(c->stream));
-    const int64_t nslots = *(int32_t*)c->host_counters;
-    c->nslots = nslots;
-    c->which = w;
-    VPS_HIP(c, hipMemsetAsync(c->bits.p, 0, (size_t)c->nwords * 4, c->stream));
-    if (nslots == 0) {
-        c->nvirt = 0;
-        return VPS_OK;
-    }
     const int rs = c->cur;
+    const int32_t* nsl = (const int32_t*)c->counters.p;   // slot count
+    int64_t* dcnt = c->counters.p;
     if (D == 3)
-        hipLaunchKernelGGL(k_virtual_sums3s, dim3((unsigned)nslots),
+        hipLaunchKernelGGL(k_virtual_sums3s, dim3((unsigned)grid),
                            dim3(512), 0, c->stream, c->k, c->slot_brick[w].p,
-                           nslots, c->rpos[rs].p, c->rvel[rs].p, c->rrho.p, c->rofs.p,
+                           nsl, c->rpos[rs].p, c->rvel[rs].p, c->rrho.p, c->rofs.p,
                            c->cell_start.p, c->bpos.p, c->bvel.p, c->bnrm.p,
                            c->bcell_start.p, c->vrho.p, c->vvol.p, c->vb.p,
                            c->bits.p);
     else
-        hipLaunchKernelGGL((k_virtual_sums<2, 64>), dim3((unsigned)nslots),
+        hipLaunchKernelGGL((k_virtual_sums<2, 64>), dim3((unsigned)grid),
                            dim3(64), 0, c->stream, c->k, c->slot_brick[w].p,
-                           nslots, c->rpos[rs].p, c->rvel[rs].p, c->rrho.p,
+                           nsl, c->rpos[rs].p, c->rvel[rs].p, c->rrho.p,
                            c->cell_start.p, c->bpos.p, c->bvel.p, c->bnrm.p,
                            c->bcell_start.p, c->vrho.p, c->vvol.p, c->vb.p,
                            c->bits.p);
     if (D == 3)
-        hipLaunchKernelGGL(k_nbr_slots<3>, dim3(blocks(nslots * 27, 256)), dim3(256),
+        hipLaunchKernelGGL(k_nbr_slots<3>, dim3(blocks(grid * 27, 256)), dim3(256),
                            0, c->stream, c->k, c->slot_brick[w].p, c->brick_slot[w].p,
-                           nslots, (int32_t)c->nbricks, c->nbr_slot.p);
+                           nsl, (int32_t)c->nbricks, c->nbr_slot.p);
     else
-        hipLaunchKernelGGL(k_nbr_slots<2>, dim3(blocks(nslots * 9, 256)), dim3(256),
+        hipLaunchKernelGGL(k_nbr_slots<2>, dim3(blocks(grid * 9, 256)), dim3(256),
                            0, c->stream, c->k, c->slot_brick[w].p, c->brick_slot[w].p,
-                           nslots, (int32_t)c->nbricks, c->nbr_slot.p);
+                           nsl, (int32_t)c->nbricks, c->nbr_slot.p);
     hipLaunchKernelGGL(k_popc, dim3(blocks(c->nwords, 256)), dim3(256), 0,
                        c->stream, c->bits.p, c->nwords,
                        (int32_t*)c->word_prefix.p);
@@ -634,20 +621,74 @@ int virtual_generate(vps_context* c) {
                                (int32_t*)c->word_prefix.p, c->nwords,
                                (int32_t*)(dcnt + 1)));
     if (D == 3)
-        hipLaunchKernelGGL((k_node_rank<3, 512>), dim3((unsigned)nslots),
+        hipLaunchKernelGGL((k_node_rank<3, 512>), dim3((unsigned)grid),
                            dim3(512), 0, c->stream, c->k, c->slot_brick[w].p,
-                           nslots, c->vvol.p, c->bits.p,
+                           nsl, c->vvol.p, c->bits.p,
                            (const int32_t*)c->word_prefix.p, c->vrank.p);
     else
-        hipLaunchKernelGGL((k_node_rank<2, 64>), dim3((unsigned)nslots),
+        hipLaunchKernelGGL((k_node_rank<2, 64>), dim3((unsigned)grid),
                            dim3(64), 0, c->stream, c->k, c->slot_brick[w].p,
-                           nslots, c->vvol.p, c->bits.p,
+                           nsl, c->vvol.p, c->bits.p,
                            (const int32_t*)c->word_prefix.p, c->vrank.p);
-    VPS_HIP(c, hipMemcpyAsync(c->host_counters + 1, dcnt + 1, 4,
-                              hipMemcpyDeviceToHost, c->stream));
-    VPS_HIP(c, hipStreamSynchronize(c->stream));
-    c->nvirt = *(int32_t*)(c->host_counters + 1);
     VPS_HIP(c, hipGetLastError());
+    // slot, virtual and gradient-span counts in one read-back
+    VPS_HIP(c, hipMemcpyAsync(c->host_counters, dcnt, 24, hipMemcpyDeviceToHost,
+                              c->stream));
+    VPS_HIP(c, hipStreamSynchronize(c->stream));
+    return VPS_OK;
+}
+
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int virtual_generate(vps_context* c) {
+    // The new slot map goes into the buffer that does NOT hold the last solved
+    // pressures, so both warm starts can still read them.
+    const int w = c->last_solve_buf >= 0 ? 1 - c->last_solve_buf : 0;
+    const int D = c->dim;
+    const int s = c->cur;
+    // brick flags are zero on entry (cleared by k_assign_slots)
+    const unsigned mb = blocks(c->n, 256) > 0 ? blocks(c->n, 256) : 1;
+    if (D == 3)
+        hipLaunchKernelGGL(k_mark_bricks<3>, dim3(mb), dim3(256),
+                           0, c->stream, c->rpos[s].p, c->n, c->k,
+                           c->brick_flag.p, c->bits.p, c->nwords,
+                           (int32_t*)(c->counters.p + 2));
+    else
+        hipLaunchKernelGGL(k_mark_bricks<2>, dim3(mb), dim3(256),
+                           0, c->stream, c->rpos[s].p, c->n, c->k,
+                           c->brick_flag.p, c->bits.p, c->nwords,
+                           (int32_t*)(c->counters.p + 2));
+    VPS_TRY(gradient_spans(c));
+    int64_t* dcnt = c->counters.p;
+    VPS_TRY(scan_exclusive_i32(c, (const int32_t*)c->brick_flag.p,
+                               c->brick_slot[w].p, c->nbricks,
+                               (int32_t*)dcnt));
+    hipLaunchKernelGGL(k_assign_slots, dim3(blocks(c->nbricks, 256)), dim3(256),
+                       0, c->stream, c->brick_flag.p, c->nbricks,
+                       c->brick_slot[w].p, c->slot_brick[w].p);
+    // The slot count stays on the device: the slot passes run on a grid
+    // sized from the last generation's count (all bricks the first time),
+    // and again on the exact grid in the rare step where it grew past that.
+    int64_t grid = c->nbricks;
+    if (c->nslots > 0) {
+        grid = c->nslots + c->nslots / 8 + 64;
+        if (grid > c->nbricks) grid = c->nbricks;
+    }
+    VPS_TRY(generate_slots(c, w, grid));
+    int64_t nslots = *(int32_t*)c->host_counters;
+    if (nslots > grid) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(blocks(c->nwords, 256)), dim3(256), 0,
+                           c->stream, c->bits.p, c->nwords, 0u);
+        VPS_TRY(generate_slots(c, w, nslots));
+        nslots = *(int32_t*)c->host_counters;
+    }
+    c->nslots = nslots;
+    c->which = w;
+    c->nvirt = nslots ? *(int32_t*)(c->host_counters + 1) : 0;
+    c->grad_nchunks = *(int32_t*)(c->host_counters + 2);
     return VPS_OK;
 }
 

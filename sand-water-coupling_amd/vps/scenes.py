@@ -199,11 +199,11 @@ def grain_scene(x, v, particle_radius, bx, bn, bv, domain_lo, domain_hi,
 
 def sand_water(dim=3, dx=1.0 / 150, box=(1.0, 1.2, 1.0), bed=0.3, water=0.6,
                seed=DEFAULT_SEED, jitter=0.0, wall_layers=3, rho0=1000.0,
-               grain_speed=0.0):
+               grain_speed=0.0, gap=0.0):
     """Sand bed under a water column, both on the cell-centred lattice of
     spacing dx, in the box [0, box]: grains of radius r = dx / 2 (spacing 2r)
-    fill 0 <= y < bed, water fills bed <= y < bed + water over the whole
-    footprint.  dx = 1/150 in the 1 x 1.2 x 1 box gives 150 x 45 x 150 =
+    fill 0 <= y < bed, water fills bed + gap <= y < bed + gap + water over
+    the whole footprint (a gap > 0 makes it a falling column).  dx = 1/150 in the 1 x 1.2 x 1 box gives 150 x 45 x 150 =
     1,012,500 grains and 150 x 90 x 150 = 2,025,000 water particles.
 
     Returns (water scene, grain scene): the water's boundary is the walls (the
@@ -216,6 +216,7 @@ def sand_water(dim=3, dx=1.0 / 150, box=(1.0, 1.2, 1.0), bed=0.3, water=0.6,
     n = np.rint(box / dx).astype(np.int64)
     nbed = int(round(bed / dx))
     nwat = int(round(water / dx))
+    ngap = int(round(gap / dx))
 
     def layer_block(j0, j1):
         axes = [(np.arange(n[a]) + 0.5) * dx for a in range(dim)]
@@ -230,7 +231,7 @@ def sand_water(dim=3, dx=1.0 / 150, box=(1.0, 1.2, 1.0), bed=0.3, water=0.6,
     if grain_speed:
         d = rng.normal(size=g.shape)
         gv = grain_speed * d / np.linalg.norm(d, axis=1, keepdims=True)
-    x = layer_block(nbed, nbed + nwat)
+    x = layer_block(nbed + ngap, nbed + ngap + nwat)
     w, wn, wv = _box_walls(dim, dx, box, wall_layers)
     origin, dims = staggered_node_grid(np.zeros(dim), box, dx, dim, 2)
     mass = rho0 * dx ** dim
