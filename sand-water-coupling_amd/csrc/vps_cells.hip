@@ -127,6 +127,74 @@ __global__ void __launch_bounds__(256) k_density(const double4* pos,
     if (set_volume) vel[i].w = k.mass / r;
 }
 
+// 3D density from the FP32 cell-corner offsets (16 B per candidate instead
+// of the FP64 position): x_i - x_j = (x_i - C_i) - (x_j - C_j) + (C_i - C_j)
+// per axis, with C the FP64 cell corners.  W_h is continuous and zero at h,
+// so the FP32 distance decides which terms enter (a pair within rounding of
+// h adds nothing measurable); particles clamped into a border cell (ofs.w <
+// 0) take the FP64 distance.
+__global__ void __launch_bounds__(256) k_density3f(const double4* pos, const float4* ofs,
+                                                   const int32_t* cstart, int64_t n,
+                                                   KConst k, float* rho, float4* vel,
+                                                   int set_volume) {
+    const int nb = (int)((n + 255) / 256);
+    const int64_t i = (int64_t)xcd_remap(blockIdx.x, nb) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double4 q = pos[i];
+    const double qc[3] = {q.x, q.y, q.z};
+    int cc[3];
+    float qo[3], corner[3][3];   // corner[a][d + 1] = C_q - C_(q + d)
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        cc[a] = cell_coord(qc[a], k.cell_lo[a], k.cell_inv, k.cdims[a]);
+        const double Cq = cell_corner(k, a, cc[a]);
+        qo[a] = (float)(qc[a] - Cq);
+#pragma unroll
+        for (int d = -1; d <= 1; d++)
+            corner[a][d + 1] = (float)(Cq - cell_corner(k, a, cc[a] + d));
+    }
+    const int x0 = cc[0] > 0 ? cc[0] - 1 : 0;
+    const int x1 = cc[0] + 1 < k.cdims[0] ? cc[0] + 1 : k.cdims[0] - 1;
+    const int zlo = cc[2] > 0 ? cc[2] - 1 : 0, zhi = cc[2] + 1 < k.cdims[2] ? cc[2] + 1 : k.cdims[2] - 1;
+    const int ylo = cc[1] > 0 ? cc[1] - 1 : 0, yhi = cc[1] + 1 < k.cdims[1] ? cc[1] + 1 : k.cdims[1] - 1;
+    // x offsets of the (up to) three cells x0, x0 + 1, x0 + 2 of each row
+    const int xb = x0 - cc[0] + 1;   // corner index of cell x0 (0, or 1 at the border)
+    const float tx0 = qo[0] + corner[0][xb];
+    const float tx1 = qo[0] + corner[0][xb + 1 < 2 ? xb + 1 : 2];
+    const float tx2 = qo[0] + corner[0][2];
+    // a particle clamped into a border cell takes the FP64 distance throughout
+    const float far = 2.f * (float)k.cell_size;
+    const bool qfar = !(fabsf(qo[0]) <= far && fabsf(qo[1]) <= far && fabsf(qo[2]) <= far);
+    const float h2 = (float)k.h2;
+    float sum = 0.f;
+    for (int z = zlo; z <= zhi; z++) {
+        const float tz = qo[2] + corner[2][z - cc[2] + 1];
+        for (int y = ylo; y <= yhi; y++) {
+            const float ty = qo[1] + corner[1][y - cc[1] + 1];
+            const int64_t row = ((int64_t)z * k.cdims[1] + y) * k.cdims[0];
+            const int b = cstart[row + x0], e = cstart[row + x1 + 1];
+            const int b1 = x0 + 1 <= x1 ? cstart[row + x0 + 1] : e;
+            const int b2 = x0 + 2 <= x1 ? cstart[row + x0 + 2] : e;
+            for (int j = b; j < e; j++) {
+                const float4 p = ofs[j];
+                const float tx = j < b1 ? tx0 : (j < b2 ? tx1 : tx2);
+                float d2;
+                if (p.w >= 0.f && !qfar) {
+                    const float dx = tx - p.x, dy = ty - p.y, dz = tz - p.z;
+                    d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                } else {
+                    const double4 pj = pos[j];
+                    d2 = (float)sq_dist3(pj.x, pj.y, pj.z, q.x, q.y, q.z);
+                }
+                if (d2 < h2) sum += spline_w(sqrtf(d2) * k.inv_h, k.c0h);
+            }
+        }
+    }
+    const float r = k.mass * sum;
+    rho[i] = r;
+    if (set_volume) vel[i].w = k.mass / r;
+}
+
 // FP32 offset of every (sorted) particle from its cell corner; w = +1, or
 // -1 when the particle is clamped into a border cell (|offset| > 2 cells).
 __global__ void __launch_bounds__(256) k_cell_offsets3(const double4* pos,
@@ -244,8 +312,8 @@ int real_density(vps_context* c, bool set_volume) {
     if (c->dim == 3) {
         hipLaunchKernelGGL(k_cell_offsets3, dim3(blocks(n, 256)), dim3(256), 0,
                            c->stream, c->rpos[s].p, n, c->k, c->rofs.p);
-        hipLaunchKernelGGL(k_density<3>, dim3(blocks(n, 256)), dim3(256), 0,
-                           c->stream, c->rpos[s].p, c->cell_start.p, n, c->k,
+        hipLaunchKernelGGL(k_density3f, dim3(blocks(n, 256)), dim3(256), 0,
+                           c->stream, c->rpos[s].p, c->rofs.p, c->cell_start.p, n, c->k,
                            c->rrho.p, c->rvel[s].p, set_volume ? 1 : 0);
     } else
         hipLaunchKernelGGL(k_density<2>, dim3(blocks(n, 256)), dim3(256), 0,
