@@ -45,8 +45,10 @@ enum vps_status {
     VPS_ERR_NONFINITE = 3,     /* NaN/Inf in the right-hand side or solution */
     VPS_ERR_OOM = 4,           /* device allocation failed */
     VPS_ERR_HIP = 5,           /* any other HIP runtime error */
-    VPS_ERR_STATE = 6          /* call order violated (e.g. project before
+    VPS_ERR_STATE = 6,         /* call order violated (e.g. project before
                                   initialize) */
+    VPS_ERR_IO = 7             /* a frame or checkpoint file could not be
+                                  written or read (reference: std::exit) */
 };
 
 /* Linear operator handed to CG.  Eigen's ConjugateGradient with
@@ -256,6 +258,51 @@ int vps_dem_step(vps_context* grains, double dt, double gravity,
  * grains act on the water as moving boundary particles.  Both contexts must
  * be on the same device. */
 int vps_couple_grains(vps_context* water, vps_context* grains);
+
+/* ---- frames and checkpoints ------------------------------------------ */
+
+/* DualParticleSphLiquid::writeFrame (Cores/Physics/DualParticleSphLiquid.cpp:
+ * 41-76): particles.mesh, virtual_particles.mesh and boundary_particles.mesh
+ * in frame_dir (an existing directory).  Each file: uint32 count, count
+ * float[Dim] positions, in 3D count float[3] normals (0, 0, 1), then count
+ * floats: |v| for the fluid, V_I for the virtual particles of the last
+ * generation, none for the boundary (walls and coupled grains, input order). */
+int vps_write_frame(vps_context* ctx, const char* frame_dir);
+
+/* SmthParticleHydrodLiquid::saveFrame (SmthParticleHydrodLiquid.cpp:34-43):
+ * particles.sav and velocities.sav, the raw FP64 Dim-vectors in caller
+ * order (ParticlesAttribute::save, ParticlesAttribute.h:90), plus
+ * projection.sav: the state the reference does not keep (frozen volumes,
+ * rho0, kappa, alpha0, the last solve's pressures and layout, the learned
+ * warm start's parameters, moments and history).  The last solve is kept
+ * when the call follows a projection (no generation in between). */
+int vps_save_frame(vps_context* ctx, const char* frame_dir);
+
+/* SmthParticleHydrodLiquid::loadFrame (SmthParticleHydrodLiquid.cpp:45-55):
+ * positions and velocities from the .sav files (the particle count is their
+ * size).  With projection.sav the context continues exactly where
+ * vps_save_frame left it (next call: vps_projection_step or refresh,
+ * generate, project); without it, the load re-initialises from the loaded
+ * positions as the reference does (vps_initialize).  The boundary is the
+ * context's own (vps_set_boundary). */
+int vps_load_frame(vps_context* ctx, const char* frame_dir);
+
+/* description.yaml of an output directory (Simulator.cpp:33-39 with
+ * DualParticleSphLiquid::writeDescription, DualParticleSphLiquid.cpp:8-39):
+ * dimension and the three point-list objects of the viewer. */
+int vps_write_description(int32_t dim, const char* output_dir);
+
+/* end_frame.txt = frame + 1 (Simulator.cpp:86-88). */
+int vps_write_end_frame(const char* output_dir, int32_t frame);
+
+/* Simulator::writeAndSaveToFrameDirectory (Simulator.cpp:79-89): creates
+ * <output_dir>/<frame>, writes and saves the frame there, updates
+ * end_frame.txt. */
+int vps_write_and_save_frame(vps_context* ctx, const char* output_dir, int32_t frame);
+
+/* Simulator::loadFromFrameDirectory (Simulator.cpp:91-100): vps_load_frame
+ * of <output_dir>/<frame>; VPS_ERR_IO when the directory is missing. */
+int vps_load_frame_number(vps_context* ctx, const char* output_dir, int32_t frame);
 
 /* ---- read-back (tests, output) --------------------------------------- */
 

@@ -471,5 +471,10 @@ void mlp_destroy(vps_context* c);
 int mlp_warm_start(vps_context* c, int mode, float* x0);
 int mlp_after_solve(vps_context* c, const vps_stats* st);
 bool mlp_active(const vps_context* c);
+// checkpoint of the learned warm start: parameters, Adam moments and, for
+// slot map lb (nslots slots; lb < 0: none), the node history of the last solve
+int mlp_state_save(vps_context* c, int lb, int64_t nslots, std::vector<char>& out);
+int mlp_state_load(vps_context* c, int lb, int64_t nslots, const char* p, size_t n);
+
 
 }  // namespace vps

@@ -947,6 +947,91 @@ int mlp_after_solve(vps_context* c, const vps_stats* st) {
     return VPS_OK;
 }
 
+// Checkpoint: parameters, Adam moments and step, and (lb >= 0, history kept)
+// the slot-major history of the last solve with the scales of the next step.
+int mlp_state_save(vps_context* c, int lb, int64_t nslots, std::vector<char>& out) {
+    MlpState* m = S(c);
+    out.clear();
+    auto put = [&](const void* p, size_t n) {
+        out.insert(out.end(), (const char*)p, (const char*)p + n);
+    };
+    std::vector<float> buf(kNParams);
+    for (const DBuf<float>* b : {&m->P, &m->m, &m->v}) {
+        VPS_HIP(c, hipMemcpy(buf.data(), b->p, kNParams * 4, hipMemcpyDeviceToHost));
+        put(buf.data(), kNParams * 4);
+    }
+    put(&m->adam_t, 8);
+    const int32_t hist = (lb >= 0 && m->have_history && m->hp1.p) ? 1 : 0;
+    put(&hist, 4);
+    if (hist) {
+        const size_t nn = (size_t)nslots * c->brick_nodes;
+        std::vector<float> f(nn);
+        std::vector<uint8_t> h(nn);
+        std::vector<float> sc(SC_N);
+        VPS_HIP(c, hipMemcpy(f.data(), m->DP[lb].p, nn * 4, hipMemcpyDeviceToHost));
+        put(f.data(), nn * 4);
+        VPS_HIP(c, hipMemcpy(f.data(), m->DP2[lb].p, nn * 4, hipMemcpyDeviceToHost));
+        put(f.data(), nn * 4);
+        VPS_HIP(c, hipMemcpy(h.data(), m->H[lb].p, nn, hipMemcpyDeviceToHost));
+        put(h.data(), nn);
+        VPS_HIP(c, hipMemcpy(sc.data(), m->scale[m->cur_scale].p, SC_N * 4,
+                             hipMemcpyDeviceToHost));
+        put(sc.data(), SC_N * 4);
+        const int32_t hs = m->hist_solves;
+        put(&hs, 4);
+    }
+    return VPS_OK;
+}
+
+int mlp_state_load(vps_context* c, int lb, int64_t nslots, const char* p, size_t n) {
+    MlpState* m = S(c);
+    const char* end = p + n;
+    auto get = [&](void* dst, size_t k) {
+        if ((size_t)(end - p) < k) return false;
+        std::memcpy(dst, p, k);
+        p += k;
+        return true;
+    };
+    std::vector<float> buf(kNParams);
+    for (DBuf<float>* b : {&m->P, &m->m, &m->v}) {
+        if (!get(buf.data(), kNParams * 4)) {
+            c->err = "checkpoint: truncated learned parameters";
+            return VPS_ERR_IO;
+        }
+        VPS_HIP(c, hipMemcpy(b->p, buf.data(), kNParams * 4, hipMemcpyHostToDevice));
+    }
+    int32_t hist = 0;
+    if (!get(&m->adam_t, 8) || !get(&hist, 4)) {
+        c->err = "checkpoint: truncated learned state";
+        return VPS_ERR_IO;
+    }
+    m->have_history = false;
+    m->hist_solves = 0;
+    if (hist && lb >= 0) {
+        VPS_TRY(ensure_history(c));
+        const size_t nn = (size_t)nslots * c->brick_nodes;
+        std::vector<float> f(nn);
+        std::vector<uint8_t> h(nn);
+        std::vector<float> sc(SC_N);
+        int32_t hs = 0;
+        bool ok = get(f.data(), nn * 4);
+        if (ok) VPS_HIP(c, hipMemcpy(m->DP[lb].p, f.data(), nn * 4, hipMemcpyHostToDevice));
+        ok = ok && get(f.data(), nn * 4);
+        if (ok) VPS_HIP(c, hipMemcpy(m->DP2[lb].p, f.data(), nn * 4, hipMemcpyHostToDevice));
+        ok = ok && get(h.data(), nn) && get(sc.data(), SC_N * 4) && get(&hs, 4);
+        if (!ok) {
+            c->err = "checkpoint: truncated learned history";
+            return VPS_ERR_IO;
+        }
+        VPS_HIP(c, hipMemcpy(m->H[lb].p, h.data(), nn, hipMemcpyHostToDevice));
+        m->cur_scale = 0;
+        VPS_HIP(c, hipMemcpy(m->scale[0].p, sc.data(), SC_N * 4, hipMemcpyHostToDevice));
+        m->have_history = true;
+        m->hist_solves = hs;
+    }
+    return VPS_OK;
+}
+
 }  // namespace vps
 
 using namespace vps;

@@ -14,6 +14,7 @@ VPS_ERR_NONFINITE = 3
 VPS_ERR_OOM = 4
 VPS_ERR_HIP = 5
 VPS_ERR_STATE = 6
+VPS_ERR_IO = 7
 
 STATUS_NAMES = {
     0: "VPS_OK",
@@ -23,6 +24,7 @@ STATUS_NAMES = {
     4: "VPS_ERR_OOM",
     5: "VPS_ERR_HIP",
     6: "VPS_ERR_STATE",
+    7: "VPS_ERR_IO",
 }
 
 OP_TRANSPOSE = 0
@@ -195,6 +197,13 @@ def declare(lib):
     _proto(lib, "vps_dem_step", C.c_int,
            [_P, C.c_double, C.c_double, C.POINTER(DemParams)])
     _proto(lib, "vps_couple_grains", C.c_int, [_P, _P])
+    _proto(lib, "vps_write_frame", C.c_int, [_P, C.c_char_p])
+    _proto(lib, "vps_save_frame", C.c_int, [_P, C.c_char_p])
+    _proto(lib, "vps_load_frame", C.c_int, [_P, C.c_char_p])
+    _proto(lib, "vps_write_description", C.c_int, [C.c_int32, C.c_char_p])
+    _proto(lib, "vps_write_end_frame", C.c_int, [C.c_char_p, C.c_int32])
+    _proto(lib, "vps_write_and_save_frame", C.c_int, [_P, C.c_char_p, C.c_int32])
+    _proto(lib, "vps_load_frame_number", C.c_int, [_P, C.c_char_p, C.c_int32])
     _proto(lib, "vps_debug_apply_operator", C.c_int, [_P, _D, _D])
     return lib
 
@@ -212,6 +221,8 @@ EXPORTED = [
     "vps_mlp_get_parameters", "vps_mlp_set_parameters",
     "vps_mlp_loss_and_grad", "vps_mlp_forward", "vps_mlp_get_features",
     "vps_mlp_training_state", "vps_dem_step", "vps_couple_grains",
+    "vps_write_frame", "vps_save_frame", "vps_load_frame", "vps_write_description",
+    "vps_write_end_frame", "vps_write_and_save_frame", "vps_load_frame_number",
     "vps_debug_apply_operator",
 ]
 

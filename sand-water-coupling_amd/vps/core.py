@@ -247,6 +247,26 @@ class Projection:
         self._check(self.L.vps_mlp_training_state(self.h, C.byref(loss), C.byref(steps)))
         return loss.value, steps.value
 
+    # -- frames and checkpoints (reference file formats, see vps.frames)
+    def write_frame(self, frame_dir):
+        self._check(self.L.vps_write_frame(self.h, os.fsencode(frame_dir)))
+
+    def save_frame(self, frame_dir):
+        self._check(self.L.vps_save_frame(self.h, os.fsencode(frame_dir)))
+
+    def load_frame(self, frame_dir):
+        self._check(self.L.vps_load_frame(self.h, os.fsencode(frame_dir)))
+        self.n = os.path.getsize(os.path.join(frame_dir, "particles.sav")) // (8 * self.dim)
+
+    def write_and_save_frame(self, output_dir, frame):
+        """Simulator::writeAndSaveToFrameDirectory: <output_dir>/<frame>/."""
+        self._check(self.L.vps_write_and_save_frame(self.h, os.fsencode(output_dir), frame))
+
+    def load_frame_number(self, output_dir, frame):
+        self._check(self.L.vps_load_frame_number(self.h, os.fsencode(output_dir), frame))
+        self.n = os.path.getsize(os.path.join(output_dir, str(frame), "particles.sav")) // (
+            8 * self.dim)
+
     # -- grains as moving boundary particles
     def couple_grains(self, grains):
         """Boundary = this context's walls followed by the grains of a
@@ -270,3 +290,11 @@ class DemSand(Projection):
 
     def advance(self, dt):
         self._check(self.L.vps_dem_step(self.h, dt, self.gravity, C.byref(self.dem)))
+
+
+def write_description(dim, output_dir):
+    """description.yaml of an output directory (Simulator.cpp:33-39)."""
+    L = load_library()
+    rc = L.vps_write_description(dim, os.fsencode(output_dir))
+    if rc:
+        raise VpsError(rc, "cannot write %s/description.yaml" % output_dir)
