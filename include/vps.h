@@ -185,6 +185,45 @@ int vps_set_boundary(vps_context* ctx, int64_t n, const double* x,
  * re-initialisation (moving boundaries). */
 int vps_refresh_boundary(vps_context* ctx);
 
+/* Implicit surfaces of Cores/Geometries/ImplicitSurface.h:27-113. */
+enum vps_surface_kind {
+    VPS_SURFACE_BOX = 0,       /* ImplicitBox(min corner a, lengths b) */
+    VPS_SURFACE_SPHERE = 1,    /* ImplicitSphere(center a, radius b[0]) */
+    VPS_SURFACE_PLANE = 2,     /* ImplicitPlane(position a, direction b) */
+    VPS_SURFACE_ELLIPSOID = 3  /* ImplicitEllipsoid(semi-axes b; the
+                                  reference's inside test and closest point
+                                  ignore its center, kept as is) */
+};
+typedef struct vps_surface {
+    int32_t kind;         /* vps_surface_kind */
+    int32_t complement;   /* 1: ComplementarySurface of it (Surface.h:50-69) */
+    double a[3];
+    double b[3];
+} vps_surface;
+
+/* BoundaryParticles::addSurface (Cores/Structures/SmoothedParticles.cpp:
+ * 111-155): appends the lattice points of the region center +- half_lengths
+ * (Dim doubles each) that lie inside the surface (signed distance <= 0) to
+ * the walls, with the surface's closest normal and zero velocity, then
+ * refreshes the resident boundary as vps_set_boundary does.  The lattice
+ * is the reference's: hexagonal rows of spacing 2r in 2D, the face and node
+ * centres of a staggered grid of spacing 2 sqrt(2) r in 3D (r the particle
+ * radius), in its order.  Sampled on the GPU. */
+int vps_add_surface(vps_context* ctx, const vps_surface* surface, const double* center,
+                    const double* half_lengths);
+
+/* Shapes::generateBox without Poisson sampling (Cores/Utilities/Shapes.h:
+ * 32-58): the same lattice over center +- half_lengths, every point (a block
+ * of fluid particles).  *count is always written; x (count * dim doubles)
+ * is filled when not NULL and capacity >= count. */
+int vps_generate_box(vps_context* ctx, const double* center, const double* half_lengths,
+                     int64_t capacity, double* x, int64_t* count);
+
+/* The walls as given (vps_set_boundary, vps_add_surface): *count and, for
+ * non-NULL pointers, positions, normals and velocities (count * dim each). */
+int vps_get_boundary(vps_context* ctx, int64_t* count, double* x, double* normals,
+                     double* v);
+
 /* Real (fluid) particle positions and velocities, n*dim doubles each. */
 int vps_set_particles(vps_context* ctx, int64_t n, const double* x,
                       const double* v);

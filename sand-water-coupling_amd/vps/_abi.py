@@ -141,6 +141,25 @@ _F = C.POINTER(C.c_float)
 _I64 = C.POINTER(C.c_int64)
 
 
+SURFACE_BOX, SURFACE_SPHERE, SURFACE_PLANE, SURFACE_ELLIPSOID = 0, 1, 2, 3
+
+
+class Surface(C.Structure):
+    """vps_surface: kind, complement, a[3], b[3] (include/vps.h)."""
+    _fields_ = [("kind", C.c_int32), ("complement", C.c_int32),
+                ("a", C.c_double * 3), ("b", C.c_double * 3)]
+
+
+def surface(kind, a, b, complement=False):
+    s = Surface()
+    s.kind = kind
+    s.complement = 1 if complement else 0
+    for i in range(3):
+        s.a[i] = float(a[i]) if i < len(a) else 0.0
+        s.b[i] = float(b[i]) if i < len(b) else 0.0
+    return s
+
+
 class DemParams(C.Structure):
     """Contact constants of DEMParticle<Dim> (Cores/Structures/DEMParticle.cpp:
     15-60): Young's modulus (K_norm = young r), Poisson ratio
@@ -197,6 +216,9 @@ def declare(lib):
     _proto(lib, "vps_dem_step", C.c_int,
            [_P, C.c_double, C.c_double, C.POINTER(DemParams)])
     _proto(lib, "vps_couple_grains", C.c_int, [_P, _P])
+    _proto(lib, "vps_add_surface", C.c_int, [_P, C.POINTER(Surface), _D, _D])
+    _proto(lib, "vps_generate_box", C.c_int, [_P, _D, _D, C.c_int64, _D, _I64])
+    _proto(lib, "vps_get_boundary", C.c_int, [_P, _I64, _D, _D, _D])
     _proto(lib, "vps_write_frame", C.c_int, [_P, C.c_char_p])
     _proto(lib, "vps_save_frame", C.c_int, [_P, C.c_char_p])
     _proto(lib, "vps_load_frame", C.c_int, [_P, C.c_char_p])
@@ -221,6 +243,7 @@ EXPORTED = [
     "vps_mlp_get_parameters", "vps_mlp_set_parameters",
     "vps_mlp_loss_and_grad", "vps_mlp_forward", "vps_mlp_get_features",
     "vps_mlp_training_state", "vps_dem_step", "vps_couple_grains",
+    "vps_add_surface", "vps_generate_box", "vps_get_boundary",
     "vps_write_frame", "vps_save_frame", "vps_load_frame", "vps_write_description",
     "vps_write_end_frame", "vps_write_and_save_frame", "vps_load_frame_number",
     "vps_debug_apply_operator",

@@ -247,6 +247,34 @@ class Projection:
         self._check(self.L.vps_mlp_training_state(self.h, C.byref(loss), C.byref(steps)))
         return loss.value, steps.value
 
+    # -- wall and block sampling (BoundaryParticles::addSurface, Shapes::generateBox)
+    def add_surface(self, kind, a, b, center, half_lengths, complement=False):
+        s = _abi.surface(kind, a, b, complement)
+        c = np.ascontiguousarray(center, dtype=np.float64)
+        h = np.ascontiguousarray(half_lengths, dtype=np.float64)
+        self._check(self.L.vps_add_surface(self.h, C.byref(s), _d(c), _d(h)))
+        self.nb = self.boundary()[0].shape[0]
+
+    def generate_box(self, center, half_lengths):
+        c = np.ascontiguousarray(center, dtype=np.float64)
+        h = np.ascontiguousarray(half_lengths, dtype=np.float64)
+        cnt = C.c_int64()
+        self._check(self.L.vps_generate_box(self.h, _d(c), _d(h), 0, None, C.byref(cnt)))
+        x = np.zeros((cnt.value, self.dim))
+        self._check(self.L.vps_generate_box(self.h, _d(c), _d(h), cnt.value, _d(x),
+                                            C.byref(cnt)))
+        return x
+
+    def boundary(self):
+        """The walls as given: positions, normals, velocities."""
+        cnt = C.c_int64()
+        self._check(self.L.vps_get_boundary(self.h, C.byref(cnt), None, None, None))
+        x = np.zeros((cnt.value, self.dim))
+        nrm = np.zeros((cnt.value, self.dim))
+        v = np.zeros((cnt.value, self.dim))
+        self._check(self.L.vps_get_boundary(self.h, C.byref(cnt), _d(x), _d(nrm), _d(v)))
+        return x, nrm, v
+
     # -- frames and checkpoints (reference file formats, see vps.frames)
     def write_frame(self, frame_dir):
         self._check(self.L.vps_write_frame(self.h, os.fsencode(frame_dir)))
