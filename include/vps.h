@@ -298,6 +298,51 @@ int vps_dem_step(vps_context* grains, double dt, double gravity,
  * be on the same device. */
 int vps_couple_grains(vps_context* water, vps_context* grains);
 
+/* ---- the rest of the SPH step ------------------------------------------ */
+
+/* A static collider (StaticCollider, Cores/Geometries/Collider.h:58-78):
+ * a surface with restitution and friction coefficients (Collider.cpp:36-63;
+ * the reference's default is 0 and 0). */
+typedef struct vps_collider {
+    vps_surface surface;
+    double restitution;
+    double friction;
+} vps_collider;
+
+/* SmthParticleHydrodLiquid::moveParticles (Cores/Physics/
+ * SmthParticleHydrodLiquid.cpp:72-82): x += v dt; each collider resolves
+ * every particle in turn (signed distance - r <= 0: velocity reflected with
+ * restitution and friction, position moved to the closest surface point + r
+ * along the normal; Collider.cpp:13-63); then the neighbour rebuild and the
+ * real densities (vps_refresh_real). */
+int vps_move_particles(vps_context* ctx, double dt, const vps_collider* colliders,
+                       int32_t n_colliders);
+
+/* applyExternalForces (SmthParticleHydrodLiquid.cpp:84-88): v.y -= g dt. */
+int vps_apply_gravity(vps_context* ctx, double dt, double gravity);
+
+/* applyViscosityForce (SmthParticleHydrodLiquid.cpp:90-98) with
+ * divFreeLaplacianAtDataPoint (ParticlesBasedVectorField.cpp:56-69):
+ * v_i += nu dt 2 (Dim + 2) m sum_j ((v_j - v_i).d / |d|^2) gradK(d) / rho_j,
+ * d = x_j - x_i over the real neighbours within h (d = 0 skipped), all from
+ * the velocities before the pass.  Needs the current densities (after
+ * vps_move_particles or vps_refresh_real). */
+int vps_apply_viscosity(vps_context* ctx, double dt, double coefficient);
+
+/* DualParticleSphLiquid::advance (DualParticleSphLiquid.cpp:107-116):
+ * vps_move_particles, gravity when enabled (the reference's default is off,
+ * SmthParticleHydrodLiquid.h:25), viscosity (0.01 there, :28), virtual
+ * generation and the projection. */
+typedef struct vps_step_params {
+    double dt;
+    double gravity;                 /* 9.8 (Utilities/Constants.h:7) */
+    double viscosity;               /* nu; 0 skips the pass */
+    const vps_collider* colliders;  /* may be NULL when n_colliders = 0 */
+    int32_t n_colliders;
+    int32_t enable_gravity;
+} vps_step_params;
+int vps_advance(vps_context* ctx, const vps_step_params* step, vps_stats* stats);
+
 /* ---- frames and checkpoints ------------------------------------------ */
 
 /* DualParticleSphLiquid::writeFrame (Cores/Physics/DualParticleSphLiquid.cpp:

@@ -78,6 +78,11 @@ def lib():
     pr("or_projection_step", C.c_int, [P, C.POINTER(OrStats)])
     pr("or_advect", C.c_int, [P, C.c_double, C.c_double])
     pr("or_refresh_boundary", C.c_int, [P])
+    pr("or_move_particles", C.c_int, [P, C.c_double, C.POINTER(_abi.Collider), C.c_int32])
+    pr("or_apply_gravity", C.c_int, [P, C.c_double, C.c_double])
+    pr("or_apply_viscosity", C.c_int, [P, C.c_double, C.c_double])
+    pr("or_advance", C.c_int, [P, C.c_double, C.c_double, C.c_int32, C.c_double,
+                               C.POINTER(_abi.Collider), C.c_int32, C.POINTER(OrStats)])
     pr("or_dem_step", C.c_int, [P, C.c_double, C.c_double, C.c_double,
                                 C.c_double, C.c_double])
     pr("or_virtual_count", C.c_int64, [P])
@@ -199,6 +204,23 @@ class Oracle:
         _check(self.L.or_set_particles(self.h, len(x), _d(x), _d(v)),
                "or_set_particles")
         self.n = len(x)
+
+    def move_particles(self, dt, colliders=()):
+        arr, n = _abi.collider_array(colliders)
+        _check(self.L.or_move_particles(self.h, dt, arr, n), "or_move_particles")
+
+    def apply_gravity(self, dt, gravity=9.8):
+        _check(self.L.or_apply_gravity(self.h, dt, gravity), "or_apply_gravity")
+
+    def apply_viscosity(self, dt, coefficient=0.01):
+        _check(self.L.or_apply_viscosity(self.h, dt, coefficient), "or_apply_viscosity")
+
+    def advance(self, dt, gravity=9.8, viscosity=0.01, colliders=(), enable_gravity=False):
+        arr, n = _abi.collider_array(colliders)
+        st = OrStats()
+        _check(self.L.or_advance(self.h, dt, gravity, 1 if enable_gravity else 0, viscosity,
+                                 arr, n, C.byref(st)), "or_advance")
+        return st.as_dict()
 
     def dem_step(self, dt, gravity=9.8, young=1e9, poisson=0.3,
                  friction_angle=0.5):

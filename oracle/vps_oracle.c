@@ -1201,6 +1201,204 @@ int or_dem_step(or_sim* s, double dt, double gravity, double young,
     return 0;
 }
 
+/* ------------------------------------------------------------------------
+ * The rest of the SPH step (SmthParticleHydrodLiquid.cpp:59-98).
+ * Implicit surfaces of Cores/Geometries/ImplicitSurface.h:27-113 and
+ * ComplementarySurface (Surface.h:50-69): signed distance, closest normal
+ * and closest position, each in the reference's order of operations.
+ * ---------------------------------------------------------------------- */
+static void or_normalize(int dim, double* v) {
+    const double z = sqnorm(dim, v);
+    if (z > 0) {
+        const double n = sqrt(z);
+        for (int a = 0; a < dim; a++) v[a] = v[a] / n;
+    }
+}
+
+/* signed distance, normal and closest position of the surface itself */
+static double surf_inner(const vps_surface* s, int dim, const double* p, double* n,
+                         double* cp) {
+    n[0] = n[1] = n[2] = 0;
+    if (s->kind == VPS_SURFACE_BOX) {
+        double d[3] = {0, 0, 0}, phi[3] = {0, 0, 0};
+        int all_in = 1, ax = 0;
+        for (int a = 0; a < dim; a++) {
+            const double c = s->a[a] + s->b[a] / 2, h = s->b[a] / 2;
+            d[a] = p[a] - c;
+            phi[a] = fabs(d[a]) - h;
+            all_in = all_in && phi[a] <= 0;
+            if (a > 0 && phi[a] > phi[ax]) ax = a;
+        }
+        double sd;
+        if (all_in) {
+            sd = phi[ax];
+            n[ax] = 1;
+        } else {
+            double m[3] = {0, 0, 0};
+            for (int a = 0; a < dim; a++) m[a] = phi[a] > 0 ? phi[a] : 0;
+            sd = sqrt(sqnorm(dim, m));
+            for (int a = 0; a < dim; a++) n[a] = m[a];
+        }
+        for (int a = 0; a < dim; a++) n[a] = n[a] * (d[a] > 0 ? 1. : (d[a] < 0 ? -1. : 0.));
+        or_normalize(dim, n);
+        /* ImplicitSurface::closestPosition: pos - signedDistance * normal */
+        for (int a = 0; a < dim; a++) cp[a] = p[a] - sd * n[a];
+        return sd;
+    }
+    if (s->kind == VPS_SURFACE_SPHERE) {
+        for (int a = 0; a < dim; a++) n[a] = p[a] - s->a[a];
+        const double sd = sqrt(sqnorm(dim, n)) - s->b[0];
+        or_normalize(dim, n);
+        /* ImplicitSphere::closestPosition: center + normal * radius */
+        for (int a = 0; a < dim; a++) cp[a] = s->a[a] + n[a] * s->b[0];
+        return sd;
+    }
+    if (s->kind == VPS_SURFACE_PLANE) {
+        double u[3] = {s->b[0], s->b[1], s->b[2]}, d[3] = {0, 0, 0};
+        or_normalize(dim, u);
+        for (int a = 0; a < dim; a++) {
+            d[a] = p[a] - s->a[a];
+            n[a] = u[a];
+        }
+        const double sd = dot3(dim, d, u);
+        for (int a = 0; a < dim; a++) cp[a] = p[a] - sd * n[a];
+        return sd;
+    }
+    /* ellipsoid (measured from the origin, as the reference does) */
+    double q[3] = {0, 0, 0};
+    for (int a = 0; a < dim; a++) q[a] = p[a] / s->b[a];
+    const double s2 = sqnorm(dim, q);
+    const int in = (s2 - 1) <= 0;
+    const double sq = sqrt(s2);
+    for (int a = 0; a < dim; a++) {
+        cp[a] = p[a] / sq;
+        n[a] = p[a] - cp[a];
+    }
+    const double dist = sqrt(sqnorm(dim, n));
+    or_normalize(dim, n);
+    const double sg = in ? -1. : 1.;
+    for (int a = 0; a < dim; a++) n[a] = n[a] * sg;
+    return dist * sg;
+}
+
+static double surf_eval(const vps_surface* s, int dim, const double* p, double* n,
+                        double* cp) {
+    double sd = surf_inner(s, dim, p, n, cp);
+    if (s->complement) {
+        sd = -sd;
+        for (int a = 0; a < dim; a++) n[a] = -n[a];
+    }
+    return sd;
+}
+
+/* Collider::collide(pos, vel, radius) with a static collider
+ * (Collider.cpp:21-59): inside when signedDistance - radius <= 0 */
+static void collide_one(const vps_collider* c, int dim, double r, double* p, double* v) {
+    double n[3], cp[3];
+    const double sd = surf_eval(&c->surface, dim, p, n, cp);
+    if (!(sd - r <= 0)) return;
+    const double nd = dot3(dim, n, v);   /* colliderVel = 0 */
+    if (nd < 0) {
+        double rn[3] = {0, 0, 0}, rt[3] = {0, 0, 0}, drn[3] = {0, 0, 0};
+        for (int a = 0; a < dim; a++) {
+            rn[a] = nd * n[a];
+            rt[a] = v[a] - rn[a];
+            drn[a] = -(c->restitution + 1) * rn[a];
+            rn[a] = rn[a] * -c->restitution;
+        }
+        int any = 0;
+        for (int a = 0; a < dim; a++) any |= rt[a] != 0;
+        if (any) {
+            double f = 1 - c->friction * sqrt(sqnorm(dim, drn)) / sqrt(sqnorm(dim, rt));
+            f = f > 0 ? f : 0;
+            for (int a = 0; a < dim; a++) rt[a] = rt[a] * f;
+        }
+        for (int a = 0; a < dim; a++) v[a] = (rn[a] + rt[a]) + 0.0;
+    }
+    for (int a = 0; a < dim; a++) p[a] = cp[a] + r * n[a];
+}
+
+/* SmthParticleHydrodLiquid::moveParticles (:72-82): x += v dt, every
+ * collider over every particle, neighbour rebuild and densities */
+int or_move_particles(or_sim* s, double dt, const vps_collider* col, int32_t ncol) {
+    int64_t i;
+    const int dim = s->dim;
+    for (i = 0; i < s->real.n; i++)
+        for (int a = 0; a < dim; a++) s->real.x[3 * i + a] += s->v[3 * i + a] * dt;
+    for (int c = 0; c < ncol; c++)
+        for (i = 0; i < s->real.n; i++)
+            collide_one(&col[c], dim, s->p.particle_radius, s->real.x + 3 * i, s->v + 3 * i);
+    set_reset(&s->real);
+    set_compute_densities(&s->real);
+    return 0;
+}
+
+/* applyExternalForces (:84-88) */
+int or_apply_gravity(or_sim* s, double dt, double gravity) {
+    for (int64_t i = 0; i < s->real.n; i++) s->v[3 * i + 1] -= gravity * dt;
+    return 0;
+}
+
+typedef struct {
+    const or_sim* s;
+    const double* q;
+    const double* vi;
+    double lap[3];
+} visc_ctx;
+
+/* ParticlesBasedVectorField::divFreeLaplacianAtDataPoint
+ * (ParticlesBasedVectorField.cpp:56-69): deltaPos = x_j - x_i, skipped
+ * when zero */
+static void cb_visc(void* c, int32_t j, const double* xj) {
+    visc_ctx* t = (visc_ctx*)c;
+    const or_sim* s = t->s;
+    const int dim = s->dim;
+    double d[3] = {xj[0] - t->q[0], xj[1] - t->q[1], xj[2] - t->q[2]};
+    if (!(d[0] != 0 || d[1] != 0 || (dim == 3 && d[2] != 0))) return;
+    double dv[3] = {0, 0, 0}, g[3];
+    for (int a = 0; a < dim; a++) dv[a] = s->v[3 * j + a] - t->vi[a];
+    const double coef = dot3(dim, dv, d) / sqnorm(dim, d);
+    kern_grad(&s->real.k, d, g);
+    for (int a = 0; a < dim; a++) t->lap[a] += coef * g[a] / s->real.rho[j];
+}
+
+/* applyViscosityForce (:90-98): v_i += nu * lapl_i * dt on a copy */
+int or_apply_viscosity(or_sim* s, double dt, double coeff) {
+    if (coeff == 0) return 0;
+    const int dim = s->dim;
+    const int64_t n = s->real.n;
+    double* vnew = (double*)malloc((size_t)n * 3 * sizeof(double));
+    if (!vnew) return VPS_ERR_OOM;
+    memcpy(vnew, s->v, (size_t)n * 3 * sizeof(double));
+    int64_t i;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < n; i++) {
+        visc_ctx c = {s, s->real.x + 3 * i, s->v + 3 * i, {0, 0, 0}};
+        set_for_each(&s->real, c.q, cb_visc, &c);
+        for (int a = 0; a < dim; a++) {
+            const double lap = c.lap[a] * 2 * (dim + 2.0) * s->real.mass;
+            vnew[3 * i + a] += coeff * lap * dt;
+        }
+    }
+    memcpy(s->v, vnew, (size_t)n * 3 * sizeof(double));
+    free(vnew);
+    return 0;
+}
+
+/* DualParticleSphLiquid::advance (DualParticleSphLiquid.cpp:107-116) */
+int or_advance(or_sim* s, double dt, double gravity, int32_t enable_gravity,
+               double viscosity, const vps_collider* col, int32_t ncol, or_stats* st) {
+    if (!s->initialized) return VPS_ERR_STATE;
+    int rc = or_move_particles(s, dt, col, ncol);
+    if (rc) return rc;
+    if (enable_gravity) or_apply_gravity(s, dt, gravity);
+    rc = or_apply_viscosity(s, dt, viscosity);
+    if (rc) return rc;
+    rc = or_generate(s);
+    if (rc) return rc;
+    return or_project(s, st);
+}
+
 /* ---- read-back ---------------------------------------------------------- */
 int64_t or_virtual_count(or_sim* s) { return s->virt.n; }
 

@@ -52,6 +52,13 @@ int or_generate(or_sim* s);
 int or_project(or_sim* s, or_stats* st);
 int or_projection_step(or_sim* s, or_stats* st);
 int or_advect(or_sim* s, double dt, double gravity);
+/* the rest of the SPH step: move + colliders + densities, gravity,
+ * div-free viscosity, and the whole DualParticleSphLiquid::advance */
+int or_move_particles(or_sim* s, double dt, const vps_collider* col, int32_t ncol);
+int or_apply_gravity(or_sim* s, double dt, double gravity);
+int or_apply_viscosity(or_sim* s, double dt, double coeff);
+int or_advance(or_sim* s, double dt, double gravity, int32_t enable_gravity,
+               double viscosity, const vps_collider* col, int32_t ncol, or_stats* st);
 /* boundary densities and volumes of the current boundary set */
 int or_refresh_boundary(or_sim* s);
 /* one DEMParticleSand::advance of the grains held as real particles, the

@@ -169,7 +169,7 @@ void free_all(vps_context* c) {
     }
     dfree(c->rrho);
     dfree(c->rofs);
-    dfree(c->dem_vnew);
+    dfree(c->vel_next);
     dfree(c->grad_chunks);
     dfree(c->grad_over);
     dfree(c->couple_stage);
@@ -648,6 +648,72 @@ int vps_project(vps_context* c, vps_stats* st) {
         st->ms_cg = elapsed(c->ev[5], c->ev[6]);
         st->ms_gradient = elapsed(c->ev[6], c->ev[7]);
         st->ms_total = elapsed(c->ev[4], c->ev[7]);
+    }
+    c->generated = false;
+    return rc;
+}
+
+int vps_move_particles(vps_context* c, double dt, const vps_collider* colliders,
+                       int32_t n_colliders) {
+    if (!c) return VPS_ERR_INVALID_ARG;
+    if (!c->initialized) return fail(c, VPS_ERR_STATE, "not initialized");
+    if (!std::isfinite(dt)) return fail(c, VPS_ERR_INVALID_ARG, "move_particles: dt");
+    VPS_TRY(move_collide(c, dt, colliders, n_colliders));
+    VPS_TRY(cells_sort_real(c));
+    VPS_TRY(real_density(c, false));
+    c->generated = false;
+    VPS_HIP(c, hipStreamSynchronize(c->stream));
+    return VPS_OK;
+}
+
+int vps_apply_gravity(vps_context* c, double dt, double gravity) {
+    if (!c) return VPS_ERR_INVALID_ARG;
+    if (c->n == 0) return fail(c, VPS_ERR_STATE, "apply_gravity: no particles");
+    if (!std::isfinite(dt) || !std::isfinite(gravity))
+        return fail(c, VPS_ERR_INVALID_ARG, "apply_gravity: non-finite dt or gravity");
+    VPS_TRY(apply_gravity(c, dt, gravity));
+    VPS_HIP(c, hipStreamSynchronize(c->stream));
+    return VPS_OK;
+}
+
+int vps_apply_viscosity(vps_context* c, double dt, double coefficient) {
+    if (!c) return VPS_ERR_INVALID_ARG;
+    if (!c->initialized) return fail(c, VPS_ERR_STATE, "not initialized");
+    if (!std::isfinite(dt) || !std::isfinite(coefficient))
+        return fail(c, VPS_ERR_INVALID_ARG, "apply_viscosity: non-finite dt or coefficient");
+    VPS_TRY(apply_viscosity(c, dt, coefficient));
+    VPS_HIP(c, hipStreamSynchronize(c->stream));
+    return VPS_OK;
+}
+
+int vps_advance(vps_context* c, const vps_step_params* sp, vps_stats* st) {
+    if (!c || !sp) return VPS_ERR_INVALID_ARG;
+    if (!c->initialized) return fail(c, VPS_ERR_STATE, "not initialized");
+    if (!std::isfinite(sp->dt) || !std::isfinite(sp->gravity) ||
+        !std::isfinite(sp->viscosity))
+        return fail(c, VPS_ERR_INVALID_ARG, "advance: non-finite dt, gravity or viscosity");
+    if (st) std::memset(st, 0, sizeof(*st));
+    VPS_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    VPS_TRY(move_collide(c, sp->dt, sp->colliders, sp->n_colliders));
+    VPS_TRY(cells_sort_real(c));
+    VPS_TRY(real_density(c, false));
+    if (sp->enable_gravity) VPS_TRY(apply_gravity(c, sp->dt, sp->gravity));
+    VPS_TRY(apply_viscosity(c, sp->dt, sp->viscosity));
+    VPS_HIP(c, hipEventRecord(c->ev[1], c->stream));
+    VPS_TRY(virtual_generate(c));
+    c->generated = true;
+    VPS_HIP(c, hipEventRecord(c->ev[2], c->stream));
+    const int rc = project_impl(c, st, 4);
+    if (rc != VPS_OK && rc != VPS_ERR_NO_CONVERGENCE) return rc;
+    VPS_HIP(c, hipStreamSynchronize(c->stream));
+    if (st) {
+        fill_common_stats(c, st);
+        st->ms_sort = elapsed(c->ev[0], c->ev[1]);
+        st->ms_generate = elapsed(c->ev[1], c->ev[2]);
+        st->ms_diag = elapsed(c->ev[4], c->ev[5]);
+        st->ms_cg = elapsed(c->ev[5], c->ev[6]);
+        st->ms_gradient = elapsed(c->ev[6], c->ev[7]);
+        st->ms_total = elapsed(c->ev[0], c->ev[7]);
     }
     c->generated = false;
     return rc;

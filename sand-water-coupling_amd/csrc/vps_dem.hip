@@ -169,9 +169,9 @@ int dem_step(vps_context* c, double dt, double g, const vps_dem_params* dp) {
                            n, c->rpos[s].p, c->rvel[s].p, dt);
     VPS_TRY(cells_sort_real(c));
     s = c->cur;
-    if (c->dem_vnew.n < (size_t)n) {
-        dfree(c->dem_vnew);
-        VPS_TRY(dalloc(c, c->dem_vnew, (size_t)n));
+    if (c->vel_next.n < (size_t)n) {
+        dfree(c->vel_next);
+        VPS_TRY(dalloc(c, c->vel_next, (size_t)n));
     }
     const double r = c->p.particle_radius;
     DemK dk;
@@ -195,13 +195,13 @@ int dem_step(vps_context* c, double dt, double g, const vps_dem_params* dp) {
     if (D == 3)
         hipLaunchKernelGGL(k_dem_forces<3>, dim3(blocks(n, 256)), dim3(256), 0, c->stream,
                            c->k, dk, n, c->rpos[s].p, c->rvel[s].p, c->cell_start.p,
-                           bp, c->bvel.p, c->bnrm.p, bcs, c->dem_vnew.p);
+                           bp, c->bvel.p, c->bnrm.p, bcs, c->vel_next.p);
     else
         hipLaunchKernelGGL(k_dem_forces<2>, dim3(blocks(n, 256)), dim3(256), 0, c->stream,
                            c->k, dk, n, c->rpos[s].p, c->rvel[s].p, c->cell_start.p,
-                           bp, c->bvel.p, c->bnrm.p, bcs, c->dem_vnew.p);
+                           bp, c->bvel.p, c->bnrm.p, bcs, c->vel_next.p);
     VPS_HIP(c, hipGetLastError());
-    VPS_HIP(c, hipMemcpyAsync(c->rvel[s].p, c->dem_vnew.p, (size_t)n * sizeof(float4),
+    VPS_HIP(c, hipMemcpyAsync(c->rvel[s].p, c->vel_next.p, (size_t)n * sizeof(float4),
                               hipMemcpyDeviceToDevice, c->stream));
     c->generated = false;
     return VPS_OK;

@@ -330,7 +330,7 @@ struct vps_context {
     int grad_span_cells = 0;            // cells per span (0: global-memory gradient)
     int grad_nspan = 0;                 // spans per cell row
     int64_t grad_nchunks = 0;           // non-empty spans of the last generation
-    vps::DBuf<float4> dem_vnew;         // DEM step: velocities after the forces
+    vps::DBuf<float4> vel_next;         // velocities after a Jacobi pass (DEM, viscosity)
     vps::DBuf<double> couple_stage;     // grain coupling: boundary x, n, v
     int64_t couple_walls = -1;          // walls already staged there (-1: none)
     vps::DBuf<int32_t> cell_of, slot_of, perm;
@@ -471,6 +471,10 @@ void mlp_destroy(vps_context* c);
 int mlp_warm_start(vps_context* c, int mode, float* x0);
 int mlp_after_solve(vps_context* c, const vps_stats* st);
 bool mlp_active(const vps_context* c);
+// the rest of the SPH step (vps_sph.hip)
+int move_collide(vps_context* c, double dt, const vps_collider* col, int32_t ncol);
+int apply_gravity(vps_context* c, double dt, double g);
+int apply_viscosity(vps_context* c, double dt, double nu);
 // checkpoint of the learned warm start: parameters, Adam moments and, for
 // slot map lb (nslots slots; lb < 0: none), the node history of the last solve
 int mlp_state_save(vps_context* c, int lb, int64_t nslots, std::vector<char>& out);

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "vps_internal.h"
+#include "vps_surface.h"
 
 namespace vps {
 namespace {
@@ -41,12 +42,6 @@ struct Lattice {
     int64_t size[4][3];
     double org[4][3];
     int64_t start[5];   // first candidate of each grid; start[4] = total
-};
-
-struct SurfK {
-    int kind, complement;
-    double a[3], b[3];
-    double pn[3];   // plane: normalised direction
 };
 
 double std_round(double x) { return std::round(x); }   // halfway away from zero
@@ -114,84 +109,6 @@ __device__ void candidate(const Lattice& L, int64_t idx, double p[3]) {
     for (int a = 0; a < 3; a++) p[a] = L.org[g][a] + (double)c[a] * L.spacing;
 }
 
-__device__ double sqnorm(int dim, const double* v) {
-#pragma clang fp contract(off)
-    double s = v[0] * v[0] + v[1] * v[1];
-    if (dim == 3) s = s + v[2] * v[2];
-    return s;
-}
-
-__device__ void normalize(int dim, double* v) {
-#pragma clang fp contract(off)
-    const double z = sqnorm(dim, v);
-    if (z > 0) {
-        const double n = sqrt(z);
-        for (int a = 0; a < dim; a++) v[a] = v[a] / n;
-    }
-}
-
-// signed distance and closest normal of the surface itself
-__device__ double inner(const SurfK& s, int dim, const double* p, double* n) {
-#pragma clang fp contract(off)
-    n[0] = n[1] = n[2] = 0.0;
-    if (s.kind == VPS_SURFACE_BOX) {
-        double d[3], phi[3];
-        bool all_in = true;
-        int ax = 0;
-        for (int a = 0; a < dim; a++) {
-            const double c = s.a[a] + s.b[a] / 2, h = s.b[a] / 2;
-            d[a] = p[a] - c;
-            phi[a] = fabs(d[a]) - h;
-            all_in = all_in && phi[a] <= 0;
-            if (a > 0 && phi[a] > phi[ax]) ax = a;   // first maximum
-        }
-        double sd;
-        if (all_in) {
-            sd = phi[ax];
-            n[ax] = 1.0;
-        } else {
-            double m[3] = {0, 0, 0};
-            for (int a = 0; a < dim; a++) m[a] = phi[a] > 0 ? phi[a] : 0.0;
-            sd = sqrt(sqnorm(dim, m));
-            for (int a = 0; a < dim; a++) n[a] = m[a];
-        }
-        for (int a = 0; a < dim; a++) {
-            const double sg = d[a] > 0 ? 1.0 : (d[a] < 0 ? -1.0 : 0.0);
-            n[a] = n[a] * sg;
-        }
-        normalize(dim, n);
-        return sd;
-    }
-    if (s.kind == VPS_SURFACE_SPHERE) {
-        for (int a = 0; a < dim; a++) n[a] = p[a] - s.a[a];
-        const double sd = sqrt(sqnorm(dim, n)) - s.b[0];
-        normalize(dim, n);
-        return sd;
-    }
-    if (s.kind == VPS_SURFACE_PLANE) {
-        double d[3];
-        for (int a = 0; a < dim; a++) {
-            d[a] = p[a] - s.a[a];
-            n[a] = s.pn[a];
-        }
-        double sd = d[0] * s.pn[0] + d[1] * s.pn[1];
-        if (dim == 3) sd = sd + d[2] * s.pn[2];
-        return sd;
-    }
-    // ellipsoid (the reference measures from the origin, not its center)
-    double q[3] = {0, 0, 0};
-    for (int a = 0; a < dim; a++) q[a] = p[a] / s.b[a];
-    const double s2 = sqnorm(dim, q);
-    const bool in = (s2 - 1) <= 0;
-    const double sq = sqrt(s2);
-    for (int a = 0; a < dim; a++) n[a] = p[a] - p[a] / sq;
-    const double dist = sqrt(sqnorm(dim, n));
-    normalize(dim, n);
-    const double sg = in ? -1.0 : 1.0;
-    for (int a = 0; a < dim; a++) n[a] = n[a] * sg;
-    return dist * sg;
-}
-
 __global__ void __launch_bounds__(256) k_sample(Lattice L, SurfK s, int use_surface,
                                                 int32_t* flag, double* pos, double* nrm) {
 #pragma clang fp contract(off)
@@ -202,15 +119,13 @@ __global__ void __launch_bounds__(256) k_sample(Lattice L, SurfK s, int use_surf
     candidate(L, idx, p);
     bool in = true;
     if (use_surface) {
-        double sd = inner(s, dim, p, n);
-        if (s.complement) {
-            sd = -sd;
-            for (int a = 0; a < dim; a++) n[a] = -n[a];
-            in = sd <= 0;
-        } else if (s.kind == VPS_SURFACE_ELLIPSOID) {
+        double cp[3];
+        const double sd = surf_eval(s, dim, p, n, cp);
+        if (!s.complement && s.kind == VPS_SURFACE_ELLIPSOID) {
+            // ImplicitEllipsoid::isInside: sum (p / semi)^2 - 1 <= 0
             double q[3] = {0, 0, 0};
             for (int a = 0; a < dim; a++) q[a] = p[a] / s.b[a];
-            in = (sqnorm(dim, q) - 1) <= 0;
+            in = (surf_sqnorm(dim, q) - 1) <= 0;
         } else {
             in = sd <= 0;
         }
@@ -306,6 +221,41 @@ int check_region(vps_context* c, double radius, const double* center, const doub
 }
 
 }  // namespace
+
+bool make_surfk(const vps_surface& in, int D, SurfK* out, std::string* why) {
+#pragma clang fp contract(off)
+    if (in.kind < VPS_SURFACE_BOX || in.kind > VPS_SURFACE_ELLIPSOID ||
+        !finite_vec(in.a, 3) || !finite_vec(in.b, 3)) {
+        *why = "unknown surface kind or non-finite parameters";
+        return false;
+    }
+    SurfK s{};
+    s.kind = in.kind;
+    s.complement = in.complement ? 1 : 0;
+    for (int a = 0; a < 3; a++) {
+        s.a[a] = a < D ? in.a[a] : 0.0;
+        s.b[a] = a < D ? in.b[a] : 0.0;
+    }
+    if (s.kind == VPS_SURFACE_PLANE) {   // ImplicitPlane normalises its direction
+        double z = s.b[0] * s.b[0] + s.b[1] * s.b[1];
+        if (D == 3) z = z + s.b[2] * s.b[2];
+        if (!(z > 0)) {
+            *why = "zero plane direction";
+            return false;
+        }
+        const double nz = std::sqrt(z);
+        for (int a = 0; a < D; a++) s.pn[a] = s.b[a] / nz;
+    }
+    if (s.kind == VPS_SURFACE_ELLIPSOID)
+        for (int a = 0; a < D; a++)
+            if (!(s.b[a] != 0)) {
+                *why = "zero ellipsoid semi-axis";
+                return false;
+            }
+    *out = s;
+    return true;
+}
+
 }  // namespace vps
 
 using namespace vps;
@@ -318,35 +268,12 @@ int vps_add_surface(vps_context* c, const vps_surface* surface, const double* ce
     const int D = c->dim;
     const double r = c->p.particle_radius;
     VPS_TRY(check_region(c, r, center, half_lengths));
-    if (surface->kind < VPS_SURFACE_BOX || surface->kind > VPS_SURFACE_ELLIPSOID ||
-        !finite_vec(surface->a, 3) || !finite_vec(surface->b, 3)) {
-        c->err = "add_surface: unknown kind or non-finite parameters";
+    SurfK s{};
+    std::string why;
+    if (!make_surfk(*surface, D, &s, &why)) {
+        c->err = "add_surface: " + why;
         return VPS_ERR_INVALID_ARG;
     }
-    SurfK s{};
-    s.kind = surface->kind;
-    s.complement = surface->complement ? 1 : 0;
-    for (int a = 0; a < 3; a++) {
-        s.a[a] = a < D ? surface->a[a] : 0.0;
-        s.b[a] = a < D ? surface->b[a] : 0.0;
-    }
-    if (s.kind == VPS_SURFACE_PLANE) {   // ImplicitPlane normalises its direction
-#pragma clang fp contract(off)
-        double z = s.b[0] * s.b[0] + s.b[1] * s.b[1];
-        if (D == 3) z = z + s.b[2] * s.b[2];
-        if (!(z > 0)) {
-            c->err = "add_surface: zero plane direction";
-            return VPS_ERR_INVALID_ARG;
-        }
-        const double nz = std::sqrt(z);
-        for (int a = 0; a < D; a++) s.pn[a] = s.b[a] / nz;
-    }
-    if (s.kind == VPS_SURFACE_ELLIPSOID)
-        for (int a = 0; a < D; a++)
-            if (!(s.b[a] != 0)) {
-                c->err = "add_surface: zero ellipsoid semi-axis";
-                return VPS_ERR_INVALID_ARG;
-            }
     const Lattice L = make_lattice(D, r, center, half_lengths);
     std::vector<double> x, n;
     VPS_TRY(sample(c, L, &s, x, n));

@@ -160,6 +160,31 @@ def surface(kind, a, b, complement=False):
     return s
 
 
+class Collider(C.Structure):
+    """vps_collider: a static collider's surface, restitution, friction."""
+    _fields_ = [("surface", Surface), ("restitution", C.c_double),
+                ("friction", C.c_double)]
+
+
+def collider(kind, a, b, complement=False, restitution=0.0, friction=0.0):
+    return Collider(surface(kind, a, b, complement), restitution, friction)
+
+
+def collider_array(colliders):
+    colliders = list(colliders)
+    arr = (Collider * max(len(colliders), 1))()
+    for i, c in enumerate(colliders):
+        arr[i] = c
+    return arr, len(colliders)
+
+
+class StepParams(C.Structure):
+    """vps_step_params (DualParticleSphLiquid::advance)."""
+    _fields_ = [("dt", C.c_double), ("gravity", C.c_double), ("viscosity", C.c_double),
+                ("colliders", C.POINTER(Collider)), ("n_colliders", C.c_int32),
+                ("enable_gravity", C.c_int32)]
+
+
 class DemParams(C.Structure):
     """Contact constants of DEMParticle<Dim> (Cores/Structures/DEMParticle.cpp:
     15-60): Young's modulus (K_norm = young r), Poisson ratio
@@ -216,6 +241,10 @@ def declare(lib):
     _proto(lib, "vps_dem_step", C.c_int,
            [_P, C.c_double, C.c_double, C.POINTER(DemParams)])
     _proto(lib, "vps_couple_grains", C.c_int, [_P, _P])
+    _proto(lib, "vps_move_particles", C.c_int, [_P, C.c_double, C.POINTER(Collider), C.c_int32])
+    _proto(lib, "vps_apply_gravity", C.c_int, [_P, C.c_double, C.c_double])
+    _proto(lib, "vps_apply_viscosity", C.c_int, [_P, C.c_double, C.c_double])
+    _proto(lib, "vps_advance", C.c_int, [_P, C.POINTER(StepParams), C.POINTER(Stats)])
     _proto(lib, "vps_add_surface", C.c_int, [_P, C.POINTER(Surface), _D, _D])
     _proto(lib, "vps_generate_box", C.c_int, [_P, _D, _D, C.c_int64, _D, _I64])
     _proto(lib, "vps_get_boundary", C.c_int, [_P, _I64, _D, _D, _D])
@@ -244,6 +273,7 @@ EXPORTED = [
     "vps_mlp_loss_and_grad", "vps_mlp_forward", "vps_mlp_get_features",
     "vps_mlp_training_state", "vps_dem_step", "vps_couple_grains",
     "vps_add_surface", "vps_generate_box", "vps_get_boundary",
+    "vps_move_particles", "vps_apply_gravity", "vps_apply_viscosity", "vps_advance",
     "vps_write_frame", "vps_save_frame", "vps_load_frame", "vps_write_description",
     "vps_write_end_frame", "vps_write_and_save_frame", "vps_load_frame_number",
     "vps_debug_apply_operator",

@@ -247,6 +247,26 @@ class Projection:
         self._check(self.L.vps_mlp_training_state(self.h, C.byref(loss), C.byref(steps)))
         return loss.value, steps.value
 
+    # -- the rest of the SPH step (SmthParticleHydrodLiquid / DualParticleSphLiquid)
+    def move_particles(self, dt, colliders=()):
+        arr, n = _abi.collider_array(colliders)
+        self._check(self.L.vps_move_particles(self.h, dt, arr, n))
+
+    def apply_gravity(self, dt, gravity=9.8):
+        self._check(self.L.vps_apply_gravity(self.h, dt, gravity))
+
+    def apply_viscosity(self, dt, coefficient=0.01):
+        self._check(self.L.vps_apply_viscosity(self.h, dt, coefficient))
+
+    def advance(self, dt, gravity=9.8, viscosity=0.01, colliders=(), enable_gravity=False):
+        """DualParticleSphLiquid::advance: move (+ colliders), gravity when
+        enabled, viscosity, generation, projection."""
+        arr, n = _abi.collider_array(colliders)
+        sp = _abi.StepParams(dt, gravity, viscosity, arr, n, 1 if enable_gravity else 0)
+        st = _abi.Stats()
+        self._check(self.L.vps_advance(self.h, C.byref(sp), C.byref(st)))
+        return st.as_dict()
+
     # -- wall and block sampling (BoundaryParticles::addSurface, Shapes::generateBox)
     def add_surface(self, kind, a, b, center, half_lengths, complement=False):
         s = _abi.surface(kind, a, b, complement)
