@@ -113,6 +113,85 @@ private:
     std::vector<double> scratch_;
 };
 
+// DualParticleSphLiquid<Dim> with its step on the GPU
+// (Cores/Physics/DualParticleSphLiquid.h:13-60): walls from addSurface,
+// particles from the caller or generateBox, advance() the whole step
+// (DualParticleSphLiquid.cpp:107-116), frames in the reference's formats.
+template <int Dim>
+class DualParticleSphLiquidGpu : public VirtualParticleGpu<Dim> {
+    using Base = VirtualParticleGpu<Dim>;
+
+public:
+    explicit DualParticleSphLiquidGpu(const vps_params& params) : Base(params) {}
+
+    // BoundaryParticles::addSurface (SmoothedParticles.cpp:111-155)
+    void addSurface(const vps_surface& s, const double* center, const double* halfLengths) {
+        check(vps_add_surface(this->handle(), &s, center, halfLengths), this->handle(),
+              "vps_add_surface");
+    }
+
+    // Shapes::generateBox without Poisson sampling (Shapes.h:32-58)
+    std::vector<double> generateBox(const double* center, const double* halfLengths) {
+        int64_t n = 0;
+        check(vps_generate_box(this->handle(), center, halfLengths, 0, nullptr, &n),
+              this->handle(), "vps_generate_box");
+        std::vector<double> x((std::size_t)n * Dim);
+        check(vps_generate_box(this->handle(), center, halfLengths, n, x.data(), &n),
+              this->handle(), "vps_generate_box");
+        return x;
+    }
+
+    // DualParticleSphLiquid::advance: move + colliders, gravity, viscosity,
+    // generateParticles, project
+    vps_stats advance(const vps_step_params& step) {
+        vps_stats st{};
+        check(vps_advance(this->handle(), &step, &st), this->handle(), "vps_advance");
+        return st;
+    }
+
+    // writeFrame / saveFrame / loadFrame (DualParticleSphLiquid.cpp:41-76;
+    // SmthParticleHydrodLiquid.cpp:34-55)
+    void writeFrame(const std::string& dir) {
+        check(vps_write_frame(this->handle(), dir.c_str()), this->handle(), "vps_write_frame");
+    }
+    void saveFrame(const std::string& dir) {
+        check(vps_save_frame(this->handle(), dir.c_str()), this->handle(), "vps_save_frame");
+    }
+    void loadFrame(const std::string& dir) {
+        check(vps_load_frame(this->handle(), dir.c_str()), this->handle(), "vps_load_frame");
+    }
+};
+
+// DEMParticleSand<Dim> (Cores/Physics/DEMParticleSand.h:13-58): the grains
+// are the context's particles, the walls its boundary.
+template <int Dim>
+class DEMParticleSandGpu : public VirtualParticleGpu<Dim> {
+    using Base = VirtualParticleGpu<Dim>;
+
+public:
+    DEMParticleSandGpu(const vps_params& params, const vps_dem_params& dem)
+        : Base(params), dem_(dem) {}
+
+    void setParticles(std::size_t n, const double* x, const double* v) {
+        check(vps_set_particles(this->handle(), (int64_t)n, x, v), this->handle(),
+              "vps_set_particles");
+    }
+
+    // DEMParticleSand::advance (DEMParticleSand.cpp:90-152)
+    void advance(double dt, double gravity = 9.8) {
+        check(vps_dem_step(this->handle(), dt, gravity, &dem_), this->handle(), "vps_dem_step");
+    }
+
+    // the grains as moving walls of a liquid context
+    void coupleInto(VirtualParticleGpu<Dim>& water) {
+        check(vps_couple_grains(water.handle(), this->handle()), water.handle(),
+              "vps_couple_grains");
+    }
+
+private:
+    vps_dem_params dem_;
+};
+
 }  // namespace vps
 
 #endif  // VPS_VIRTUAL_PARTICLE_HPP
