@@ -626,12 +626,15 @@ static int project_impl(vps_context* c, vps_stats* st, int e0) {
     vps_stats* s = st ? st : &local;
     const int rc = cg_solve(c, s);
     if (rc != VPS_OK && rc != VPS_ERR_NO_CONVERGENCE) return rc;
-    // history and training read this solve's pressures before the slot map
-    // changes, so they run before after_solve flips the buffers
-    if (mlp_active(c) && rc == VPS_OK) VPS_TRY(mlp_after_solve(c, s));
+    // The network's history and training read this solve's pressures in the
+    // slot map of this solve (index `which`), which after_solve keeps; they
+    // touch neither the particles nor V p, so they are queued behind the
+    // gradient, which then starts without waiting for the host to launch
+    // the short history kernels.
     VPS_TRY(after_solve(c));
     VPS_HIP(c, hipEventRecord(c->ev[e0 + 2], c->stream));
     if (rc == VPS_OK) VPS_TRY(pressure_gradient(c));
+    if (mlp_active(c) && rc == VPS_OK) VPS_TRY(mlp_after_solve(c, s));
     VPS_HIP(c, hipEventRecord(c->ev[e0 + 3], c->stream));
     return rc;
 }

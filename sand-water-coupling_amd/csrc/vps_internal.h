@@ -228,6 +228,32 @@ __device__ __forceinline__ T wave_sum(T v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+// FP64 wave sum without the LDS crossbar, for call sites where all 64 lanes
+// are active: DPP butterflies within each row of 16 lanes (xor 1, xor 2,
+// half-row mirror, row mirror leave the row sum in every lane of the row),
+// then the four row sums read as scalars and added in a fixed order.  Every
+// lane returns the same value.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)u, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+__device__ __forceinline__ double wave_sum_all(double v) {
+    v += dpp_f64<0xb1>(v);    // quad_perm [1, 0, 3, 2]
+    v += dpp_f64<0x4e>(v);    // quad_perm [2, 3, 0, 1]
+    v += dpp_f64<0x141>(v);   // row_half_mirror
+    v += dpp_f64<0x140>(v);   // row_mirror
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) +
+           (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
 #pragma unroll

@@ -73,6 +73,13 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ short f2bf(float f) {
     return __builtin_bit_cast(short, (__bf16)f);
 }
+typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
+// four values rounded to BF16 with two v_cvt_pk_bf16_f32 (same rounding as
+// f2bf)
+__device__ __forceinline__ bf16x4 f2bf4(float a, float b, float c, float d) {
+    const f32x4 v = {a, b, c, d};
+    return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bf16x4v));
+}
 __device__ __forceinline__ float bf2f(short b) {
     return __uint_as_float(((uint32_t)(uint16_t)b) << 16);
 }
@@ -80,10 +87,14 @@ __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 // GELU in its tanh form: gelu(z) = 0.5 z (1 + tanh(u)) = z sigmoid(2u),
-// u = sqrt(2/pi) (z + 0.044715 z^3); one exp and one reciprocal per value.
+// u = sqrt(2/pi) (z + 0.044715 z^3).  sigmoid(2u) = 1 / (1 + 2^(-2u log2 e)):
+// one v_exp_f32 and one v_rcp_f32 per value (the network's values are
+// rounded to BF16, far above their 1-ulp errors).  Large |u| saturates to
+// 0 or 1: 2^+big = inf and 1 / inf = 0.
 __device__ __forceinline__ float gelu_sig(float z) {
-    const float u2 = 1.5957691216057308f * z * (1.f + 0.044715f * z * z);   // 2u
-    return __fdividef(1.f, 1.f + __expf(-u2));
+    // 2 sqrt(2/pi) log2(e)
+    const float u2l = 2.3022081985f * z * (1.f + 0.044715f * z * z);
+    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-u2l));
 }
 __device__ __forceinline__ float gelu(float z) { return z * gelu_sig(z); }
 // d/dz [z s(2u)] = s + z s (1 - s) 2u'(z)
@@ -148,8 +159,8 @@ __device__ __forceinline__ void forward(const Wregs& W, const uint16_t* sW2,
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) {
         F.z1[mt] = mfma(W.w1[mt], xb, W.b1[mt]);
-#pragma unroll
-        for (int i = 0; i < 4; i++) F.h1[mt][i] = f2bf(gelu(F.z1[mt][i]));
+        F.h1[mt] = f2bf4(gelu(F.z1[mt][0]), gelu(F.z1[mt][1]), gelu(F.z1[mt][2]),
+                         gelu(F.z1[mt][3]));
     }
 #pragma unroll
     for (int mo = 0; mo < 4; mo++) {
@@ -160,8 +171,7 @@ __device__ __forceinline__ void forward(const Wregs& W, const uint16_t* sW2,
             else acc = mfma(W.w2[mo][kt], F.h1[kt], acc);
         }
         F.z2[mo] = acc;
-#pragma unroll
-        for (int i = 0; i < 4; i++) F.h2[mo][i] = f2bf(gelu(acc[i]));
+        F.h2[mo] = f2bf4(gelu(acc[0]), gelu(acc[1]), gelu(acc[2]), gelu(acc[3]));
     }
     f32x4 z3 = {W.b3, W.b3, W.b3, W.b3};
 #pragma unroll
@@ -336,9 +346,8 @@ __global__ void __launch_bounds__(256) k_mlp_infer_pipe(const float* P, FeatSrc 
             } else {
                 f[0] = f[1] = f[2] = f[3] = 0.f;
             }
-            bf16x4 xb;
-#pragma unroll
-            for (int j = 0; j < 4; j++) xb[j] = f2bf(ok ? f[j] : 0.f);
+            const bf16x4 xb = f2bf4(ok ? f[0] : 0.f, ok ? f[1] : 0.f, ok ? f[2] : 0.f,
+                                    ok ? f[3] : 0.f);
             Fwd F;
             forward<false>(W, nullptr, xb, F);
             // lanes 0..15 hold node r's output and (q = 0) its raw history

@@ -134,17 +134,17 @@ struct StencilArgs {
 };
 
 // Sum of n per-block partials, identical in every block and lane: lane l
-// adds partials l, l + 64, ... in order, then a fixed xor tree.
+// adds partials l, l + 64, ... in order, then a fixed DPP tree.
 __device__ __forceinline__ double sum_partials64(const double* p, int n) {
     const int lane = threadIdx.x & 63;
     double s = 0;
     for (int i = lane; i < n; i += 64) s += p[i];
-    return wave_sum(s);
+    return wave_sum_all(s);
 }
 
 template <int BS>
 __device__ __forceinline__ double block_sum(double v, double* sbuf) {
-    v = wave_sum(v);
+    v = wave_sum_all(v);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     __syncthreads();
     if (lane == 0) sbuf[wv] = v;
@@ -155,30 +155,35 @@ __device__ __forceinline__ double block_sum(double v, double* sbuf) {
     return t;
 }
 
-// Plane sums P_k, k = 0..3, at the LDS point pl (row stride kT).
+// Plane sums P_k, k = 0..3, at the LDS point pl (row stride RS).  The
+// in-plane points are first summed by squared radius m = a^2 + b^2 (the nine
+// values 0, 1, 2, 4, 5, 8, 9, 10, 13 of the disk m < 16), then
+// P_k = sum_m cc[m + k^2] U_m over m + k^2 < 16 (k^2 = 0 excluded).
 template <int RS, typename T, typename C>
 __device__ __forceinline__ void plane4(const T* pl, const C& cc, T P[4]) {
-    P[0] = P[1] = P[2] = P[3] = T(0);
-#pragma unroll
-    for (int a = 0; a <= 3; a++) {
-#pragma unroll
-        for (int b = 0; b <= 3; b++) {
-            if (a * a + b * b > 15) continue;
-            T u;
-            if (a == 0 && b == 0) u = pl[0];
-            else if (a == 0) u = pl[b * RS] + pl[-b * RS];
-            else if (b == 0) u = pl[a] + pl[-a];
-            else
-                u = (pl[a + b * RS] + pl[-a + b * RS]) +
-                    (pl[a - b * RS] + pl[-a - b * RS]);
-#pragma unroll
-            for (int kk = 0; kk <= 3; kk++) {
-                const int k2 = a * a + b * b + kk * kk;
-                if (k2 >= 16 || k2 == 0) continue;
-                P[kk] += (T)cc.c[k2] * u;
-            }
-        }
-    }
+    auto at = [&](int a, int b) { return pl[a + b * RS]; };
+    // the 4 points (+-a, 0), (0, +-a)
+    auto axis4 = [&](int a) { return (at(a, 0) + at(-a, 0)) + (at(0, a) + at(0, -a)); };
+    // the 4 points (+-a, +-a)
+    auto diag4 = [&](int a) { return (at(a, a) + at(-a, a)) + (at(a, -a) + at(-a, -a)); };
+    // the 8 points (+-a, +-b), (+-b, +-a), a != b
+    auto ring8 = [&](int a, int b) {
+        return ((at(a, b) + at(-a, b)) + (at(a, -b) + at(-a, -b))) +
+               ((at(b, a) + at(-b, a)) + (at(b, -a) + at(-b, -a)));
+    };
+    const T U0 = pl[0], U1 = axis4(1), U2 = diag4(1), U4 = axis4(2), U5 = ring8(1, 2),
+            U8 = diag4(2), U9 = axis4(3), U10 = ring8(1, 3), U13 = ring8(2, 3);
+    const T c1 = (T)cc.c[1], c2 = (T)cc.c[2], c3 = (T)cc.c[3], c4 = (T)cc.c[4],
+            c5 = (T)cc.c[5], c6 = (T)cc.c[6], c8 = (T)cc.c[8], c9 = (T)cc.c[9],
+            c10 = (T)cc.c[10], c11 = (T)cc.c[11], c12 = (T)cc.c[12], c13 = (T)cc.c[13],
+            c14 = (T)cc.c[14];
+    P[0] = c1 * U1 + c2 * U2 + c4 * U4 + c5 * U5 + c8 * U8 + c9 * U9 + c10 * U10 +
+           c13 * U13;
+    P[1] = c1 * U0 + c2 * U1 + c3 * U2 + c5 * U4 + c6 * U5 + c9 * U8 + c10 * U9 +
+           c11 * U10 + c14 * U13;
+    P[2] = c4 * U0 + c5 * U1 + c6 * U2 + c8 * U4 + c9 * U5 + c12 * U8 + c13 * U9 +
+           c14 * U10;
+    P[3] = c9 * U0 + c10 * U1 + c11 * U2 + c13 * U4 + c14 * U5;
 }
 
 // 2D: in-plane sum only.
@@ -261,6 +266,7 @@ constexpr int kRS = 24;                     // tile row stride (floats)
 constexpr int kPS = kRS * kT;               // tile plane stride
 constexpr int kUnits = kT * kT * 4;         // 16-byte loads per tile: 784
 constexpr int kUnitIters = (kUnits + kS3Threads - 1) / kS3Threads;   // 4
+static_assert(kS3Threads * (kUnitIters - 1) <= kUnits, "full unit rounds");
 
 template <int KIND, int OP, typename T, typename C>
 __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
@@ -307,7 +313,8 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
     for (int i = 0; i < kUnitIters; i++) {
         const int u = t + kS3Threads * i;
         ra[i] = rb[i] = rv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (u < kUnits) {
+        // units of the first kUnitIters - 1 rounds always exist (t < 256)
+        if (i < kUnitIters - 1 || u < kUnits) {
             const int k = u & 3, line = u >> 2;
             const int yt = line % kT, zt = line / kT;
             const int yy = yt + 5, zz = zt + 5;   // + 8 - 3: 0..23
@@ -331,7 +338,7 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
 #pragma unroll
     for (int i = 0; i < kUnitIters; i++) {
         const int u = t + kS3Threads * i;
-        if (u < kUnits) {
+        if (i < kUnitIters - 1 || u < kUnits) {
             const int k = u & 3, line = u >> 2;
             const int yt = line % kT, zt = line / kT;
             T v[4];
@@ -425,8 +432,8 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
         }
     }
     if constexpr (KIND != K_DIAG && KIND != K_APPLY) {
-        red0 = wave_sum(red0);
-        if (KIND == K_CG_INIT) red1 = wave_sum(red1);
+        red0 = wave_sum_all(red0);
+        if (KIND == K_CG_INIT) red1 = wave_sum_all(red1);
         if (l == 0) {
             sred[0][wv] = red0;
             sred[1][wv] = red1;
@@ -524,8 +531,8 @@ __global__ void __launch_bounds__(64) k_stencil2(StencilArgs a, C cc) {
         }
     }
     if constexpr (KIND != K_DIAG && KIND != K_APPLY) {
-        red0 = wave_sum(red0);
-        if (KIND == K_CG_INIT) red1 = wave_sum(red1);
+        red0 = wave_sum_all(red0);
+        if (KIND == K_CG_INIT) red1 = wave_sum_all(red1);
         if (t == 0) {
             a.part0[blockIdx.x] = red0;
             if (KIND == K_CG_INIT) a.part1[blockIdx.x] = red1;
