@@ -141,36 +141,33 @@ __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const flo
     const int64_t i = (int64_t)xcd_remap(blockIdx.x, nb) * 256 + threadIdx.x;
     if (i >= n) return;
     const double4 q = pos[i];
-    const double qc[3] = {q.x, q.y, q.z};
-    int cc[3];
-    float qo[3], corner[3][3];   // corner[a][d + 1] = C_q - C_(q + d)
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        cc[a] = cell_coord(qc[a], k.cell_lo[a], k.cell_inv, k.cdims[a]);
-        const double Cq = cell_corner(k, a, cc[a]);
-        qo[a] = (float)(qc[a] - Cq);
-#pragma unroll
-        for (int d = -1; d <= 1; d++)
-            corner[a][d + 1] = (float)(Cq - cell_corner(k, a, cc[a] + d));
-    }
-    const int x0 = cc[0] > 0 ? cc[0] - 1 : 0;
-    const int x1 = cc[0] + 1 < k.cdims[0] ? cc[0] + 1 : k.cdims[0] - 1;
-    const int zlo = cc[2] > 0 ? cc[2] - 1 : 0, zhi = cc[2] + 1 < k.cdims[2] ? cc[2] + 1 : k.cdims[2] - 1;
-    const int ylo = cc[1] > 0 ? cc[1] - 1 : 0, yhi = cc[1] + 1 < k.cdims[1] ? cc[1] + 1 : k.cdims[1] - 1;
-    // x offsets of the (up to) three cells x0, x0 + 1, x0 + 2 of each row
-    const int xb = x0 - cc[0] + 1;   // corner index of cell x0 (0, or 1 at the border)
-    const float tx0 = qo[0] + corner[0][xb];
-    const float tx1 = qo[0] + corner[0][xb + 1 < 2 ? xb + 1 : 2];
-    const float tx2 = qo[0] + corner[0][2];
+    const int cc0 = cell_coord(q.x, k.cell_lo[0], k.cell_inv, k.cdims[0]);
+    const int cc1 = cell_coord(q.y, k.cell_lo[1], k.cell_inv, k.cdims[1]);
+    const int cc2 = cell_coord(q.z, k.cell_lo[2], k.cell_inv, k.cdims[2]);
+    const int x0 = cc0 > 0 ? cc0 - 1 : 0;
+    const int x1 = cc0 + 1 < k.cdims[0] ? cc0 + 1 : k.cdims[0] - 1;
+    const int zlo = cc2 > 0 ? cc2 - 1 : 0, zhi = cc2 + 1 < k.cdims[2] ? cc2 + 1 : k.cdims[2] - 1;
+    const int ylo = cc1 > 0 ? cc1 - 1 : 0, yhi = cc1 + 1 < k.cdims[1] ? cc1 + 1 : k.cdims[1] - 1;
+    // q - C_c of the (up to) three cells x0, x0 + 1, x0 + 2 of each row,
+    // rounded once from FP64 (the staged x_j - C_j complete q - x_j)
+    float tx0 = (float)(q.x - cell_corner(k, 0, x0));
+    float tx1 = (float)(q.x - cell_corner(k, 0, x0 + 1));
+    float tx2 = (float)(q.x - cell_corner(k, 0, x0 + 2));
+    // kept in registers: without this the compiler selects the cell index in
+    // the candidate loop and recomputes the FP64 corner per candidate
+    asm volatile("" : "+v"(tx0), "+v"(tx1), "+v"(tx2));
     // a particle clamped into a border cell takes the FP64 distance throughout
     const float far = 2.f * (float)k.cell_size;
-    const bool qfar = !(fabsf(qo[0]) <= far && fabsf(qo[1]) <= far && fabsf(qo[2]) <= far);
+    const float qo0 = (float)(q.x - cell_corner(k, 0, cc0));
+    const float qo1 = (float)(q.y - cell_corner(k, 1, cc1));
+    const float qo2 = (float)(q.z - cell_corner(k, 2, cc2));
+    const bool qfar = !(fabsf(qo0) <= far && fabsf(qo1) <= far && fabsf(qo2) <= far);
     const float h2 = (float)k.h2;
     float sum = 0.f;
     for (int z = zlo; z <= zhi; z++) {
-        const float tz = qo[2] + corner[2][z - cc[2] + 1];
+        const float tz = (float)(q.z - cell_corner(k, 2, z));
         for (int y = ylo; y <= yhi; y++) {
-            const float ty = qo[1] + corner[1][y - cc[1] + 1];
+            const float ty = (float)(q.y - cell_corner(k, 1, y));
             const int64_t row = ((int64_t)z * k.cdims[1] + y) * k.cdims[0];
             const int b = cstart[row + x0], e = cstart[row + x1 + 1];
             const int b1 = x0 + 1 <= x1 ? cstart[row + x0 + 1] : e;
@@ -178,15 +175,15 @@ __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const flo
             for (int j = b; j < e; j++) {
                 const float4 p = ofs[j];
                 const float tx = j < b1 ? tx0 : (j < b2 ? tx1 : tx2);
-                float d2;
-                if (p.w >= 0.f && !qfar) {
-                    const float dx = tx - p.x, dy = ty - p.y, dz = tz - p.z;
-                    d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                } else {
+                const float dx = tx - p.x, dy = ty - p.y, dz = tz - p.z;
+                float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                if (p.w < 0.f || qfar) {   // rare: FP64 distance
                     const double4 pj = pos[j];
                     d2 = (float)sq_dist3(pj.x, pj.y, pj.z, q.x, q.y, q.z);
                 }
-                if (d2 < h2) sum += spline_w(sqrtf(d2) * k.inv_h, k.c0h);
+                // v_sqrt_f32 (1 ulp): W is rounded to FP32 anyway
+                const float w = spline_w(__builtin_amdgcn_sqrtf(d2) * k.inv_h, k.c0h);
+                sum += d2 < h2 ? w : 0.f;
             }
         }
     }

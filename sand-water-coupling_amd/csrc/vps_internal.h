@@ -150,16 +150,20 @@ __device__ __forceinline__ KW kernel_f32(double d2, double kr2, float kr,
 // (SmoothedParticles.h:60-72: W' = c1 x (3x - 2) below x = 1/2, so
 // -W'/r = c1 (2 - 3x) / kr there; -c1 (1 - x)^2 above, so -W'/r =
 // c1 (1 - x)^2 / r).  rinv = 1 / r is only used for x >= 1/2.
+// Both pieces are evaluated and selected (no branch: the lanes of a wave
+// mix both sides); max(1 - x, 0) makes the outer piece vanish for x >= 1.
 __device__ __forceinline__ float spline_w(float x, float c0) {
-    if (x < 0.5f) return c0 * (6.f * x * x * (x - 1.f) + 1.f);
-    const float omx = 1.f - x;
-    return x < 1.f ? 2.f * c0 * omx * omx * omx : 0.f;
+    const float in = c0 * (6.f * x * x * (x - 1.f) + 1.f);
+    const float omx = fmaxf(1.f - x, 0.f);
+    const float out = 2.f * c0 * omx * omx * omx;
+    return x < 0.5f ? in : out;
 }
 __device__ __forceinline__ float spline_s(float x, float rinv, float c1,
                                           float inv_kr) {
-    if (x < 0.5f) return c1 * inv_kr * (2.f - 3.f * x);
-    const float omx = 1.f - x;
-    return x < 1.f ? c1 * omx * omx * rinv : 0.f;
+    const float in = c1 * inv_kr * (2.f - 3.f * x);
+    const float omx = fmaxf(1.f - x, 0.f);
+    const float out = c1 * omx * omx * rinv;
+    return x < 0.5f ? in : out;
 }
 
 // ---- launch geometry ----------------------------------------------------

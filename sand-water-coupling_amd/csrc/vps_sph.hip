@@ -111,34 +111,28 @@ __global__ void __launch_bounds__(256) k_viscosity3f(const double4* pos, const f
     if (i >= n) return;
     const double4 q = pos[i];
     const float4 vi = vel[i];
-    const double qc[3] = {q.x, q.y, q.z};
-    int cc[3];
-    float qo[3], corner[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        cc[a] = cell_coord(qc[a], k.cell_lo[a], k.cell_inv, k.cdims[a]);
-        const double Cq = cell_corner(k, a, cc[a]);
-        qo[a] = (float)(qc[a] - Cq);
-#pragma unroll
-        for (int d = -1; d <= 1; d++)
-            corner[a][d + 1] = (float)(Cq - cell_corner(k, a, cc[a] + d));
-    }
-    const int x0 = cc[0] > 0 ? cc[0] - 1 : 0;
-    const int x1 = cc[0] + 1 < k.cdims[0] ? cc[0] + 1 : k.cdims[0] - 1;
-    const int zlo = cc[2] > 0 ? cc[2] - 1 : 0, zhi = cc[2] + 1 < k.cdims[2] ? cc[2] + 1 : k.cdims[2] - 1;
-    const int ylo = cc[1] > 0 ? cc[1] - 1 : 0, yhi = cc[1] + 1 < k.cdims[1] ? cc[1] + 1 : k.cdims[1] - 1;
-    const int xb = x0 - cc[0] + 1;
-    const float tx0 = qo[0] + corner[0][xb];
-    const float tx1 = qo[0] + corner[0][xb + 1 < 2 ? xb + 1 : 2];
-    const float tx2 = qo[0] + corner[0][2];
+    const int cc0 = cell_coord(q.x, k.cell_lo[0], k.cell_inv, k.cdims[0]);
+    const int cc1 = cell_coord(q.y, k.cell_lo[1], k.cell_inv, k.cdims[1]);
+    const int cc2 = cell_coord(q.z, k.cell_lo[2], k.cell_inv, k.cdims[2]);
+    const int x0 = cc0 > 0 ? cc0 - 1 : 0;
+    const int x1 = cc0 + 1 < k.cdims[0] ? cc0 + 1 : k.cdims[0] - 1;
+    const int zlo = cc2 > 0 ? cc2 - 1 : 0, zhi = cc2 + 1 < k.cdims[2] ? cc2 + 1 : k.cdims[2] - 1;
+    const int ylo = cc1 > 0 ? cc1 - 1 : 0, yhi = cc1 + 1 < k.cdims[1] ? cc1 + 1 : k.cdims[1] - 1;
+    // q - C_c per candidate cell, rounded once from FP64
+    const float tx0 = (float)(q.x - cell_corner(k, 0, x0));
+    const float tx1 = (float)(q.x - cell_corner(k, 0, x0 + 1));
+    const float tx2 = (float)(q.x - cell_corner(k, 0, x0 + 2));
     const float far = 2.f * (float)k.cell_size;
-    const bool qfar = !(fabsf(qo[0]) <= far && fabsf(qo[1]) <= far && fabsf(qo[2]) <= far);
+    const float qo0 = (float)(q.x - cell_corner(k, 0, cc0));
+    const float qo1 = (float)(q.y - cell_corner(k, 1, cc1));
+    const float qo2 = (float)(q.z - cell_corner(k, 2, cc2));
+    const bool qfar = !(fabsf(qo0) <= far && fabsf(qo1) <= far && fabsf(qo2) <= far);
     const float h2 = (float)k.h2;
     ViscAcc acc;
     for (int z = zlo; z <= zhi; z++) {
-        const float tz = qo[2] + corner[2][z - cc[2] + 1];
+        const float tz = (float)(q.z - cell_corner(k, 2, z));
         for (int y = ylo; y <= yhi; y++) {
-            const float ty = qo[1] + corner[1][y - cc[1] + 1];
+            const float ty = (float)(q.y - cell_corner(k, 1, y));
             const int64_t row = ((int64_t)z * k.cdims[1] + y) * k.cdims[0];
             const int b = cstart[row + x0], e = cstart[row + x1 + 1];
             const int b1 = x0 + 1 <= x1 ? cstart[row + x0 + 1] : e;

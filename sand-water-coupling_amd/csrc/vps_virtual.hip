@@ -433,48 +433,51 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
                        vvol, vb, bits);
         return;
     }
-    const double qc[3] = {qx, qy, qz};
-    int cc[3];
-    float qo[3], corner[3][3];   // corner[a][d + 1] = C_q - C_(q + d)
-#pragma unroll
-    for (int ax = 0; ax < 3; ax++) {
-        cc[ax] = cell_coord(qc[ax], k.cell_lo[ax], k.cell_inv, k.cdims[ax]);
-        const double Cq = cell_corner(k, ax, cc[ax]);
-        qo[ax] = (float)(qc[ax] - Cq);
-#pragma unroll
-        for (int d = -1; d <= 1; d++)
-            corner[ax][d + 1] = (float)(Cq - cell_corner(k, ax, cc[ax] + d));
-    }
-    const int xlo = max(cc[0] - 1, 0), xhi = min(cc[0] + 1, k.cdims[0] - 1);
-    const int zlo = max(cc[2] - 1, 0), zhi = min(cc[2] + 1, k.cdims[2] - 1);
-    const int ylo = max(cc[1] - 1, 0), yhi = min(cc[1] + 1, k.cdims[1] - 1);
+    // q - C_c per axis for the candidate cell c, rounded once from FP64
+    // (the staged x_j - C_j complete the difference q - x_j)
+    const int cc0 = cell_coord(qx, k.cell_lo[0], k.cell_inv, k.cdims[0]);
+    const int cc1 = cell_coord(qy, k.cell_lo[1], k.cell_inv, k.cdims[1]);
+    const int cc2 = cell_coord(qz, k.cell_lo[2], k.cell_inv, k.cdims[2]);
+    const int xlo = max(cc0 - 1, 0), xhi = min(cc0 + 1, k.cdims[0] - 1);
+    const int zlo = max(cc2 - 1, 0), zhi = min(cc2 + 1, k.cdims[2] - 1);
+    const int ylo = max(cc1 - 1, 0), yhi = min(cc1 + 1, k.cdims[1] - 1);
+    // the (up to) three cells xlo, xlo + 1, xlo + 2 of a row are one
+    // contiguous staged range; x offsets kept in registers (otherwise the
+    // compiler selects the cell index per candidate and redoes the FP64)
+    float tx0 = (float)(qx - cell_corner(k, 0, xlo));
+    float tx1 = (float)(qx - cell_corner(k, 0, xlo + 1));
+    float tx2 = (float)(qx - cell_corner(k, 0, xlo + 2));
+    asm volatile("" : "+v"(tx0), "+v"(tx1), "+v"(tx2));
+    const int i0 = xlo - s_meta[0], i1 = xhi - s_meta[0];
     for (int z = zlo; z <= zhi; z++) {
-        const float tz = qo[2] + corner[2][z - cc[2] + 1];
+        const float tz = (float)(qz - cell_corner(k, 2, z));
         for (int y = ylo; y <= yhi; y++) {
-            const float ty = qo[1] + corner[1][y - cc[1] + 1];
+            const float ty = (float)(qy - cell_corner(k, 1, y));
             const int r = (z - s_meta[2]) * ncy + (y - s_meta[1]);
             const int rg = s_rg[r];
-            for (int x = xlo; x <= xhi; x++) {
-                const float tx = qo[0] + corner[0][x - cc[0] + 1];
-                const int i = x - s_meta[0];
-                const int jb = s_cs[r][i], je = s_cs[r][i + 1];
-                for (int j = jb; j < je; j++) {
-                    const float4 pr = s_p[j];
-                    const float dx = tx - pr.x, dy = ty - pr.y, dz = tz - pr.z;
-                    const float d2f = dx * dx + dy * dy + dz * dz;
-                    if (d2f >= k.h2f_hi && pr.w >= 0.f) continue;
+            const int jb = s_cs[r][i0], je = s_cs[r][i1 + 1];
+            const int b1 = s_cs[r][min(i0 + 1, i1 + 1)];
+            const int b2 = s_cs[r][min(i0 + 2, i1 + 1)];
+            for (int j = jb; j < je; j++) {
+                const float4 pr = s_p[j];
+                const float tx = j < b1 ? tx0 : (j < b2 ? tx1 : tx2);
+                const float dx = tx - pr.x, dy = ty - pr.y, dz = tz - pr.z;
+                const float d2f = dx * dx + dy * dy + dz * dz;
+                // fast: inside h by the FP32 band; exact: near h, at r = 0,
+                // or far from its cell (bitwise ops: one mask each, no
+                // nested branches)
+                const bool staged_ok = pr.w >= 0.f;
+                const bool fast = staged_ok & (d2f > 0.f) & (d2f < k.h2f_lo);
+                const bool exact = !fast & (!staged_ok | (d2f < k.h2f_hi));
+                if (fast) a.add_fast<3>(k, d2f, dx, dy, dz, rvel[j + rg], pr.w);
+                if (exact) {
+                    // the reference's FP64 test and the accurate spline
                     const int g = j + rg;
-                    if (d2f < k.h2f_lo && pr.w >= 0.f && d2f > 0.f) {
-                        a.add_fast<3>(k, d2f, dx, dy, dz, rvel[g], pr.w);
-                    } else {
-                        // near h, at r = 0, or far from its cell: the
-                        // reference's FP64 test and the accurate spline
-                        const double4 pj = rpos[g];
-                        const double d2 = sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz);
-                        if (!(d2 < k.h2)) continue;
+                    const double4 pj = rpos[g];
+                    const double d2 = sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz);
+                    if (d2 < k.h2)
                         a.add<3>(k, d2, (float)(qx - pj.x), (float)(qy - pj.y),
                                  (float)(qz - pj.z), rvel[g], fabsf(pr.w));
-                    }
                 }
             }
         }
