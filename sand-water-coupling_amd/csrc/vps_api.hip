@@ -169,6 +169,7 @@ void free_all(vps_context* c) {
     }
     dfree(c->rrho);
     dfree(c->rofs);
+    dfree(c->bofs);
     dfree(c->vel_next);
     dfree(c->grad_chunks);
     dfree(c->grad_over);

@@ -237,6 +237,29 @@ __global__ void __launch_bounds__(256) k_bdensity(const double4* pos,
     vel[i].w = k.mass / r;
 }
 
+// FP32 offsets of the (sorted) boundary particles from their cell corners,
+// w = +1/rho_b, or -1/rho_b when clamped into a border cell (as the real
+// offsets); recomputed with every boundary density.
+__global__ void __launch_bounds__(256) k_boundary_offsets3(const double4* pos,
+                                                           const float4* nrm, int64_t n,
+                                                           KConst k, float4* ofs) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double4 p = pos[i];
+    const double pc[3] = {p.x, p.y, p.z};
+    const float far = 2.f * (float)k.cell_size;
+    float of[3];
+    bool is_far = false;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int c = cell_coord(pc[a], k.cell_lo[a], k.cell_inv, k.cdims[a]);
+        of[a] = (float)(pc[a] - cell_corner(k, a, c));
+        is_far |= !(fabsf(of[a]) <= far);
+    }
+    const float ir = 1.f / nrm[i].w;
+    ofs[i] = make_float4(of[0], of[1], of[2], is_far ? -ir : ir);
+}
+
 inline unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -323,9 +346,14 @@ int real_density(vps_context* c, bool set_volume) {
 int boundary_density(vps_context* c) {
     const int64_t nb = c->nb;
     if (c->dim == 3)
+    {
         hipLaunchKernelGGL(k_bdensity<3>, dim3(blocks(nb, 256)), dim3(256), 0,
                            c->stream, c->bpos.p, c->bcell_start.p, nb, c->k,
                            c->bnrm.p, c->bvel.p);
+        VPS_TRY(dalloc(c, c->bofs, (size_t)nb));
+        hipLaunchKernelGGL(k_boundary_offsets3, dim3(blocks(nb, 256)), dim3(256), 0,
+                           c->stream, c->bpos.p, c->bnrm.p, nb, c->k, c->bofs.p);
+    }
     else
         hipLaunchKernelGGL(k_bdensity<2>, dim3(blocks(nb, 256)), dim3(256), 0,
                            c->stream, c->bpos.p, c->bcell_start.p, nb, c->k,

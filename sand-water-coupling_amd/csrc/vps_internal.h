@@ -166,6 +166,20 @@ __device__ __forceinline__ float spline_s(float x, float rinv, float c1,
     return x < 0.5f ? in : out;
 }
 
+// Candidate masks of an FP32 pre-test on cell-relative offsets (w < 0 marks
+// a particle clamped into a border cell): `fast` inside h by the FP32 band,
+// `exact` where the FP64 test decides (the band around h^2, r = 0, clamped
+// particles); neither: outside h.
+struct Pretest {
+    bool fast, exact;
+};
+__device__ __forceinline__ Pretest pretest(const KConst& k, float d2f, float w) {
+    const bool ok = w >= 0.f;
+    const bool fast = ok & (d2f > 0.f) & (d2f < k.h2f_lo);
+    const bool exact = !fast & (!ok | (d2f < k.h2f_hi));
+    return Pretest{fast, exact};
+}
+
 // ---- launch geometry ----------------------------------------------------
 // Workgroups are dealt round-robin to the 8 XCDs of an MI355X.  Remapping
 // the hardware block id so that each XCD gets a contiguous range of logical
@@ -355,6 +369,9 @@ struct vps_context {
     // sorted order: FP32 offset of each particle from its cell corner, w =
     // +-1/rho (negative: clamped into a border cell, exact test only)
     vps::DBuf<float4> rofs;
+    // cell-sorted boundary, 3D: FP32 offset from the cell corner, w = +-1/rho_b
+    // (negative: clamped into a border cell, exact test only)
+    vps::DBuf<float4> bofs;
     vps::DBuf<int4> grad_chunks;        // 3D gradient: particle and cell ranges of spans
     vps::DBuf<int32_t> grad_over;       // spans left to the global-memory path
     int grad_span_cells = 0;            // cells per span (0: global-memory gradient)

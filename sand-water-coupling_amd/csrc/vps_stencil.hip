@@ -663,6 +663,31 @@ __global__ void __launch_bounds__(kUpdThreads) k_replace(int64_t n4,
     if (threadIdx.x == 0) part[blockIdx.x] = rr;
 }
 
+// Residual replacement keeping the direction: r = rt; partial r.r into the
+// ring slot that the next SpMV reads as the newest r.r; the solve is open
+// again (block 0 clears the flag: no kernel of this launch reads it).
+__global__ void __launch_bounds__(kUpdThreads) k_replace_keep(int64_t n4,
+                                                              const float4* rt, float4* r,
+                                                              double* part, CgState* st) {
+    __shared__ double sbuf[kUpdThreads / 64];
+    double rr = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kUpdThreads + threadIdx.x; i < n4;
+         i += (int64_t)kUpdBlocks * kUpdThreads) {
+        const float4 v = rt[i];
+        r[i] = v;
+        rr += ((double)v.x * v.x + (double)v.y * v.y) +
+              ((double)v.z * v.z + (double)v.w * v.w);
+    }
+    rr = block_sum<kUpdThreads>(rr, sbuf);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = rr;
+        if (blockIdx.x == 0) {
+            st->done = 0;
+            st->true_rr = -1.0;
+        }
+    }
+}
+
 // After a replacement: the restart's r.r becomes abs0.
 __global__ void __launch_bounds__(1024) k_restart(const double* part,
                                                   CgState* st) {
@@ -1031,8 +1056,24 @@ int cg_solve(vps_context* c, vps_stats* stats) {
         // state was computed right after the batch
         if (!c->check_true) break;
         if (hs.true_rr < hs.thr || replacements >= 4) break;
-        // restart from the true residual
         replacements++;
+        if (hs.iter > iter_base) {
+            // residual replacement (van der Vorst and Ye): r takes the true
+            // residual, the direction is kept and the iteration continues
+            // where convergence was declared.  The SpMV that declared it is
+            // launched again: its prologue forms beta from the new r.r,
+            // written into the ring slot of the previous update.
+            kk = hs.iter - iter_base + 1;
+            hipLaunchKernelGGL(k_replace_keep, dim3(kUpdBlocks), dim3(kUpdThreads), 0,
+                               c->stream, n4, (const float4*)t, (float4*)r,
+                               part_rr(c, kk + 2), c->cg.p);
+            VPS_HIP(c, hipGetLastError());
+            hs.done = 0;   // k_replace_keep reopened the solve on the device
+            predicted = iter_base + kk + 2;
+            continue;
+        }
+        // converged before any iteration of this (re)start: restart from the
+        // true residual
         iter_base = hs.iter;
         kk = 0;
         hipLaunchKernelGGL(k_replace, dim3(kUpdBlocks), dim3(kUpdThreads), 0,

@@ -139,17 +139,71 @@ struct RealAcc {
     }
 };
 
+// Boundary sums of one node: sum V_b W, sum v_b W / rho_b, sum V_b gradK,
+// sum V_b v_b . gradK (VirtualParticle.cpp:76-107).
+struct BAcc {
+    float den = 0.f, bnx = 0.f, bny = 0.f, bnz = 0.f;
+    float hx = 0.f, hy = 0.f, hz = 0.f, db = 0.f;
+    // boundary particle at pj within h (FP64 test passed): accurate spline
+    template <int D>
+    __device__ __forceinline__ void add_exact(const KConst& k, double qx, double qy,
+                                              double qz, const double4& pj, double d2,
+                                              const float4& vj, float rho_b) {
+        const KW kh = kernel_f32(d2, k.h2, k.hf, k.inv_h, k.c0h, k.c1h);
+        const float Vj = vj.w;
+        const float wr = kh.w / rho_b;
+        den += Vj * kh.w;
+        bnx += vj.x * wr;
+        bny += vj.y * wr;
+        bnz += vj.z * wr;
+        if (kh.r > 0.f) {
+            const float s = -kh.dw / kh.r;
+            const float ex = (float)(qx - pj.x) * s;
+            const float ey = (float)(qy - pj.y) * s;
+            const float ez = D == 3 ? (float)(qz - pj.z) * s : 0.f;
+            hx += Vj * ex;
+            hy += Vj * ey;
+            hz += Vj * ez;
+            db += Vj * (vj.x * ex + vj.y * ey + vj.z * ez);
+        }
+    }
+    // the same from an FP32 squared distance 0 < d2f < h^2 and d = q - x_b,
+    // division-free (as the real sums)
+    __device__ __forceinline__ void add_fast(const KConst& k, float d2f, float dx, float dy,
+                                             float dz, const float4& vj, float inv_rho_b) {
+        const float rinv = __builtin_amdgcn_rsqf(d2f);
+        const float xh = d2f * rinv * k.inv_h;
+        const float wh = spline_w(xh, k.c0h);
+        const float sh = spline_s(xh, rinv, k.c1h, k.inv_h);
+        const float Vj = vj.w;
+        const float wr = wh * inv_rho_b;
+        den += Vj * wh;
+        bnx += vj.x * wr;
+        bny += vj.y * wr;
+        bnz += vj.z * wr;
+        const float ex = dx * sh, ey = dy * sh, ez = dz * sh;
+        hx += Vj * ex;
+        hy += Vj * ey;
+        hz += Vj * ez;
+        db += Vj * (vj.x * ex + vj.y * ey + vj.z * ez);
+    }
+};
+
 // Everything after the real walk: the exact threshold decision, the boundary
 // walk, v_I, D_I and the outputs.
-// Boundary walk of one node from the global cell table.
+// Boundary walk of one node from the global cell table (FP64 throughout).
 template <int D>
 struct BoundaryWalkGlobal {
     const int32_t* bcstart;
     const double4* bpos;
-    template <typename F>
-    __device__ __forceinline__ void operator()(const KConst& k, double qx,
-                                               double qy, double qz, F&& f) const {
-        for_each_candidate<D>(k, bcstart, bpos, qx, qy, qz, f);
+    __device__ __forceinline__ void operator()(const KConst& k, double qx, double qy,
+                                               double qz, const float4* bvel,
+                                               const float4* bnrm, BAcc& b) const {
+        for_each_candidate<D>(k, bcstart, bpos, qx, qy, qz, [&](int j, const double4& pj) {
+            const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz)
+                                     : sq_dist2(pj.x, pj.y, qx, qy);
+            if (d2 < k.h2) b.add_exact<D>(k, qx, qy, qz, pj, d2, bvel[j], bnrm[j].w);
+        });
     }
 };
 
@@ -177,42 +231,18 @@ __device__ __forceinline__ void finish_node(
         return;
     }
     // boundary neighbours within h
-    float den = a.den;
-    float bnx = 0.f, bny = 0.f, bnz = 0.f;
-    float hx = 0.f, hy = 0.f, hz = 0.f, db = 0.f;
-    bwalk(k, qx, qy, qz, [&](int j, const double4& pj) {
-        const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz)
-                                 : sq_dist2(pj.x, pj.y, qx, qy);
-        if (d2 < k.h2) {
-            const KW kh = kernel_f32(d2, k.h2, k.hf, k.inv_h, k.c0h, k.c1h);
-            const float4 vj = bvel[j];
-            const float Vj = vj.w;
-            const float wr = kh.w / bnrm[j].w;
-            den += Vj * kh.w;
-            bnx += vj.x * wr;
-            bny += vj.y * wr;
-            bnz += vj.z * wr;
-            if (kh.r > 0.f) {
-                const float s = -kh.dw / kh.r;
-                const float ex = (float)(qx - pj.x) * s;
-                const float ey = (float)(qy - pj.y) * s;
-                const float ez = D == 3 ? (float)(qz - pj.z) * s : 0.f;
-                hx += Vj * ex;
-                hy += Vj * ey;
-                hz += Vj * ez;
-                db += Vj * (vj.x * ex + vj.y * ey + vj.z * ez);
-            }
-        }
-    });
+    BAcc b;
+    b.den = a.den;
+    bwalk(k, qx, qy, qz, bvel, bnrm, b);
     // v_I = (m sum v_j W/rho_j + m_b sum v_b W/rho_b) / (sum V W + sum V_b W + 1e-6)
-    const float inv_den = 1.f / (den + 1e-6f);
-    const float ux = (k.mass * a.nvx + k.mass * bnx) * inv_den;
-    const float uy = (k.mass * a.nvy + k.mass * bny) * inv_den;
-    const float uz = (k.mass * a.nvz + k.mass * bnz) * inv_den;
+    const float inv_den = 1.f / (b.den + 1e-6f);
+    const float ux = (k.mass * a.nvx + k.mass * b.bnx) * inv_den;
+    const float uy = (k.mass * a.nvy + k.mass * b.bny) * inv_den;
+    const float uz = (k.mass * a.nvz + k.mass * b.bnz) * inv_den;
     const float rho = k.mass * a.sH;
     const float drho = rho - k.rho0;
-    const float D_I = a.dr - (ux * a.gx + uy * a.gy + uz * a.gz) + db -
-                      (ux * hx + uy * hy + uz * hz) -
+    const float D_I = a.dr - (ux * a.gx + uy * a.gy + uz * a.gz) + b.db -
+                      (ux * b.hx + uy * b.hy + uz * b.hz) -
                       k.kappa * (drho > 0.f ? drho : 0.f) / k.rho0;
     vrho[out] = rho;
     vvol[out] = a.sV;
@@ -293,24 +323,57 @@ constexpr int kStageCap = 4096;
 constexpr int kStageRows = 64;
 constexpr int kStageCx = 8;
 
+// Boundary walk of one node over the staged rows' boundary cell table, with
+// the FP32 pre-test on the boundary's cell offsets (bofs) and the FP64 test
+// in the band around h (the boundary set stays exact).
 struct BoundaryWalkStaged {
     const int (*bcs)[kStageCx + 1];   // global starts of the staged rows
     const int* meta;                  // clo x,y,z; ncx, ncy
     const double4* bpos;
-    template <typename F>
-    __device__ __forceinline__ void operator()(const KConst& k, double qx,
-                                               double qy, double qz, F&& f) const {
+    const float4* bofs;
+    __device__ __forceinline__ void operator()(const KConst& k, double qx, double qy,
+                                               double qz, const float4* bvel,
+                                               const float4* bnrm, BAcc& b) const {
         const int cx = cell_coord(qx, k.cell_lo[0], k.cell_inv, k.cdims[0]);
         const int cy = cell_coord(qy, k.cell_lo[1], k.cell_inv, k.cdims[1]);
         const int cz = cell_coord(qz, k.cell_lo[2], k.cell_inv, k.cdims[2]);
-        const int i0 = max(cx - 1, 0) - meta[0];
-        const int i1 = min(cx + 1, k.cdims[0] - 1) - meta[0] + 1;
-        for (int z = max(cz - 1, 0); z <= min(cz + 1, k.cdims[2] - 1); z++)
+        const int xlo = max(cx - 1, 0), xhi = min(cx + 1, k.cdims[0] - 1);
+        const int i0 = xlo - meta[0], i1 = xhi - meta[0] + 1;
+        float tx0 = (float)(qx - cell_corner(k, 0, xlo));
+        float tx1 = (float)(qx - cell_corner(k, 0, xlo + 1));
+        float tx2 = (float)(qx - cell_corner(k, 0, xlo + 2));
+        asm volatile("" : "+v"(tx0), "+v"(tx1), "+v"(tx2));
+        for (int z = max(cz - 1, 0); z <= min(cz + 1, k.cdims[2] - 1); z++) {
+            const float tz = (float)(qz - cell_corner(k, 2, z));
             for (int y = max(cy - 1, 0); y <= min(cy + 1, k.cdims[1] - 1); y++) {
                 const int r = (z - meta[2]) * meta[4] + (y - meta[1]);
-                const int b = bcs[r][i0], e = bcs[r][i1];
-                for (int j = b; j < e; j++) f(j, bpos[j]);
+                const int jb = bcs[r][i0], je = bcs[r][i1];
+                if (jb >= je) continue;
+                const float ty = (float)(qy - cell_corner(k, 1, y));
+                const int b1 = bcs[r][min(i0 + 1, i1)], b2 = bcs[r][min(i0 + 2, i1)];
+                for (int j0 = jb; j0 < je; j0 += 4) {
+                  float4 bov[4];
+#pragma unroll
+                  for (int u = 0; u < 4; u++)
+                      bov[u] = j0 + u < je ? bofs[j0 + u] : make_float4(1e30f, 0.f, 0.f, 1.f);
+#pragma unroll
+                  for (int u = 0; u < 4; u++) {
+                    const int j = j0 + u;
+                    const float4 bo = bov[u];
+                    const float tx = j < b1 ? tx0 : (j < b2 ? tx1 : tx2);
+                    const float dx = tx - bo.x, dy = ty - bo.y, dz = tz - bo.z;
+                    const float d2f = dx * dx + dy * dy + dz * dz;
+                    const Pretest t = pretest(k, d2f, bo.w);
+                    if (t.fast) b.add_fast(k, d2f, dx, dy, dz, bvel[j], bo.w);
+                    if (t.exact) {
+                        const double4 pj = bpos[j];
+                        const double d2 = sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz);
+                        if (d2 < k.h2) b.add_exact<3>(k, qx, qy, qz, pj, d2, bvel[j], bnrm[j].w);
+                    }
+                  }
+                }
             }
+        }
     }
 };
 
@@ -318,7 +381,7 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
     KConst k, const int32_t* brick_of_slot, const int32_t* nslots_p,
     const double4* rpos, const float4* rvel, const float* rrho,
     const float4* rofs, const int32_t* rcstart, const double4* bpos,
-    const float4* bvel, const float4* bnrm, const int32_t* bcstart,
+    const float4* bofs, const float4* bvel, const float4* bnrm, const int32_t* bcstart,
     float* vrho, float* vvol, float* vb, uint32_t* bits) {
     __shared__ float4 s_p[kStageCap];   // (x_j - C_j, +-1/rho_j; < 0: exact only)
     __shared__ int s_cs[kStageRows][kStageCx + 1];  // real cell starts (staged)
@@ -463,14 +526,9 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
                 const float tx = j < b1 ? tx0 : (j < b2 ? tx1 : tx2);
                 const float dx = tx - pr.x, dy = ty - pr.y, dz = tz - pr.z;
                 const float d2f = dx * dx + dy * dy + dz * dz;
-                // fast: inside h by the FP32 band; exact: near h, at r = 0,
-                // or far from its cell (bitwise ops: one mask each, no
-                // nested branches)
-                const bool staged_ok = pr.w >= 0.f;
-                const bool fast = staged_ok & (d2f > 0.f) & (d2f < k.h2f_lo);
-                const bool exact = !fast & (!staged_ok | (d2f < k.h2f_hi));
-                if (fast) a.add_fast<3>(k, d2f, dx, dy, dz, rvel[j + rg], pr.w);
-                if (exact) {
+                const Pretest t = pretest(k, d2f, pr.w);
+                if (t.fast) a.add_fast<3>(k, d2f, dx, dy, dz, rvel[j + rg], pr.w);
+                if (t.exact) {
                     // the reference's FP64 test and the accurate spline
                     const int g = j + rg;
                     const double4 pj = rpos[g];
@@ -483,7 +541,7 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
         }
     }
     finish_node<3>(k, a, qx, qy, qz, cx, cy, cz, out, rpos, rcstart,
-                   BoundaryWalkStaged{s_bcs, s_meta, bpos}, bvel, bnrm, vrho,
+                   BoundaryWalkStaged{s_bcs, s_meta, bpos, bofs}, bvel, bnrm, vrho,
                    vvol, vb, bits);
 }
 
@@ -599,7 +657,7 @@ static int generate_slots(vps_context* c, int w, int64_t grid) {
         hipLaunchKernelGGL(k_virtual_sums3s, dim3((unsigned)grid),
                            dim3(512), 0, c->stream, c->k, c->slot_brick[w].p,
                            nsl, c->rpos[rs].p, c->rvel[rs].p, c->rrho.p, c->rofs.p,
-                           c->cell_start.p, c->bpos.p, c->bvel.p, c->bnrm.p,
+                           c->cell_start.p, c->bpos.p, c->bofs.p, c->bvel.p, c->bnrm.p,
                            c->bcell_start.p, c->vrho.p, c->vvol.p, c->vb.p,
                            c->bits.p);
     else
