@@ -339,6 +339,7 @@ def main_sand_water(args):
         "cg_iters": iters,
         "cg_iters_while_training": train_iters,
         "true_rel_residual_max": max(s["cg_true_rel_residual"] for s in stats),
+        "cg_replacements": [s["cg_replacements"] for s in stats],
         "phase_ms_mean": {k: round(float(np.mean([s[k] for s in stats])), 4)
                           for k in ("ms_dem", "ms_couple", "ms_sort", "ms_generate",
                                     "ms_diag", "ms_cg", "ms_gradient", "ms_total")},
@@ -360,6 +361,7 @@ def summary(r, n, steps):
         "cg_iters_mean": round(float(np.mean([s["cg_iterations"] for s in st])), 2),
         "cg_ms_mean": round(float(np.mean([s["ms_cg"] for s in st])), 4),
         "true_rel_residual_max": max(s["cg_true_rel_residual"] for s in st),
+        "replacements_mean": round(float(np.mean([s["cg_replacements"] for s in st])), 2),
     }
 
 
@@ -435,6 +437,7 @@ def main():
         "cg_iters": iters,
         "n_virtual_mean": round(float(np.mean(nv)), 1),
         "true_rel_residual_max": max(s["cg_true_rel_residual"] for s in stats),
+        "cg_replacements": [s["cg_replacements"] for s in stats],
         "phase_ms_mean": {k: round(float(np.mean([s[k] for s in stats])), 4)
                           for k in ("ms_sort", "ms_generate", "ms_diag",
                                     "ms_cg", "ms_gradient", "ms_total")},
