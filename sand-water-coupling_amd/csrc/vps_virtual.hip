@@ -521,21 +521,41 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
             const int jb = s_cs[r][i0], je = s_cs[r][i1 + 1];
             const int b1 = s_cs[r][min(i0 + 1, i1 + 1)];
             const int b2 = s_cs[r][min(i0 + 2, i1 + 1)];
-            for (int j = jb; j < je; j++) {
-                const float4 pr = s_p[j];
-                const float tx = j < b1 ? tx0 : (j < b2 ? tx1 : tx2);
-                const float dx = tx - pr.x, dy = ty - pr.y, dz = tz - pr.z;
-                const float d2f = dx * dx + dy * dy + dz * dz;
-                const Pretest t = pretest(k, d2f, pr.w);
-                if (t.fast) a.add_fast<3>(k, d2f, dx, dy, dz, rvel[j + rg], pr.w);
-                if (t.exact) {
+            // Two passes per chunk of 32 candidates: the cheap FP32 test
+            // sets one bit per member, then each lane visits only its own
+            // members (a lane has ~15% of its candidates within h, so the
+            // wave no longer runs the accumulation for every candidate).
+            for (int c0 = jb; c0 < je; c0 += 32) {
+                const int c1 = min(c0 + 32, je);
+                uint32_t mfast = 0u, mexact = 0u;
+                for (int j = c0; j < c1; j++) {
+                    const float4 pr = s_p[j];
+                    const float tx = j < b1 ? tx0 : (j < b2 ? tx1 : tx2);
+                    const float dx = tx - pr.x, dy = ty - pr.y, dz = tz - pr.z;
+                    const float d2f = dx * dx + dy * dy + dz * dz;
+                    const Pretest t = pretest(k, d2f, pr.w);
+                    mfast |= (uint32_t)t.fast << (j - c0);
+                    mexact |= (uint32_t)t.exact << (j - c0);
+                }
+                while (mfast) {
+                    const int j = c0 + __builtin_ctz(mfast);
+                    mfast &= mfast - 1u;
+                    const float4 pr = s_p[j];
+                    const float tx = j < b1 ? tx0 : (j < b2 ? tx1 : tx2);
+                    const float dx = tx - pr.x, dy = ty - pr.y, dz = tz - pr.z;
+                    const float d2f = dx * dx + dy * dy + dz * dz;
+                    a.add_fast<3>(k, d2f, dx, dy, dz, rvel[j + rg], pr.w);
+                }
+                while (mexact) {
                     // the reference's FP64 test and the accurate spline
+                    const int j = c0 + __builtin_ctz(mexact);
+                    mexact &= mexact - 1u;
                     const int g = j + rg;
                     const double4 pj = rpos[g];
                     const double d2 = sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz);
                     if (d2 < k.h2)
                         a.add<3>(k, d2, (float)(qx - pj.x), (float)(qy - pj.y),
-                                 (float)(qz - pj.z), rvel[g], fabsf(pr.w));
+                                 (float)(qz - pj.z), rvel[g], fabsf(s_p[j].w));
                 }
             }
         }
