@@ -89,14 +89,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# the one-launch 3D CG iteration (update of the previous iteration + SpMV)
+SPMV_KERNEL = "k_stencil3<SPMV_UPD, TRANSPOSE>"
+SPMV_BYTES = 40.0
+
+
 def traffic_from_profiles(workload):
-    """HBM bytes per SpMV launch measured with rocprofv3 counters, if a
-    summary for this workload is committed under profiles/."""
+    """HBM bytes per launch of the CG iteration kernel measured with
+    rocprofv3 counters, if a summary for this workload and kernel is
+    committed under profiles/."""
     path = os.path.join(ROOT, "profiles", "spmv_traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
-        if d.get("workload") == workload:
+        if d.get("workload") == workload and d.get("kernel") == SPMV_KERNEL:
             return d.get("bytes_per_launch")
     except (OSError, ValueError):
         pass
@@ -309,11 +315,11 @@ def main_sand_water(args):
     roof = None
     if spmv:
         t_launch = float(np.mean(spmv)) * 1e-3
-        bytes_launch = 24.0 * float(np.mean([s["n_virtual"] for s in kt]))
+        bytes_launch = SPMV_BYTES * float(np.mean([s["n_virtual"] for s in kt]))
         ach = bytes_launch / t_launch / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_stencil3<SPMV_FUSED, TRANSPOSE>",
+                "kernel": SPMV_KERNEL,
                 "avg_launch_us": round(t_launch * 1e6, 2),
                 "bytes_per_launch": bytes_launch}
     out = {
@@ -385,33 +391,33 @@ def main():
     ms_step = 1e3 * elapsed / args.steps
     iters = [s["cg_iterations"] for s in stats]
     nv = [s["n_virtual"] for s in stats]
-    # roofline of the CG SpMV (dominant kernel of the plain step): algorithmic
-    # bytes per launch = 24 B per virtual particle (read p_old, r, V, diag;
-    # write p = r + beta p_old and t = M p, all FP32)
+    # roofline of the CG iteration kernel (dominant kernel of the plain step):
+    # algorithmic bytes per launch = 40 B per virtual particle (read p_old,
+    # r_old, t_old, x, V, diag; write p, r, t, x, all FP32)
     kt = plain["timed_launch"]
     spmv = [s["ms_spmv_avg"] for s in kt if s["ms_spmv_avg"] > 0]
     roof = None
     if spmv:
         t_launch = float(np.mean(spmv)) * 1e-3
         nv_kt = float(np.mean([s["n_virtual"] for s in kt]))
-        bytes_launch = 24.0 * nv_kt
+        bytes_launch = SPMV_BYTES * nv_kt
         ach = bytes_launch / t_launch / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profiles(workload),
-                "kernel": "k_stencil3<SPMV_FUSED, TRANSPOSE>",
+                "kernel": SPMV_KERNEL,
                 "avg_launch_us": round(t_launch * 1e6, 2),
                 "bytes_per_launch": bytes_launch,
                 "slot_nodes_per_launch": 512 * float(np.mean([s["n_active_bricks"] for s in kt]))}
-    # whole-step algorithmic bytes: 144 N + 76 N_V + 8 C + 48 k N_V (cell
+    # whole-step algorithmic bytes: 144 N + 76 N_V + 8 C + 40 k N_V (cell
     # sort, densities, virtual marking and sums, diagonal, CG init, k CG
-    # iterations of 24 B SpMV + 24 B update, gradient; the boundary term
-    # 80 N_B' is left out, so the fraction is a lower bound)
+    # iterations of the 40 B one-launch update + SpMV, gradient; the
+    # boundary term 80 N_B' is left out, so the fraction is a lower bound)
     h = 2.0 * sc["particle_radius"] * 2.0
     cells = float(np.prod(np.ceil((np.asarray(sc["domain_hi"]) - np.asarray(sc["domain_lo"]))[:3] / h)))
     nvm = float(np.mean(nv))
     kmean = float(np.mean(iters))
-    b_step = 144.0 * n + 76.0 * nvm + 8.0 * cells + 48.0 * kmean * nvm
+    b_step = 144.0 * n + 76.0 * nvm + 8.0 * cells + SPMV_BYTES * kmean * nvm
     step_roof = {"bytes_per_step": b_step, "achieved": round(b_step / (ms_step * 1e-3) / 1e9, 1),
                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": round(b_step / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}

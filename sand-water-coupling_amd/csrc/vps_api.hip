@@ -195,6 +195,9 @@ void free_all(vps_context* c) {
     dfree(c->vdiag);
     dfree(c->vr);
     dfree(c->vt);
+    dfree(c->vr2);
+    dfree(c->vt2);
+    dfree(c->vrt);
     dfree(c->vq);
     dfree(c->vdiag64);
     dfree(c->vrowsum);
@@ -345,6 +348,9 @@ int vps_create(const vps_params* params, vps_context** out) {
     ALLOC(c->vrowsum, cap);
     ALLOC(c->vr, cap);
     ALLOC(c->vt, cap);
+    ALLOC(c->vr2, cap);
+    ALLOC(c->vt2, cap);
+    ALLOC(c->vrt, cap);
     ALLOC(c->vq, cap);
     ALLOC(c->vrank, cap);
     ALLOC(c->nbr_slot, (size_t)(c->nbricks + 1) * 27);
@@ -383,6 +389,9 @@ int vps_create(const vps_params* params, vps_context** out) {
             hipMemset(c->vrowsum.p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->vr.p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->vt.p, 0, cap * 4) != hipSuccess ||
+            hipMemset(c->vr2.p, 0, cap * 4) != hipSuccess ||
+            hipMemset(c->vt2.p, 0, cap * 4) != hipSuccess ||
+            hipMemset(c->vrt.p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->vx[0].p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->vx[1].p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->vp[0].p, 0, cap * 4) != hipSuccess ||

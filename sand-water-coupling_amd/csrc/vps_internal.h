@@ -345,6 +345,9 @@ struct CgState {
     int first;        // 1 before the first SpMV of a (re)start
     int pad;
     unsigned counter[4];
+    double rr_prev;   // 3D iteration: r.r before the last update
+    int final_kk;     // 3D iteration: the launch that applies the last x update
+    int pad2;
 };
 
 }  // namespace vps
@@ -410,6 +413,9 @@ struct vps_context {
     vps::DBuf<int32_t> nbr_slot;
     // per-node fields, slot-major
     vps::DBuf<float> vrho, vvol, vb, vdiag, vx[2], vr, vp[2], vt, vq, vrowsum;
+    // 3D CG iteration: second r and t (ping-pong with vr, vt) and the
+    // FP64-checked true residual
+    vps::DBuf<float> vr2, vt2, vrt;
     vps::DBuf<double> vdiag64;
     vps::DBuf<int32_t> vrank;
     // lattice bitmask for reference (x-fastest) ranks

@@ -100,7 +100,11 @@ enum Kind {
     K_CG_INIT = 2,     // r = b - M x0; p = r; partial b.b, r.r     (float)
     K_SPMV_FIRST = 3,  // t = M p; partial p.t                       (float)
     K_SPMV_FUSED = 4,  // p = r + beta p_old; t = M p; partial p.t   (float)
-    K_TRUE_RES = 5     // rt = b - M x (double); partial rt.rt      (double)
+    K_TRUE_RES = 5,    // rt = b - M x (double); partial rt.rt      (double)
+    // 3D CG iteration in one launch: the previous update and this SpMV,
+    //   x += alpha p_old; r = r_old - alpha t_old (halo: on the fly);
+    //   p = r + beta p_old; t = M p; partials p.t, r.t, t.t      (float)
+    K_SPMV_UPD = 6
 };
 
 struct StencilArgs {
@@ -115,16 +119,21 @@ struct StencilArgs {
     const double* vdiag64;
     const float* vb;
     const float* in;       // p / x / x0 / in (p_old when fused)
-    const float* in2;      // r for the fused update
+    const float* in2;      // r for the fused update (r_old: SPMV_UPD)
+    const float* in3;      // t_old (SPMV_UPD)
     float* out;            // t / r / out
     float* out2;           // p_new (fused) or p (init)
+    float* out3;           // r_new (SPMV_UPD)
+    float* xio;            // x, updated in place (SPMV_UPD)
     double* out_d;         // diag64
     float* out_f;          // diag32
     float* out_s;          // unclamped row sum s (feature of the learned start)
     double alpha0;
     CgState* st;
     double* part0;         // this kernel's per-block partials (sum 0)
-    double* part1;         // per-block partials (sum 1, init only)
+    double* part1;         // per-block partials (sum 1: init, FIRST, UPD)
+    double* part2;         // per-block partials (sum 2: FIRST, UPD)
+    double* part3;         // per-block partials (sum 3: FIRST, UPD)
     const double* rr1;     // update partials of iteration kk-1
     const double* rr2;     // update partials of iteration kk-2
     int kk;                // iteration index since the (re)start
@@ -273,9 +282,12 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
     __shared__ T tile[kT * kPS];
     __shared__ T psum[kT][4][64];   // plane sums P_k of plane zp, column l
     __shared__ float own[512];      // raw own-brick input (p / x)
+    __shared__ float own_r[KIND == K_SPMV_UPD ? 512 : 1];   // r_new of the brick
     __shared__ uint32_t sbase[27];
-    __shared__ double sred[2][4];
+    __shared__ double sred[4][4];
     __shared__ int s_done;
+    __shared__ int s_final;           // SPMV_UPD: only the last x update
+    __shared__ float s_ab[2];         // SPMV_UPD: alpha_(k-1), beta_k
     const int slot = xcd_remap(blockIdx.x, (int)a.nslots);
     const int t = threadIdx.x;
     const int l = t & 63, wv = t >> 6;
@@ -285,6 +297,17 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
         // convergence of a batch); a speculative true-residual check runs
         // only once the recursive residual has converged
         if (t == 32) s_done = a.st->done;
+    }
+    if constexpr (KIND == K_SPMV_UPD) {
+        // the scalars come from the reduction launched just before (no
+        // kernel of this launch writes the state)
+        if (t == 32) {
+            const CgState* st = a.st;
+            s_done = st->done;
+            s_final = st->done != 0 && st->final_kk == a.kk;
+            s_ab[0] = st->alpha;
+            s_ab[1] = st->beta;
+        }
     }
     const int x = l & 7, y = l >> 3;
     const int64_t base = (int64_t)slot * 512;
@@ -301,6 +324,9 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
     if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_FUSED) {
         if (s_done) return;   // uniform over the block
     }
+    if constexpr (KIND == K_SPMV_UPD) {
+        if (s_done && !s_final) return;   // uniform over the block
+    }
     if constexpr (KIND == K_TRUE_RES) {
         if (a.spec && s_done != 1) return;   // uniform over the block
     }
@@ -308,11 +334,11 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
     // the left brick (x = -4..-1), 1 and 2 the center row, 3 the lower half
     // row of the right brick (x = 8..11).  The raw loads do not depend on
     // beta, so they are issued before the prologue's reductions.
-    float4 ra[kUnitIters], rb[kUnitIters], rv[kUnitIters];
+    float4 ra[kUnitIters], rb[kUnitIters], rv[kUnitIters], rc[kUnitIters], xv[kUnitIters];
 #pragma unroll
     for (int i = 0; i < kUnitIters; i++) {
         const int u = t + kS3Threads * i;
-        ra[i] = rb[i] = rv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        ra[i] = rb[i] = rv[i] = rc[i] = xv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         // units of the first kUnitIters - 1 rounds always exist (t < 256)
         if (i < kUnitIters - 1 || u < kUnits) {
             const int k = u & 3, line = u >> 2;
@@ -326,7 +352,14 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
                 rv[i] = *(const float4*)(a.vvol + g);
             } else {
                 ra[i] = *(const float4*)(a.in + g);
-                if constexpr (KIND == K_SPMV_FUSED) rb[i] = *(const float4*)(a.in2 + g);
+                if constexpr (KIND == K_SPMV_FUSED || KIND == K_SPMV_UPD)
+                    rb[i] = *(const float4*)(a.in2 + g);
+                if constexpr (KIND == K_SPMV_UPD) {
+                    rc[i] = *(const float4*)(a.in3 + g);
+                    if ((k == 1 || k == 2) && yt >= 3 && yt < 11 && zt >= 3 && zt < 11)
+                        xv[i] = *(const float4*)(a.xio + base + 4 * (k - 1) +
+                                                 8 * ((yt - 3) + 8 * (zt - 3)));
+                }
                 if constexpr (OP == VPS_OP_ROW) rv[i] = *(const float4*)(a.vvol + g);
             }
         }
@@ -342,7 +375,7 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
             const int k = u & 3, line = u >> 2;
             const int yt = line % kT, zt = line / kT;
             T v[4];
-            float pn[4];
+            float pn[4], rn[4] = {0.f, 0.f, 0.f, 0.f};
             if constexpr (KIND == K_DIAG) {
                 v[0] = (T)rv[i].x; v[1] = (T)rv[i].y; v[2] = (T)rv[i].z; v[3] = (T)rv[i].w;
                 pn[0] = pn[1] = pn[2] = pn[3] = 0.f;
@@ -351,6 +384,17 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
                 if constexpr (KIND == K_SPMV_FUSED)
                     p = make_float4(rb[i].x + beta * ra[i].x, rb[i].y + beta * ra[i].y,
                                     rb[i].z + beta * ra[i].z, rb[i].w + beta * ra[i].w);
+                if constexpr (KIND == K_SPMV_UPD) {
+                    // every block forms these from the same inputs with the
+                    // same operations, so halo copies equal the owner's
+                    const float al = s_ab[0], be = s_ab[1];
+                    rn[0] = rb[i].x - al * rc[i].x;
+                    rn[1] = rb[i].y - al * rc[i].y;
+                    rn[2] = rb[i].z - al * rc[i].z;
+                    rn[3] = rb[i].w - al * rc[i].w;
+                    p = make_float4(rn[0] + be * ra[i].x, rn[1] + be * ra[i].y,
+                                    rn[2] + be * ra[i].z, rn[3] + be * ra[i].w);
+                }
                 pn[0] = p.x; pn[1] = p.y; pn[2] = p.z; pn[3] = p.w;
                 if constexpr (OP == VPS_OP_ROW) {
                     v[0] = (T)rv[i].x * (T)p.x; v[1] = (T)rv[i].y * (T)p.y;
@@ -369,9 +413,26 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
                     for (int c = 0; c < 4; c++) own[loc + c] = pn[c];
                     if constexpr (KIND == K_SPMV_FUSED)
                         store_nt4(a.out2 + base + loc, pn[0], pn[1], pn[2], pn[3]);
+                    if constexpr (KIND == K_SPMV_UPD) {
+                        // x += alpha p_old (the update of the previous
+                        // iteration; the last one also when the solve ended)
+                        const float al = s_ab[0];
+                        store_nt4(a.xio + base + loc, xv[i].x + al * ra[i].x,
+                                  xv[i].y + al * ra[i].y, xv[i].z + al * ra[i].z,
+                                  xv[i].w + al * ra[i].w);
+                        if (!s_final) {
+#pragma unroll
+                            for (int c = 0; c < 4; c++) own_r[loc + c] = rn[c];
+                            store_nt4(a.out2 + base + loc, pn[0], pn[1], pn[2], pn[3]);
+                            store_nt4(a.out3 + base + loc, rn[0], rn[1], rn[2], rn[3]);
+                        }
+                    }
                 }
             }
         }
+    }
+    if constexpr (KIND == K_SPMV_UPD) {
+        if (s_final) return;   // uniform: the solve ended with this update
     }
     __syncthreads();
     // plane sums, each plane once; column (x, y) sits at tile x = x + 8
@@ -382,7 +443,7 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
         for (int q = 0; q < 4; q++) psum[zp][q][l] = P[q];
     }
     __syncthreads();
-    double red0 = 0, red1 = 0;
+    double red0 = 0, red1 = 0, red2 = 0, red3 = 0;
 #pragma unroll
     for (int q = 0; q < 2; q++) {
         const int z = 2 * wv + q;
@@ -428,21 +489,44 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
                 // t = M p is read by the next kernel only: streaming store
                 __builtin_nontemporal_store(mp, &a.out[g]);
                 red0 += (double)pi * (double)mp;
+                if constexpr (KIND == K_SPMV_UPD) {
+                    const double rv = (double)own_r[loc];
+                    red1 += rv * (double)mp;
+                    red3 += rv * rv;
+                }
+                if constexpr (KIND == K_SPMV_FIRST) red3 += (double)pi * (double)pi;
+                if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_UPD)
+                    red2 += (double)mp * (double)mp;
             }
         }
     }
     if constexpr (KIND != K_DIAG && KIND != K_APPLY) {
+        // SPMV_FIRST also gives r.t (= p.t, r = p at a start), t.t and r.r
+        // for the 3D iteration's reduction (part1..3 set only there)
+        constexpr bool three = KIND == K_SPMV_FIRST || KIND == K_SPMV_UPD;
         red0 = wave_sum_all(red0);
-        if (KIND == K_CG_INIT) red1 = wave_sum_all(red1);
+        if (KIND == K_CG_INIT || KIND == K_SPMV_UPD) red1 = wave_sum_all(red1);
+        if (three) {
+            red2 = wave_sum_all(red2);
+            red3 = wave_sum_all(red3);
+        }
         if (l == 0) {
             sred[0][wv] = red0;
             sred[1][wv] = red1;
+            sred[2][wv] = red2;
+            sred[3][wv] = red3;
         }
         __syncthreads();
         if (t == 0) {
-            a.part0[blockIdx.x] = (sred[0][0] + sred[0][1]) + (sred[0][2] + sred[0][3]);
-            if (KIND == K_CG_INIT)
+            const double s0 = (sred[0][0] + sred[0][1]) + (sred[0][2] + sred[0][3]);
+            a.part0[blockIdx.x] = s0;
+            if (KIND == K_CG_INIT || KIND == K_SPMV_UPD)
                 a.part1[blockIdx.x] = (sred[1][0] + sred[1][1]) + (sred[1][2] + sred[1][3]);
+            if (three && a.part2) {
+                if (KIND == K_SPMV_FIRST) a.part1[blockIdx.x] = s0;
+                a.part2[blockIdx.x] = (sred[2][0] + sred[2][1]) + (sred[2][2] + sred[2][3]);
+                a.part3[blockIdx.x] = (sred[3][0] + sred[3][1]) + (sred[3][2] + sred[3][3]);
+            }
         }
     }
 }
@@ -620,6 +704,7 @@ __global__ void __launch_bounds__(1024) k_cg_start(const double* pbb,
     st->abs0 = rr;
     st->iter = 0;
     st->true_rr = -1.0;
+    st->final_kk = -1;
     // IterativeSolver.h:25 setTolerance(max(tol, eps / ||b||)); Eigen's
     // threshold max(tol^2 ||b||^2, DBL_MIN) (ConjugateGradient.h)
     const double eps_tol = DBL_EPSILON / sqrt(bb);
@@ -629,6 +714,94 @@ __global__ void __launch_bounds__(1024) k_cg_start(const double* pbb,
     if (!isfinite(bb) || !isfinite(rr)) st->done = 3;
     else if (bb == 0.0) st->done = 4;   // Eigen: x = 0, zero iterations
     else st->done = rr < st->thr ? 1 : 0;
+}
+
+// 3D iteration, between two SPMV_UPD launches (one block): the global sums
+// p.t, r.t, t.t and r.r of the SpMV of iteration k (its launch kk; r.r of
+// the stored r_k, formed by that launch), then
+//   alpha_k = r.r / p.t;  r.r after r -= alpha_k t, expanded as
+//   r.r - 2 alpha_k r.t + alpha_k^2 t.t (anchored on the stored r_k every
+//   iteration, so no error carries over); Eigen's test on it; beta_(k+1).
+// The next SPMV_UPD applies alpha_k to x and r, and stops after the x update
+// when this launch ended the solve (final_kk).
+__global__ void __launch_bounds__(1024) k_cg_reduce(const double* ppt, const double* prt,
+                                                    const double* ptt, const double* prr,
+                                                    int n, CgState* st, int kk, int iter_base,
+                                                    int64_t max_iter) {
+    __shared__ double sbuf[16];
+    __shared__ int s_done;
+    if (threadIdx.x == 0) s_done = st->done;
+    __syncthreads();
+    if (s_done) return;   // uniform: an earlier launch ended the solve
+    double a = 0, b = 0, c = 0, d = 0;
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        a += ppt[i];
+        b += prt[i];
+        c += ptt[i];
+        d += prr[i];
+    }
+    const double pt = block_sum<1024>(a, sbuf);
+    const double rt = block_sum<1024>(b, sbuf);
+    const double tt = block_sum<1024>(c, sbuf);
+    const double rr = block_sum<1024>(d, sbuf);
+    if (threadIdx.x != 0) return;
+    const int k = iter_base + kk;   // Eigen's index of this SpMV
+    const double ald = rr / pt;
+    const float al = (float)ald;
+    const double a64 = (double)al;   // the alpha the vectors are updated with
+    const double rn = (rr - 2.0 * a64 * rt) + a64 * a64 * tt;
+    st->alpha = al;
+    st->pt = pt;
+    st->rr_prev = rr;
+    st->rr = rn;
+    st->final_kk = kk + 1;
+    if (!isfinite(ald) || !isfinite(rn)) {
+        st->iter = k;
+        st->final_kk = -1;   // no update with a non-finite alpha
+        st->done = 3;
+    } else if (rn < st->thr) {
+        st->iter = k;
+        st->done = 1;
+    } else if (k + 1 >= max_iter) {
+        st->iter = k + 1;
+        st->done = 2;
+    } else {
+        st->beta = (float)(rn / rr);
+    }
+}
+
+// 3D residual replacement: r (the buffer the next SPMV_UPD reads) takes the
+// FP64-checked true residual rt; partial r.r per block.
+__global__ void __launch_bounds__(kUpdThreads) k_copy_rr(int64_t n4, const float4* rt,
+                                                         float4* r, double* part) {
+    __shared__ double sbuf[kUpdThreads / 64];
+    double rr = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kUpdThreads + threadIdx.x; i < n4;
+         i += (int64_t)kUpdBlocks * kUpdThreads) {
+        const float4 v = rt[i];
+        r[i] = v;
+        rr += ((double)v.x * v.x + (double)v.y * v.y) +
+              ((double)v.z * v.z + (double)v.w * v.w);
+    }
+    rr = block_sum<kUpdThreads>(rr, sbuf);
+    if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+
+// ... and the iteration resumes from the same x: alpha = 0 (x already has
+// the last update), beta = r_true.r_true / r.r before that update.
+__global__ void __launch_bounds__(1024) k_resume(const double* part, CgState* st) {
+    __shared__ double sbuf[16];
+    double a = 0;
+    for (int i = threadIdx.x; i < kUpdBlocks; i += 1024) a += part[i];
+    a = block_sum<1024>(a, sbuf);
+    if (threadIdx.x == 0) {
+        st->alpha = 0.f;
+        st->beta = (float)(a / st->rr_prev);
+        st->rr = a;
+        st->true_rr = -1.0;
+        st->final_kk = -1;
+        st->done = 0;
+    }
 }
 
 // Deterministic one-block sum of n partials into *out.
@@ -698,6 +871,7 @@ __global__ void __launch_bounds__(1024) k_restart(const double* part,
     if (threadIdx.x == 0) {
         st->abs0 = a;
         st->rr = a;
+        st->final_kk = -1;
         st->done = a < st->thr ? 1 : 0;
     }
 }
@@ -885,7 +1059,10 @@ static int read_state(vps_context* c, CgState* out) {
 
 static double ms_between(hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();   // not a failure of the solve
+        return 0.0;
+    }
     return ms;
 }
 
@@ -946,6 +1123,13 @@ int cg_solve(vps_context* c, vps_stats* stats) {
     int iter_base = 0;   // Eigen iterations before the (re)start
     int replacements = 0;
     int predicted = c->prev_iters > 0 ? c->prev_iters + 1 : 8;
+    // 3D: ping-pong vectors of the one-launch iteration; resume_pending: the
+    // next launch is an update after a replacement (no reduction before it)
+    const bool three = c->dim == 3;
+    float* pb[2] = {p0, p1};
+    float* rbuf[2] = {r, c->vr2.p};
+    float* tbuf[2] = {t, c->vt2.p};
+    bool resume_pending = false;
     std::vector<int> timed_iter;
     double host_launch_us = 0;
     while (true) {
@@ -953,7 +1137,61 @@ int cg_solve(vps_context* c, vps_stats* stats) {
             int batch = iter_base + kk < predicted ? predicted - (iter_base + kk) : 4;
             if (batch > 64) batch = 64;
             const auto h0 = std::chrono::steady_clock::now();
-            for (int j = 0; j < batch; j++) {
+            for (int j = 0; j < batch && three; j++) {
+                // 3D: FIRST, then per iteration the one-block reduction of the
+                // previous SpMV and the fused update + SpMV
+                StencilArgs s = base_args(c);
+                s.kk = kk;
+                s.iter_base = iter_base;
+                s.max_iter = max_iter;
+                s.part0 = part_pt(c, 0);
+                s.part1 = part_pt(c, 1);
+                s.part2 = part_pt(c, 2);
+                s.part3 = part_init(c, 1);   // free once k_cg_start has run
+                if (kk == 0) {
+                    s.in = p0;
+                    s.out = t;
+                    launch_op<K_SPMV_FIRST, float, CoefF>(c, s, cf);
+                    spmv_launches++;
+                    kk++;
+                    continue;
+                }
+                // (an update resumed after a replacement has no reduction
+                // before it: not timed)
+                const bool do_time = prof && timed < max_timed && !resume_pending;
+                hipEvent_t* ev = do_time ? &c->kev[4 * timed] : nullptr;
+                if (!resume_pending) {
+                    if (ev)
+                        hipExtLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(1024), 0, c->stream,
+                                              ev[2], ev[3], 0, part_pt(c, 0), part_pt(c, 1),
+                                              part_pt(c, 2), part_init(c, 1), nslots, c->cg.p,
+                                              kk - 1, iter_base, max_iter);
+                    else
+                        hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(1024), 0, c->stream,
+                                           part_pt(c, 0), part_pt(c, 1), part_pt(c, 2),
+                                           part_init(c, 1), nslots, c->cg.p, kk - 1, iter_base,
+                                           max_iter);
+                }
+                resume_pending = false;
+                const int o = (kk - 1) & 1, nw = kk & 1;
+                s.in = pb[o];
+                s.in2 = rbuf[o];
+                s.in3 = tbuf[o];
+                s.out = tbuf[nw];
+                s.out2 = pb[nw];
+                s.out3 = rbuf[nw];
+                s.xio = x;
+                launch_op<K_SPMV_UPD, float, CoefF>(c, s, cf, ev ? ev[0] : nullptr,
+                                                    ev ? ev[1] : nullptr);
+                if (do_time) {
+                    // an iteration of index kk - 1 (its reduction and update)
+                    timed_iter.push_back(iter_base + kk - 1);
+                    timed++;
+                }
+                spmv_launches++;
+                kk++;
+            }
+            for (int j = 0; j < batch && !three; j++) {
                 StencilArgs s = base_args(c);
                 s.kk = kk;
                 s.iter_base = iter_base;
@@ -1002,7 +1240,7 @@ int cg_solve(vps_context* c, vps_stats* stats) {
                 // runs on the device only if this batch converged
                 StencilArgs tr = base_args(c);
                 tr.in = x;
-                tr.out = t;   // r_true
+                tr.out = three ? c->vrt.p : t;   // r_true
                 tr.part0 = part_true(c);
                 tr.spec = 1;
                 launch_op<K_TRUE_RES, double, CoefD>(c, tr, cd);
@@ -1057,7 +1295,24 @@ int cg_solve(vps_context* c, vps_stats* stats) {
         if (!c->check_true) break;
         if (hs.true_rr < hs.thr || replacements >= 4) break;
         replacements++;
-        if (hs.iter > iter_base) {
+        if (three && hs.iter > iter_base) {
+            // residual replacement (van der Vorst and Ye): r takes the true
+            // residual, the direction is kept and the iteration continues
+            // from the same x.  The launch that applied the last update
+            // (kk) runs again with alpha = 0 and beta from the new r.r.
+            kk = hs.iter - iter_base + 1;
+            hipLaunchKernelGGL(k_copy_rr, dim3(kUpdBlocks), dim3(kUpdThreads), 0, c->stream,
+                               n4, (const float4*)c->vrt.p, (float4*)rbuf[(kk - 1) & 1],
+                               part_repl(c));
+            hipLaunchKernelGGL(k_resume, dim3(1), dim3(1024), 0, c->stream, part_repl(c),
+                               c->cg.p);
+            VPS_HIP(c, hipGetLastError());
+            resume_pending = true;
+            hs.done = 0;
+            predicted = iter_base + kk + 2;
+            continue;
+        }
+        if (!three && hs.iter > iter_base) {
             // residual replacement (van der Vorst and Ye): r takes the true
             // residual, the direction is kept and the iteration continues
             // where convergence was declared.  The SpMV that declared it is
@@ -1077,7 +1332,7 @@ int cg_solve(vps_context* c, vps_stats* stats) {
         iter_base = hs.iter;
         kk = 0;
         hipLaunchKernelGGL(k_replace, dim3(kUpdBlocks), dim3(kUpdThreads), 0,
-                           c->stream, n4, (const float4*)t, (float4*)r,
+                           c->stream, n4, (const float4*)(three ? c->vrt.p : t), (float4*)r,
                            (float4*)p0, part_repl(c));
         hipLaunchKernelGGL(k_restart, dim3(1), dim3(1024), 0, c->stream,
                            part_repl(c), c->cg.p);
