@@ -728,11 +728,9 @@ __global__ void __launch_bounds__(1024) k_cg_reduce(const double* ppt, const dou
                                                     const double* ptt, const double* prr,
                                                     int n, CgState* st, int kk, int iter_base,
                                                     int64_t max_iter) {
-    __shared__ double sbuf[16];
-    __shared__ int s_done;
-    if (threadIdx.x == 0) s_done = st->done;
-    __syncthreads();
-    if (s_done) return;   // uniform: an earlier launch ended the solve
+    __shared__ double sred[4][16];
+    // the partial loads do not depend on the flag: issued first; the flag
+    // is the same in every thread (written by an earlier launch)
     double a = 0, b = 0, c = 0, d = 0;
     for (int i = threadIdx.x; i < n; i += 1024) {
         a += ppt[i];
@@ -740,11 +738,33 @@ __global__ void __launch_bounds__(1024) k_cg_reduce(const double* ppt, const dou
         c += ptt[i];
         d += prr[i];
     }
-    const double pt = block_sum<1024>(a, sbuf);
-    const double rt = block_sum<1024>(b, sbuf);
-    const double tt = block_sum<1024>(c, sbuf);
-    const double rr = block_sum<1024>(d, sbuf);
+    if (st->done) return;   // an earlier launch ended the solve
+    // the four sums in one pass: DPP within waves, one barrier, fixed order
+    a = wave_sum_all(a);
+    b = wave_sum_all(b);
+    c = wave_sum_all(c);
+    d = wave_sum_all(d);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        sred[0][wv] = a;
+        sred[1][wv] = b;
+        sred[2][wv] = c;
+        sred[3][wv] = d;
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
+    double sum[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        double s0 = 0, s1 = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i += 2) {
+            s0 += sred[q][i];
+            s1 += sred[q][i + 1];
+        }
+        sum[q] = s0 + s1;
+    }
+    const double pt = sum[0], rt = sum[1], tt = sum[2], rr = sum[3];
     const int k = iter_base + kk;   // Eigen's index of this SpMV
     const double ald = rr / pt;
     const float al = (float)ald;
