@@ -346,6 +346,7 @@ struct CgState {
     int pad;
     unsigned counter[4];
     double rr_prev;   // 3D iteration: r.r before the last update
+    double thr_rec;   // 3D iteration: threshold of the recursive r.r
     int final_kk;     // 3D iteration: the launch that applies the last x update
     int pad2;
 };
@@ -430,6 +431,10 @@ struct vps_context {
     bool generated = false;
     bool check_true = true;
     int prev_iters = 0;
+    // 3D CG: ||b - M x||^2 / (recursive r.r) at the last solve's convergence;
+    // the FP32 recursion runs to tol^2 b.b divided by it (at most 4x) so the
+    // FP64 check usually passes without a replacement
+    double cg_drift = 1.0;
 
     // scratch
     vps::DBuf<double> partials;

@@ -42,6 +42,7 @@
 // convergence the FP64 true residual ||b - M x|| is checked; if it is above
 // the tolerance CG restarts from the true residual (residual replacement).
 #include <cfloat>
+#include <algorithm>
 #include <cmath>
 
 #include <chrono>
@@ -688,7 +689,8 @@ __global__ void __launch_bounds__(kUpdThreads) k_cg_update(
 // CG start: b.b and r0.r0 from the init partials -> tolerance, flags.
 __global__ void __launch_bounds__(1024) k_cg_start(const double* pbb,
                                                    const double* prr, int n,
-                                                   double tol_in, CgState* st) {
+                                                   double tol_in, double rec_scale,
+                                                   CgState* st) {
     __shared__ double sbuf[16];
     double a = 0, b = 0;
     for (int i = threadIdx.x; i < n; i += 1024) {
@@ -711,6 +713,7 @@ __global__ void __launch_bounds__(1024) k_cg_start(const double* pbb,
     const double tol = tol_in > eps_tol ? tol_in : eps_tol;
     const double thr = tol * tol * bb;
     st->thr = thr < DBL_MIN ? DBL_MIN : thr;
+    st->thr_rec = st->thr * rec_scale;
     if (!isfinite(bb) || !isfinite(rr)) st->done = 3;
     else if (bb == 0.0) st->done = 4;   // Eigen: x = 0, zero iterations
     else st->done = rr < st->thr ? 1 : 0;
@@ -779,7 +782,7 @@ __global__ void __launch_bounds__(1024) k_cg_reduce(const double* ppt, const dou
         st->iter = k;
         st->final_kk = -1;   // no update with a non-finite alpha
         st->done = 3;
-    } else if (rn < st->thr) {
+    } else if (rn < st->thr_rec) {
         st->iter = k;
         st->done = 1;
     } else if (k + 1 >= max_iter) {
@@ -1118,8 +1121,12 @@ int cg_solve(vps_context* c, vps_stats* stats) {
         a.part0 = part_init(c, 0);
         a.part1 = part_init(c, 1);
         launch_op<K_CG_INIT, float, CoefF>(c, a, cf);
+        // 3D: the recursion's threshold absorbs the FP32 drift measured at
+        // the last solve (with a 10% margin); the FP64 check keeps tol
+        double rec_scale = 1.0;
+        if (c->dim == 3 && c->cg_drift > 0.9) rec_scale = std::max(0.25, 0.9 / c->cg_drift);
         hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(1024), 0, c->stream,
-                           part_init(c, 0), part_init(c, 1), nslots, tol_in,
+                           part_init(c, 0), part_init(c, 1), nslots, tol_in, rec_scale,
                            c->cg.p);
         VPS_HIP(c, hipGetLastError());
     }
@@ -1313,6 +1320,9 @@ int cg_solve(vps_context* c, vps_stats* stats) {
         // converged on the recursive residual; the true residual of this
         // state was computed right after the batch
         if (!c->check_true) break;
+        if (three && replacements == 0 && hs.rr > 0 && hs.true_rr >= 0 &&
+            std::isfinite(hs.true_rr / hs.rr))
+            c->cg_drift = hs.true_rr / hs.rr;
         if (hs.true_rr < hs.thr || replacements >= 4) break;
         replacements++;
         if (three && hs.iter > iter_base) {
