@@ -281,7 +281,9 @@ static_assert(kS3Threads * (kUnitIters - 1) <= kUnits, "full unit rounds");
 template <int KIND, int OP, typename T, typename C>
 __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
     __shared__ T tile[kT * kPS];
-    __shared__ T psum[kT][4][64];   // plane sums P_k of plane zp, column l
+    // plane sums P_k of plane zp, column l; reused for the 3D iteration's
+    // block partials after the outputs (8-byte aligned for that)
+    __shared__ __align__(16) T psum[kT][4][64];
     __shared__ float own[512];      // raw own-brick input (p / x)
     __shared__ float own_r[KIND == K_SPMV_UPD ? 512 : 1];   // r_new of the brick
     __shared__ uint32_t sbase[27];
@@ -501,10 +503,25 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
             }
         }
     }
-    if constexpr (KIND != K_DIAG && KIND != K_APPLY) {
-        // SPMV_FIRST also gives r.t (= p.t, r = p at a start), t.t and r.r
-        // for the 3D iteration's reduction (part1..3 set only there)
-        constexpr bool three = KIND == K_SPMV_FIRST || KIND == K_SPMV_UPD;
+    if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_UPD) {
+        // four block sums (p.t, r.t, t.t, r.r; r = p after SPMV_FIRST): the
+        // lanes' values through LDS, then wave w reduces sum w -- one DPP
+        // tree per wave instead of four
+        static_assert(sizeof(psum) >= 4 * kS3Threads * sizeof(double), "partials fit psum");
+        __syncthreads();   // every wave has read its plane sums
+        double* sp = reinterpret_cast<double*>(&psum[0][0][0]);
+        sp[t] = red0;
+        sp[kS3Threads + t] = KIND == K_SPMV_FIRST ? red0 : red1;
+        sp[2 * kS3Threads + t] = red2;
+        sp[3 * kS3Threads + t] = red3;
+        __syncthreads();
+        const double* q = sp + kS3Threads * wv;
+        double v = (q[l] + q[l + 64]) + (q[l + 128] + q[l + 192]);
+        v = wave_sum_all(v);
+        double* dst = wv == 0 ? a.part0 : (wv == 1 ? a.part1 : (wv == 2 ? a.part2 : a.part3));
+        if (l == 0 && dst) dst[blockIdx.x] = v;
+    } else if constexpr (KIND != K_DIAG && KIND != K_APPLY) {
+        constexpr bool three = false;
         red0 = wave_sum_all(red0);
         if (KIND == K_CG_INIT || KIND == K_SPMV_UPD) red1 = wave_sum_all(red1);
         if (three) {
