@@ -709,8 +709,16 @@ __global__ void __launch_bounds__(1024) k_cg_start(const double* pbb,
                                                    double tol_in, double rec_scale,
                                                    CgState* st) {
     __shared__ double sbuf[16];
-    double a = 0, b = 0;
-    for (int i = threadIdx.x; i < n; i += 1024) {
+    double va[4], vb[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {   // loads issued together
+        const int i = threadIdx.x + 1024 * u;
+        va[u] = i < n ? pbb[i] : 0.0;
+        vb[u] = i < n ? prr[i] : 0.0;
+    }
+    double a = (va[0] + va[1]) + (va[2] + va[3]);
+    double b = (vb[0] + vb[1]) + (vb[2] + vb[3]);
+    for (int i = threadIdx.x + 4096; i < n; i += 1024) {
         a += pbb[i];
         b += prr[i];
     }
@@ -749,16 +757,32 @@ __global__ void __launch_bounds__(1024) k_cg_reduce(const double* ppt, const dou
                                                     int n, CgState* st, int kk, int iter_base,
                                                     int64_t max_iter) {
     __shared__ double sred[4][16];
-    // the partial loads do not depend on the flag: issued first; the flag
-    // is the same in every thread (written by an earlier launch)
-    double a = 0, b = 0, c = 0, d = 0;
-    for (int i = threadIdx.x; i < n; i += 1024) {
+    // the partial loads do not depend on the flag: issued first, the first
+    // four per thread all at once (one memory round trip for up to 4,096
+    // blocks); the flag is the same in every thread (written by an earlier
+    // launch)
+    const int done = st->done;
+    double va[4], vb[4], vc[4], vd[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int i = threadIdx.x + 1024 * u;
+        const bool in = i < n;
+        va[u] = in ? ppt[i] : 0.0;
+        vb[u] = in ? prt[i] : 0.0;
+        vc[u] = in ? ptt[i] : 0.0;
+        vd[u] = in ? prr[i] : 0.0;
+    }
+    double a = (va[0] + va[1]) + (va[2] + va[3]);
+    double b = (vb[0] + vb[1]) + (vb[2] + vb[3]);
+    double c = (vc[0] + vc[1]) + (vc[2] + vc[3]);
+    double d = (vd[0] + vd[1]) + (vd[2] + vd[3]);
+    for (int i = threadIdx.x + 4096; i < n; i += 1024) {
         a += ppt[i];
         b += prt[i];
         c += ptt[i];
         d += prr[i];
     }
-    if (st->done) return;   // an earlier launch ended the solve
+    if (done) return;   // an earlier launch ended the solve
     // the four sums in one pass: DPP within waves, one barrier, fixed order
     a = wave_sum_all(a);
     b = wave_sum_all(b);
@@ -851,8 +875,14 @@ __global__ void __launch_bounds__(1024) k_sum_partials(const double* p, int n,
                                                        double* out, const CgState* st) {
     __shared__ double sbuf[16];
     if (st && st->done != 1) return;   // same value in every thread
-    double a = 0;
-    for (int i = threadIdx.x; i < n; i += 1024) a += p[i];
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {   // loads issued together
+        const int i = threadIdx.x + 1024 * u;
+        v[u] = i < n ? p[i] : 0.0;
+    }
+    double a = (v[0] + v[1]) + (v[2] + v[3]);
+    for (int i = threadIdx.x + 4096; i < n; i += 1024) a += p[i];
     a = block_sum<1024>(a, sbuf);
     if (threadIdx.x == 0) *out = a;
 }
