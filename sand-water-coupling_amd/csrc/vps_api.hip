@@ -168,6 +168,7 @@ void free_all(vps_context* c) {
         dfree(c->p_by_rank[b]);
     }
     dfree(c->rrho);
+    dfree(c->rrinv);
     dfree(c->rofs);
     dfree(c->bofs);
     dfree(c->vel_next);
@@ -217,6 +218,7 @@ int ensure_particle_capacity(vps_context* c, int64_t n) {
         VPS_TRY(dalloc(c, c->rid[b], (size_t)n));
     }
     VPS_TRY(dalloc(c, c->rrho, (size_t)n));
+    VPS_TRY(dalloc(c, c->rrinv, (size_t)n));
     VPS_TRY(dalloc(c, c->rofs, (size_t)n));
     const int64_t m = std::max(n, c->nb);
     VPS_TRY(dalloc(c, c->cell_of, (size_t)m));

@@ -135,8 +135,8 @@ __global__ void __launch_bounds__(256) k_density(const double4* pos,
 // 0) take the FP64 distance.
 __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const float4* ofs,
                                                    const int32_t* cstart, int64_t n,
-                                                   KConst k, float* rho, float4* vel,
-                                                   int set_volume) {
+                                                   KConst k, float* rho, float* rinv,
+                                                   float4* vel, int set_volume) {
     const int nb = (int)((n + 255) / 256);
     const int64_t i = (int64_t)xcd_remap(blockIdx.x, nb) * 256 + threadIdx.x;
     if (i >= n) return;
@@ -189,6 +189,7 @@ __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const flo
     }
     const float r = k.mass * sum;
     rho[i] = r;
+    rinv[i] = 1.f / r;
     if (set_volume) vel[i].w = k.mass / r;
 }
 
@@ -334,7 +335,7 @@ int real_density(vps_context* c, bool set_volume) {
                            c->stream, c->rpos[s].p, n, c->k, c->rofs.p);
         hipLaunchKernelGGL(k_density3f, dim3(blocks(n, 256)), dim3(256), 0,
                            c->stream, c->rpos[s].p, c->rofs.p, c->cell_start.p, n, c->k,
-                           c->rrho.p, c->rvel[s].p, set_volume ? 1 : 0);
+                           c->rrho.p, c->rrinv.p, c->rvel[s].p, set_volume ? 1 : 0);
     } else
         hipLaunchKernelGGL(k_density<2>, dim3(blocks(n, 256)), dim3(256), 0,
                            c->stream, c->rpos[s].p, c->cell_start.p, n, c->k,

@@ -370,6 +370,7 @@ struct vps_context {
     vps::DBuf<int32_t> rid[2];
     int cur = 0;
     vps::DBuf<float> rrho;
+    vps::DBuf<float> rrinv;   // 1 / rrho (3D, for the staged virtual sums)
     // sorted order: FP32 offset of each particle from its cell corner, w =
     // +-1/rho (negative: clamped into a border cell, exact test only)
     vps::DBuf<float4> rofs;

@@ -379,11 +379,15 @@ struct BoundaryWalkStaged {
 
 __global__ void __launch_bounds__(512) k_virtual_sums3s(
     KConst k, const int32_t* brick_of_slot, const int32_t* nslots_p,
-    const double4* rpos, const float4* rvel, const float* rrho,
+    const double4* rpos, const float4* rvel, const float* rrho, const float* rrinv,
     const float4* rofs, const int32_t* rcstart, const double4* bpos,
     const float4* bofs, const float4* bvel, const float4* bnrm, const int32_t* bcstart,
     float* vrho, float* vvol, float* vb, uint32_t* bits) {
-    __shared__ float4 s_p[kStageCap];   // (x_j - C_j, +-1/rho_j; < 0: exact only)
+    // x_j - C_clo (NaN: exact test only), y_j - C_j, z_j - C_j, x_j - C_j, as
+    // separate arrays: x from the row's first cell for the membership test
+    // (which reads 12 bytes per candidate as three 4-byte loads), and from
+    // its own cell for the sums
+    __shared__ float s_px[kStageCap], s_py[kStageCap], s_pz[kStageCap], s_pw[kStageCap];
     __shared__ int s_cs[kStageRows][kStageCx + 1];  // real cell starts (staged)
     __shared__ int s_bcs[kStageRows][kStageCx + 1]; // boundary cell starts
     __shared__ int s_rg[kStageRows];    // global - staged index
@@ -450,10 +454,11 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
         const int total = s_meta[6];
         for (int e0 = 0; e0 < total; e0 += 4 * 512) {
             float4 pj[4];
-            float rj[4];
+            int ci[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int e = e0 + u * 512 + t;
+                ci[u] = 0;
                 if (e < total) {
                     // row of staged element e: last r with s_off[r] <= e
                     int lo = 0, hi = nrows - 1;
@@ -463,15 +468,25 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
                         else hi = mid - 1;
                     }
                     pj[u] = rofs[e + s_rg[lo]];
-                    rj[u] = rrho[e + s_rg[lo]];
+                    // its cell within the row (x = clo + ci)
+                    for (int i = 1; i < ncx; i++) ci[u] += s_cs[lo][i] <= e;
                 }
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int e = e0 + u * 512 + t;
                 if (e < total) {
-                    const float ir = 1.f / rj[u];
-                    s_p[e] = make_float4(pj[u].x, pj[u].y, pj[u].z, pj[u].w < 0.f ? -ir : ir);
+                    // x relative to the corner of the row's first cell
+                    // (C_(clo+ci) - C_clo = ci cells), rounded once from
+                    // FP64; a particle marked for the exact test only gets
+                    // x = NaN, which no FP32 test passes
+                    const float xr = pj[u].w < 0.f
+                                         ? __builtin_nanf("")
+                                         : (float)((double)pj[u].x + (double)ci[u] * k.cell_size);
+                    s_px[e] = xr;
+                    s_py[e] = pj[u].y;
+                    s_pz[e] = pj[u].z;
+                    s_pw[e] = pj[u].x;
                 }
             }
         }
@@ -504,58 +519,84 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
     const int xlo = max(cc0 - 1, 0), xhi = min(cc0 + 1, k.cdims[0] - 1);
     const int zlo = max(cc2 - 1, 0), zhi = min(cc2 + 1, k.cdims[2] - 1);
     const int ylo = max(cc1 - 1, 0), yhi = min(cc1 + 1, k.cdims[1] - 1);
-    // the (up to) three cells xlo, xlo + 1, xlo + 2 of a row are one
-    // contiguous staged range; x offsets kept in registers (otherwise the
-    // compiler selects the cell index per candidate and redoes the FP64)
+    // The (up to) three cells xlo .. xhi of a row are one contiguous staged
+    // range.  The membership test takes x relative to the same corner C_clo
+    // for all of them (one offset per node; its larger rounding stays inside
+    // the FP64 band); a member's sums use its accurate offset from its own
+    // cell corner with that cell's q - C.
+    const float tx = (float)(qx - cell_corner(k, 0, s_meta[0]));
     float tx0 = (float)(qx - cell_corner(k, 0, xlo));
     float tx1 = (float)(qx - cell_corner(k, 0, xlo + 1));
     float tx2 = (float)(qx - cell_corner(k, 0, xlo + 2));
     asm volatile("" : "+v"(tx0), "+v"(tx1), "+v"(tx2));
     const int i0 = xlo - s_meta[0], i1 = xhi - s_meta[0];
+    // h2_min < d2f < h2f_lo as one unsigned test on the bits (NaN fails).
+    // Below h2_min (the band's width, far above the error of d2f near 0)
+    // the exact test decides, so a fast member's accurate d2f is > 0.
+    const uint32_t lo_min = __float_as_uint(k.h2f_hi - k.h2f_lo);
+    const uint32_t lo_span = __float_as_uint(k.h2f_lo) - lo_min - 1u;
+    // Cells whose nearest point lies beyond h are skipped: a whole row by
+    // its y and z distance, the end cells of a row by their x distance.
+    // Only interior cells are culled (a border cell may hold particles
+    // clamped into it from outside its extent), with a margin of 1e-4 h^2
+    // far above the FP32 error of these distances.
+    const float csf = (float)k.cell_size;
+    const float h2c = (float)k.h2 * 1.0001f;
+    auto gap = [csf](float t) { return fmaxf(fmaxf(-t, t - csf), 0.f); };   // q - C -> distance
+    const float txh = xhi - xlo == 2 ? tx2 : (xhi - xlo == 1 ? tx1 : tx0);
+    const bool cut_l = xlo < xhi && xlo > 0, cut_r = xhi > xlo && xhi < k.cdims[0] - 1;
+    const float gl = gap(tx0), gr = gap(txh);
+    const float dl2 = cut_l ? gl * gl : 1e30f, dr2 = cut_r ? gr * gr : 1e30f;
     for (int z = zlo; z <= zhi; z++) {
         const float tz = (float)(qz - cell_corner(k, 2, z));
+        const float gz = z > 0 && z < k.cdims[2] - 1 ? gap(tz) : 0.f;
         for (int y = ylo; y <= yhi; y++) {
             const float ty = (float)(qy - cell_corner(k, 1, y));
+            const float gy = y > 0 && y < k.cdims[1] - 1 ? gap(ty) : 0.f;
+            const float rem = h2c - gy * gy - gz * gz;
+            if (!(rem > 0.f)) continue;
             const int r = (z - s_meta[2]) * ncy + (y - s_meta[1]);
             const int rg = s_rg[r];
-            const int jb = s_cs[r][i0], je = s_cs[r][i1 + 1];
+            const int jb = s_cs[r][i0 + (dl2 >= rem)], je = s_cs[r][i1 + 1 - (dr2 >= rem)];
             const int b1 = s_cs[r][min(i0 + 1, i1 + 1)];
             const int b2 = s_cs[r][min(i0 + 2, i1 + 1)];
             // Two passes per chunk of 32 candidates: the cheap FP32 test
             // sets one bit per member, then each lane visits only its own
             // members (a lane has ~15% of its candidates within h, so the
             // wave no longer runs the accumulation for every candidate).
+            // Bits enter at the bottom: candidate c0 + i sits at bit
+            // c1 - 1 - (c0 + i), so the highest bit is the next in order.
             for (int c0 = jb; c0 < je; c0 += 32) {
                 const int c1 = min(c0 + 32, je);
                 uint32_t mfast = 0u, mexact = 0u;
                 for (int j = c0; j < c1; j++) {
-                    const float4 pr = s_p[j];
-                    const float tx = j < b1 ? tx0 : (j < b2 ? tx1 : tx2);
-                    const float dx = tx - pr.x, dy = ty - pr.y, dz = tz - pr.z;
+                    const float dx = tx - s_px[j], dy = ty - s_py[j], dz = tz - s_pz[j];
                     const float d2f = dx * dx + dy * dy + dz * dz;
-                    const Pretest t = pretest(k, d2f, pr.w);
-                    mfast |= (uint32_t)t.fast << (j - c0);
-                    mexact |= (uint32_t)t.exact << (j - c0);
+                    const bool fast = __float_as_uint(d2f) - lo_min - 1u < lo_span;
+                    const bool exact = !(d2f >= k.h2f_hi) & !fast;   // NaN: exact
+                    mfast = mfast + mfast + (uint32_t)fast;
+                    mexact = mexact + mexact + (uint32_t)exact;
                 }
                 while (mfast) {
-                    const int j = c0 + __builtin_ctz(mfast);
-                    mfast &= mfast - 1u;
-                    const float4 pr = s_p[j];
-                    const float tx = j < b1 ? tx0 : (j < b2 ? tx1 : tx2);
-                    const float dx = tx - pr.x, dy = ty - pr.y, dz = tz - pr.z;
+                    const int p = 31 - __builtin_clz(mfast);
+                    mfast ^= 1u << p;
+                    const int j = c1 - 1 - p;
+                    const float txc = j < b1 ? tx0 : (j < b2 ? tx1 : tx2);
+                    const float dx = txc - s_pw[j], dy = ty - s_py[j], dz = tz - s_pz[j];
                     const float d2f = dx * dx + dy * dy + dz * dz;
-                    a.add_fast<3>(k, d2f, dx, dy, dz, rvel[j + rg], pr.w);
+                    a.add_fast<3>(k, d2f, dx, dy, dz, rvel[j + rg], rrinv[j + rg]);
                 }
                 while (mexact) {
                     // the reference's FP64 test and the accurate spline
-                    const int j = c0 + __builtin_ctz(mexact);
-                    mexact &= mexact - 1u;
+                    const int p = 31 - __builtin_clz(mexact);
+                    mexact ^= 1u << p;
+                    const int j = c1 - 1 - p;
                     const int g = j + rg;
                     const double4 pj = rpos[g];
                     const double d2 = sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz);
                     if (d2 < k.h2)
                         a.add<3>(k, d2, (float)(qx - pj.x), (float)(qy - pj.y),
-                                 (float)(qz - pj.z), rvel[g], fabsf(s_p[j].w));
+                                 (float)(qz - pj.z), rvel[g], rrinv[g]);
                 }
             }
         }
@@ -676,7 +717,7 @@ static int generate_slots(vps_context* c, int w, int64_t grid) {
     if (D == 3)
         hipLaunchKernelGGL(k_virtual_sums3s, dim3((unsigned)grid),
                            dim3(512), 0, c->stream, c->k, c->slot_brick[w].p,
-                           nsl, c->rpos[rs].p, c->rvel[rs].p, c->rrho.p, c->rofs.p,
+                           nsl, c->rpos[rs].p, c->rvel[rs].p, c->rrho.p, c->rrinv.p, c->rofs.p,
                            c->cell_start.p, c->bpos.p, c->bofs.p, c->bvel.p, c->bnrm.p,
                            c->bcell_start.p, c->vrho.p, c->vvol.p, c->vb.p,
                            c->bits.p);
