@@ -90,7 +90,7 @@ def log(*a):
 
 
 # the one-launch 3D CG iteration (update of the previous iteration + SpMV)
-SPMV_KERNEL = "k_stencil3<SPMV_UPD, TRANSPOSE>"
+SPMV_KERNEL = "k_cg_iter3<TRANSPOSE>"
 SPMV_BYTES = 40.0
 
 

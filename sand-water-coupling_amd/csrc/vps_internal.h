@@ -348,8 +348,13 @@ struct CgState {
     double rr_prev;   // 3D iteration: r.r before the last update
     double thr_rec;   // 3D iteration: threshold of the recursive r.r
     int final_kk;     // 3D iteration: the launch that applies the last x update
-    int pad2;
+    // 3D iteration launches: 0 while running; kk + 1 of the launch that
+    // ended the solve; kStopBefore when it ended before any launch (read by
+    // the iteration launches only: one word, so a block never mixes a stop
+    // declared in its own launch with one from an earlier launch)
+    int stop;
 };
+constexpr int kStopBefore = 0x7fffffff;
 
 }  // namespace vps
 

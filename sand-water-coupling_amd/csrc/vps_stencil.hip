@@ -34,9 +34,11 @@
 //                        t = M p; partial p.t per block
 //             update(k): alpha = rr_{k-1} / p.t; x += alpha p;
 //                        r -= alpha t; partial r.r per block
+// 3D: one launch per iteration (k_cg_iter3, persistent blocks) forms the
+// sums of the previous launch's partials, applies the update and the SpMV.
 // Every grid-wide sum is formed by the CONSUMING kernel: its blocks re-sum
-// the producer's per-block partials in a fixed order (a ring of three
-// iterations).  No device-scope fence or atomic is issued inside the loop --
+// the producer's per-block partials in a fixed order (2D: a ring of three
+// iterations; 3D: two alternating sets).  No device-scope fence or atomic is issued inside the loop --
 // on a multi-XCD part an agent-scope release writes back the L2 -- and the
 // result is bitwise reproducible.  FP32 vectors, FP64 dots and norms.  At
 // convergence the FP64 true residual ||b - M x|| is checked; if it is above
@@ -73,6 +75,7 @@ constexpr int kT2 = kT * kT;
 constexpr int kHaloIters2 = (kT2 + 63) / 64;   // 4
 constexpr int kUpdBlocks = 256;                // grid of the CG update
 constexpr int kUpdThreads = 1024;
+constexpr int kIterBlocks = 1024;                // persistent blocks of the 3D iteration
 
 // 2D tile element e -> (neighbour brick 0..8) << 9 | local node; elements
 // past the tile map to the zero brick slot 9.  (3D decodes arithmetically.)
@@ -101,11 +104,8 @@ enum Kind {
     K_CG_INIT = 2,     // r = b - M x0; p = r; partial b.b, r.r     (float)
     K_SPMV_FIRST = 3,  // t = M p; partial p.t                       (float)
     K_SPMV_FUSED = 4,  // p = r + beta p_old; t = M p; partial p.t   (float)
-    K_TRUE_RES = 5,    // rt = b - M x (double); partial rt.rt      (double)
-    // 3D CG iteration in one launch: the previous update and this SpMV,
-    //   x += alpha p_old; r = r_old - alpha t_old (halo: on the fly);
-    //   p = r + beta p_old; t = M p; partials p.t, r.t, t.t      (float)
-    K_SPMV_UPD = 6
+    K_TRUE_RES = 5     // rt = b - M x (double); partial rt.rt      (double)
+    // (the 3D CG iteration is k_cg_iter3)
 };
 
 struct StencilArgs {
@@ -120,12 +120,12 @@ struct StencilArgs {
     const double* vdiag64;
     const float* vb;
     const float* in;       // p / x / x0 / in (p_old when fused)
-    const float* in2;      // r for the fused update (r_old: SPMV_UPD)
-    const float* in3;      // t_old (SPMV_UPD)
+    const float* in2;      // r for the fused update (r_old: 3D iteration)
+    const float* in3;      // t_old (3D iteration)
     float* out;            // t / r / out
     float* out2;           // p_new (fused) or p (init)
-    float* out3;           // r_new (SPMV_UPD)
-    float* xio;            // x, updated in place (SPMV_UPD)
+    float* out3;           // r_new (3D iteration)
+    float* xio;            // x, updated in place (3D iteration)
     double* out_d;         // diag64
     float* out_f;          // diag32
     float* out_s;          // unclamped row sum s (feature of the learned start)
@@ -281,16 +281,13 @@ static_assert(kS3Threads * (kUnitIters - 1) <= kUnits, "full unit rounds");
 template <int KIND, int OP, typename T, typename C>
 __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
     __shared__ T tile[kT * kPS];
-    // plane sums P_k of plane zp, column l; reused for the 3D iteration's
+    // plane sums P_k of plane zp, column l; reused for the first SpMV's
     // block partials after the outputs (8-byte aligned for that)
     __shared__ __align__(16) T psum[kT][4][64];
     __shared__ float own[512];      // raw own-brick input (p / x)
-    __shared__ float own_r[KIND == K_SPMV_UPD ? 512 : 1];   // r_new of the brick
     __shared__ uint32_t sbase[27];
     __shared__ double sred[4][4];
     __shared__ int s_done;
-    __shared__ int s_final;           // SPMV_UPD: only the last x update
-    __shared__ float s_ab[2];         // SPMV_UPD: alpha_(k-1), beta_k
     const int slot = xcd_remap(blockIdx.x, (int)a.nslots);
     const int t = threadIdx.x;
     const int l = t & 63, wv = t >> 6;
@@ -300,17 +297,6 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
         // convergence of a batch); a speculative true-residual check runs
         // only once the recursive residual has converged
         if (t == 32) s_done = a.st->done;
-    }
-    if constexpr (KIND == K_SPMV_UPD) {
-        // the scalars come from the reduction launched just before (no
-        // kernel of this launch writes the state)
-        if (t == 32) {
-            const CgState* st = a.st;
-            s_done = st->done;
-            s_final = st->done != 0 && st->final_kk == a.kk;
-            s_ab[0] = st->alpha;
-            s_ab[1] = st->beta;
-        }
     }
     const int x = l & 7, y = l >> 3;
     const int64_t base = (int64_t)slot * 512;
@@ -327,9 +313,6 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
     if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_FUSED) {
         if (s_done) return;   // uniform over the block
     }
-    if constexpr (KIND == K_SPMV_UPD) {
-        if (s_done && !s_final) return;   // uniform over the block
-    }
     if constexpr (KIND == K_TRUE_RES) {
         if (a.spec && s_done != 1) return;   // uniform over the block
     }
@@ -337,11 +320,11 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
     // the left brick (x = -4..-1), 1 and 2 the center row, 3 the lower half
     // row of the right brick (x = 8..11).  The raw loads do not depend on
     // beta, so they are issued before the prologue's reductions.
-    float4 ra[kUnitIters], rb[kUnitIters], rv[kUnitIters], rc[kUnitIters], xv[kUnitIters];
+    float4 ra[kUnitIters], rb[kUnitIters], rv[kUnitIters];
 #pragma unroll
     for (int i = 0; i < kUnitIters; i++) {
         const int u = t + kS3Threads * i;
-        ra[i] = rb[i] = rv[i] = rc[i] = xv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        ra[i] = rb[i] = rv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         // units of the first kUnitIters - 1 rounds always exist (t < 256)
         if (i < kUnitIters - 1 || u < kUnits) {
             const int k = u & 3, line = u >> 2;
@@ -355,14 +338,7 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
                 rv[i] = *(const float4*)(a.vvol + g);
             } else {
                 ra[i] = *(const float4*)(a.in + g);
-                if constexpr (KIND == K_SPMV_FUSED || KIND == K_SPMV_UPD)
-                    rb[i] = *(const float4*)(a.in2 + g);
-                if constexpr (KIND == K_SPMV_UPD) {
-                    rc[i] = *(const float4*)(a.in3 + g);
-                    if ((k == 1 || k == 2) && yt >= 3 && yt < 11 && zt >= 3 && zt < 11)
-                        xv[i] = *(const float4*)(a.xio + base + 4 * (k - 1) +
-                                                 8 * ((yt - 3) + 8 * (zt - 3)));
-                }
+                if constexpr (KIND == K_SPMV_FUSED) rb[i] = *(const float4*)(a.in2 + g);
                 if constexpr (OP == VPS_OP_ROW) rv[i] = *(const float4*)(a.vvol + g);
             }
         }
@@ -378,7 +354,7 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
             const int k = u & 3, line = u >> 2;
             const int yt = line % kT, zt = line / kT;
             T v[4];
-            float pn[4], rn[4] = {0.f, 0.f, 0.f, 0.f};
+            float pn[4];
             if constexpr (KIND == K_DIAG) {
                 v[0] = (T)rv[i].x; v[1] = (T)rv[i].y; v[2] = (T)rv[i].z; v[3] = (T)rv[i].w;
                 pn[0] = pn[1] = pn[2] = pn[3] = 0.f;
@@ -387,17 +363,6 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
                 if constexpr (KIND == K_SPMV_FUSED)
                     p = make_float4(rb[i].x + beta * ra[i].x, rb[i].y + beta * ra[i].y,
                                     rb[i].z + beta * ra[i].z, rb[i].w + beta * ra[i].w);
-                if constexpr (KIND == K_SPMV_UPD) {
-                    // every block forms these from the same inputs with the
-                    // same operations, so halo copies equal the owner's
-                    const float al = s_ab[0], be = s_ab[1];
-                    rn[0] = rb[i].x - al * rc[i].x;
-                    rn[1] = rb[i].y - al * rc[i].y;
-                    rn[2] = rb[i].z - al * rc[i].z;
-                    rn[3] = rb[i].w - al * rc[i].w;
-                    p = make_float4(rn[0] + be * ra[i].x, rn[1] + be * ra[i].y,
-                                    rn[2] + be * ra[i].z, rn[3] + be * ra[i].w);
-                }
                 pn[0] = p.x; pn[1] = p.y; pn[2] = p.z; pn[3] = p.w;
                 if constexpr (OP == VPS_OP_ROW) {
                     v[0] = (T)rv[i].x * (T)p.x; v[1] = (T)rv[i].y * (T)p.y;
@@ -416,26 +381,9 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
                     for (int c = 0; c < 4; c++) own[loc + c] = pn[c];
                     if constexpr (KIND == K_SPMV_FUSED)
                         store_nt4(a.out2 + base + loc, pn[0], pn[1], pn[2], pn[3]);
-                    if constexpr (KIND == K_SPMV_UPD) {
-                        // x += alpha p_old (the update of the previous
-                        // iteration; the last one also when the solve ended)
-                        const float al = s_ab[0];
-                        store_nt4(a.xio + base + loc, xv[i].x + al * ra[i].x,
-                                  xv[i].y + al * ra[i].y, xv[i].z + al * ra[i].z,
-                                  xv[i].w + al * ra[i].w);
-                        if (!s_final) {
-#pragma unroll
-                            for (int c = 0; c < 4; c++) own_r[loc + c] = rn[c];
-                            store_nt4(a.out2 + base + loc, pn[0], pn[1], pn[2], pn[3]);
-                            store_nt4(a.out3 + base + loc, rn[0], rn[1], rn[2], rn[3]);
-                        }
-                    }
                 }
             }
         }
-    }
-    if constexpr (KIND == K_SPMV_UPD) {
-        if (s_final) return;   // uniform: the solve ended with this update
     }
     __syncthreads();
     // plane sums, each plane once; column (x, y) sits at tile x = x + 8
@@ -492,18 +440,13 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
                 // t = M p is read by the next kernel only: streaming store
                 __builtin_nontemporal_store(mp, &a.out[g]);
                 red0 += (double)pi * (double)mp;
-                if constexpr (KIND == K_SPMV_UPD) {
-                    const double rv = (double)own_r[loc];
-                    red1 += rv * (double)mp;
-                    red3 += rv * rv;
-                }
                 if constexpr (KIND == K_SPMV_FIRST) red3 += (double)pi * (double)pi;
-                if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_UPD)
+                if constexpr (KIND == K_SPMV_FIRST)
                     red2 += (double)mp * (double)mp;
             }
         }
     }
-    if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_UPD) {
+    if constexpr (KIND == K_SPMV_FIRST) {
         // four block sums (p.t, r.t, t.t, r.r; r = p after SPMV_FIRST): the
         // lanes' values through LDS, then wave w reduces sum w -- one DPP
         // tree per wave instead of four
@@ -523,7 +466,7 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
     } else if constexpr (KIND != K_DIAG && KIND != K_APPLY) {
         constexpr bool three = false;
         red0 = wave_sum_all(red0);
-        if (KIND == K_CG_INIT || KIND == K_SPMV_UPD) red1 = wave_sum_all(red1);
+        if (KIND == K_CG_INIT) red1 = wave_sum_all(red1);
         if (three) {
             red2 = wave_sum_all(red2);
             red3 = wave_sum_all(red3);
@@ -538,7 +481,7 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
         if (t == 0) {
             const double s0 = (sred[0][0] + sred[0][1]) + (sred[0][2] + sred[0][3]);
             a.part0[blockIdx.x] = s0;
-            if (KIND == K_CG_INIT || KIND == K_SPMV_UPD)
+            if (KIND == K_CG_INIT)
                 a.part1[blockIdx.x] = (sred[1][0] + sred[1][1]) + (sred[1][2] + sred[1][3]);
             if (three && a.part2) {
                 if (KIND == K_SPMV_FIRST) a.part1[blockIdx.x] = s0;
@@ -546,6 +489,268 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
                 a.part3[blockIdx.x] = (sred[3][0] + sred[3][1]) + (sred[3][2] + sred[3][3]);
             }
         }
+    }
+}
+
+// 3D CG iteration with persistent blocks (G <= kIterBlocks, a multiple of 8
+// when G >= 8, every block owning bricks blockIdx.x, + G, ...).  Each block
+// first forms the global sums of the previous launch's block partials itself
+// -- the same loads, order and arithmetic in every block, so every block
+// reaches the same alpha_(kk-1) = r.r / p.t, the r.r after the update
+// (expanded as r.r - 2 alpha r.t + alpha^2 t.t, anchored every iteration on
+// the direct r.r of the stored r), Eigen's test on it and beta_kk
+// without a reduction launch in between (about 1K partials per set, one
+// round of loads).  Then per brick: x += alpha p_old, r = r_old - alpha t_old,
+// p = r + beta p_old, t = M p with k_stencil3's tile and plane sums, and one partial
+// per block of p.t, r.t, t.t and r.r.  The launch that ends the solve
+// applies only the last x update; block 0 records the state.
+struct IterArgs {
+    const double* prev[4];   // the previous launch's partials (p.t, r.t, t.t, r.r)
+    double* out[4];          // this launch's partials, one per block
+    int nprev;               // their count
+    int resume;              // alpha, beta from the state (after a replacement)
+};
+
+template <int OP>
+__global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc, IterArgs it) {
+    __shared__ float tile[kT * kPS];
+    __shared__ __align__(16) float psum[kT][4][64];
+    __shared__ float own[512];
+    __shared__ float own_r[512];
+    __shared__ uint32_t sbase[27];
+    __shared__ double sred[4][4];
+    __shared__ float s_ab[2];
+    __shared__ int s_mode;
+    CgState* st = a.st;
+    const int kk = a.kk;
+    {
+        // a solve ended by an earlier launch (or before any): nothing to do
+        const int stop = __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (stop != 0 && stop != kk + 1) return;   // uniform over the grid
+    }
+    const int t = threadIdx.x;
+    const int l = t & 63, wv = t >> 6;
+    const int x = l & 7, y = l >> 3;
+    const int nsl = (int)a.nslots;
+    const int G = (int)gridDim.x;
+    auto slot_of = [nsl, G](int v) { return (G & 7) == 0 ? xcd_remap(v, nsl) : v; };
+    int slot = slot_of(blockIdx.x);
+    if (t < 27) sbase[t] = (uint32_t)a.nbr_slot[(int64_t)slot * 27 + t] * 512u;
+    __syncthreads();
+    // halo loads of a brick (see k_stencil3) and its epilogue operands
+    float4 ra[kUnitIters], rb[kUnitIters], rv[kUnitIters], rc[kUnitIters], xv[kUnitIters];
+    float Vz[2], Dz[2];
+    int64_t base = (int64_t)slot * 512;
+    auto issue = [&]() {
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int64_t g = base + x + 8 * y + 64 * (2 * wv + q);
+            Vz[q] = a.vvol[g];
+            Dz[q] = a.vdiag[g];
+        }
+#pragma unroll
+        for (int i = 0; i < kUnitIters; i++) {
+            const int u = t + kS3Threads * i;
+            ra[i] = rb[i] = rv[i] = rc[i] = xv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (i < kUnitIters - 1 || u < kUnits) {
+                const int k = u & 3, line = u >> 2;
+                const int yt = line % kT, zt = line / kT;
+                const int yy = yt + 5, zz = zt + 5;
+                const int bxo = k == 0 ? 0 : (k == 3 ? 2 : 1);
+                const int nb = (zz >> 3) * 9 + (yy >> 3) * 3 + bxo;
+                const int xl = (k == 0 || k == 2) ? 4 : 0;
+                const uint32_t g = sbase[nb] + (uint32_t)(xl + 8 * ((yy & 7) + 8 * (zz & 7)));
+                ra[i] = *(const float4*)(a.in + g);
+                rb[i] = *(const float4*)(a.in2 + g);
+                rc[i] = *(const float4*)(a.in3 + g);
+                if ((k == 1 || k == 2) && yt >= 3 && yt < 11 && zt >= 3 && zt < 11)
+                    xv[i] = *(const float4*)(a.xio + base + 4 * (k - 1) +
+                                             8 * ((yt - 3) + 8 * (zt - 3)));
+                if constexpr (OP == VPS_OP_ROW) rv[i] = *(const float4*)(a.vvol + g);
+            }
+        }
+    };
+    // alpha_(kk-1), beta_kk
+    if (it.resume) {
+        if (t == 0) {
+            s_mode = 0;
+            s_ab[0] = st->alpha;
+            s_ab[1] = st->beta;
+        }
+    } else {
+        double sv[4] = {0, 0, 0, 0};
+        for (int i0 = 0; i0 < it.nprev; i0 += 4 * kS3Threads) {
+            double v4[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int i = i0 + t + kS3Threads * u;
+#pragma unroll
+                for (int q = 0; q < 4; q++) v4[q][u] = i < it.nprev ? it.prev[q][i] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) sv[q] += (v4[q][0] + v4[q][1]) + (v4[q][2] + v4[q][3]);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) sv[q] = wave_sum_all(sv[q]);
+        if (l == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) sred[q][wv] = sv[q];
+        }
+        __syncthreads();
+        if (t == 0) {
+            double sum[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) sum[q] = (sred[q][0] + sred[q][1]) + (sred[q][2] + sred[q][3]);
+            const double pt = sum[0], rt = sum[1], tt = sum[2], rr = sum[3];
+            const int k = a.iter_base + kk - 1;   // Eigen's index of the reduced SpMV
+            const double ald = rr / pt;
+            const float al = (float)ald;
+            const double a64 = (double)al;   // the alpha the vectors are updated with
+            const double rn = (rr - 2.0 * a64 * rt) + a64 * a64 * tt;
+            int done = 0, iter = 0;
+            float be = 0.f;
+            if (!isfinite(ald) || !isfinite(rn)) {
+                done = 3;
+                iter = k;
+            } else if (rn < st->thr_rec) {
+                done = 1;
+                iter = k;
+            } else if (k + 1 >= a.max_iter) {
+                done = 2;
+                iter = k + 1;
+            } else {
+                be = (float)(rn / rr);
+            }
+            s_mode = done == 0 ? 0 : (done == 3 ? 2 : 1);
+            s_ab[0] = al;
+            s_ab[1] = be;
+            if (blockIdx.x == 0) {
+                st->alpha = al;
+                st->pt = pt;
+                st->rr_prev = rr;
+                st->rr = rn;
+                if (done) {
+                    st->iter = iter;
+                    st->done = done;
+                    __hip_atomic_store(&st->stop, kk + 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                } else {
+                    st->beta = be;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int mode = s_mode;
+    const float al = s_ab[0], be = s_ab[1];
+    if (mode == 2) return;   // non-finite alpha: no update
+    if (mode == 1) {
+        // the solve ended: the last x update only (own nodes of every brick)
+        for (int v = blockIdx.x; v < nsl; v += G) {
+            const int64_t b0 = (int64_t)slot_of(v) * 512;
+            for (int e = t; e < 512; e += kS3Threads)
+                a.xio[b0 + e] += al * a.in[b0 + e];
+        }
+        return;
+    }
+    double red0 = 0, red1 = 0, red2 = 0, red3 = 0;
+    for (int v = blockIdx.x; v < nsl; v += G) {
+        if (v != (int)blockIdx.x) {   // uniform over the block
+            slot = slot_of(v);
+            base = (int64_t)slot * 512;
+            __syncthreads();   // the previous brick's LDS has been read
+            if (t < 27) sbase[t] = (uint32_t)a.nbr_slot[(int64_t)slot * 27 + t] * 512u;
+            __syncthreads();
+        }
+        issue();
+#pragma unroll
+        for (int i = 0; i < kUnitIters; i++) {
+            const int u = t + kS3Threads * i;
+            if (i < kUnitIters - 1 || u < kUnits) {
+                const int k = u & 3, line = u >> 2;
+                const int yt = line % kT, zt = line / kT;
+                float rn[4], pn[4];
+                rn[0] = rb[i].x - al * rc[i].x;
+                rn[1] = rb[i].y - al * rc[i].y;
+                rn[2] = rb[i].z - al * rc[i].z;
+                rn[3] = rb[i].w - al * rc[i].w;
+                pn[0] = rn[0] + be * ra[i].x;
+                pn[1] = rn[1] + be * ra[i].y;
+                pn[2] = rn[2] + be * ra[i].z;
+                pn[3] = rn[3] + be * ra[i].w;
+                float* dst = &tile[zt * kPS + yt * kRS + 4 + 4 * k];
+                if constexpr (OP == VPS_OP_ROW) {
+                    dst[0] = rv[i].x * pn[0];
+                    dst[1] = rv[i].y * pn[1];
+                    dst[2] = rv[i].z * pn[2];
+                    dst[3] = rv[i].w * pn[3];
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 4; c++) dst[c] = pn[c];
+                }
+                if ((k == 1 || k == 2) && yt >= 3 && yt < 11 && zt >= 3 && zt < 11) {
+                    const int loc = 4 * (k - 1) + 8 * ((yt - 3) + 8 * (zt - 3));
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        own[loc + c] = pn[c];
+                        own_r[loc + c] = rn[c];
+                    }
+                    store_nt4(a.xio + base + loc, xv[i].x + al * ra[i].x,
+                              xv[i].y + al * ra[i].y, xv[i].z + al * ra[i].z,
+                              xv[i].w + al * ra[i].w);
+                    store_nt4(a.out2 + base + loc, pn[0], pn[1], pn[2], pn[3]);
+                    store_nt4(a.out3 + base + loc, rn[0], rn[1], rn[2], rn[3]);
+                }
+            }
+        }
+        __syncthreads();
+        for (int zp = wv; zp < kT; zp += 4) {
+            float P[4];
+            plane4<kRS, float>(&tile[zp * kPS + (y + 3) * kRS + (x + 8)], cc, P);
+#pragma unroll
+            for (int q = 0; q < 4; q++) psum[zp][q][l] = P[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int z = 2 * wv + q;
+            float acc = 0.f;
+            acc += psum[z][3][l];
+            acc += psum[z + 1][2][l];
+            acc += psum[z + 2][1][l];
+            acc += psum[z + 3][0][l];
+            acc += psum[z + 4][1][l];
+            acc += psum[z + 5][2][l];
+            acc += psum[z + 6][3][l];
+            const int loc = x + 8 * y + 64 * z;
+            const float V = Vz[q], Dg = Dz[q];
+            const float pi = own[loc];
+            const float stv = OP == VPS_OP_ROW ? acc : V * acc;
+            const float mp = V > 0.f ? Dg * pi + stv : 0.f;
+            __builtin_nontemporal_store(mp, &a.out[base + loc]);
+            const double rvv = (double)own_r[loc];
+            red0 += (double)pi * (double)mp;
+            red1 += rvv * (double)mp;
+            red2 += (double)mp * (double)mp;
+            red3 += rvv * rvv;
+        }
+    }
+    // this block's partials: DPP within waves, the waves in order
+    red0 = wave_sum_all(red0);
+    red1 = wave_sum_all(red1);
+    red2 = wave_sum_all(red2);
+    red3 = wave_sum_all(red3);
+    __syncthreads();   // sred of the reduction above has been read
+    if (l == 0) {
+        sred[0][wv] = red0;
+        sred[1][wv] = red1;
+        sred[2][wv] = red2;
+        sred[3][wv] = red3;
+    }
+    __syncthreads();
+    if (t < 4) {
+        double* dst = t == 0 ? it.out[0] : (t == 1 ? it.out[1] : (t == 2 ? it.out[2] : it.out[3]));
+        dst[blockIdx.x] = (sred[t][0] + sred[t][1]) + (sred[t][2] + sred[t][3]);
     }
 }
 
@@ -742,99 +947,10 @@ __global__ void __launch_bounds__(1024) k_cg_start(const double* pbb,
     if (!isfinite(bb) || !isfinite(rr)) st->done = 3;
     else if (bb == 0.0) st->done = 4;   // Eigen: x = 0, zero iterations
     else st->done = rr < st->thr ? 1 : 0;
+    st->stop = st->done ? kStopBefore : 0;
 }
 
-// 3D iteration, between two SPMV_UPD launches (one block): the global sums
-// p.t, r.t, t.t and r.r of the SpMV of iteration k (its launch kk; r.r of
-// the stored r_k, formed by that launch), then
-//   alpha_k = r.r / p.t;  r.r after r -= alpha_k t, expanded as
-//   r.r - 2 alpha_k r.t + alpha_k^2 t.t (anchored on the stored r_k every
-//   iteration, so no error carries over); Eigen's test on it; beta_(k+1).
-// The next SPMV_UPD applies alpha_k to x and r, and stops after the x update
-// when this launch ended the solve (final_kk).
-__global__ void __launch_bounds__(1024) k_cg_reduce(const double* ppt, const double* prt,
-                                                    const double* ptt, const double* prr,
-                                                    int n, CgState* st, int kk, int iter_base,
-                                                    int64_t max_iter) {
-    __shared__ double sred[4][16];
-    // the partial loads do not depend on the flag: issued first, the first
-    // four per thread all at once (one memory round trip for up to 4,096
-    // blocks); the flag is the same in every thread (written by an earlier
-    // launch)
-    const int done = st->done;
-    double va[4], vb[4], vc[4], vd[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const int i = threadIdx.x + 1024 * u;
-        const bool in = i < n;
-        va[u] = in ? ppt[i] : 0.0;
-        vb[u] = in ? prt[i] : 0.0;
-        vc[u] = in ? ptt[i] : 0.0;
-        vd[u] = in ? prr[i] : 0.0;
-    }
-    double a = (va[0] + va[1]) + (va[2] + va[3]);
-    double b = (vb[0] + vb[1]) + (vb[2] + vb[3]);
-    double c = (vc[0] + vc[1]) + (vc[2] + vc[3]);
-    double d = (vd[0] + vd[1]) + (vd[2] + vd[3]);
-    for (int i = threadIdx.x + 4096; i < n; i += 1024) {
-        a += ppt[i];
-        b += prt[i];
-        c += ptt[i];
-        d += prr[i];
-    }
-    if (done) return;   // an earlier launch ended the solve
-    // the four sums in one pass: DPP within waves, one barrier, fixed order
-    a = wave_sum_all(a);
-    b = wave_sum_all(b);
-    c = wave_sum_all(c);
-    d = wave_sum_all(d);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) {
-        sred[0][wv] = a;
-        sred[1][wv] = b;
-        sred[2][wv] = c;
-        sred[3][wv] = d;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    double sum[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        double s0 = 0, s1 = 0;
-#pragma unroll
-        for (int i = 0; i < 16; i += 2) {
-            s0 += sred[q][i];
-            s1 += sred[q][i + 1];
-        }
-        sum[q] = s0 + s1;
-    }
-    const double pt = sum[0], rt = sum[1], tt = sum[2], rr = sum[3];
-    const int k = iter_base + kk;   // Eigen's index of this SpMV
-    const double ald = rr / pt;
-    const float al = (float)ald;
-    const double a64 = (double)al;   // the alpha the vectors are updated with
-    const double rn = (rr - 2.0 * a64 * rt) + a64 * a64 * tt;
-    st->alpha = al;
-    st->pt = pt;
-    st->rr_prev = rr;
-    st->rr = rn;
-    st->final_kk = kk + 1;
-    if (!isfinite(ald) || !isfinite(rn)) {
-        st->iter = k;
-        st->final_kk = -1;   // no update with a non-finite alpha
-        st->done = 3;
-    } else if (rn < st->thr_rec) {
-        st->iter = k;
-        st->done = 1;
-    } else if (k + 1 >= max_iter) {
-        st->iter = k + 1;
-        st->done = 2;
-    } else {
-        st->beta = (float)(rn / rr);
-    }
-}
-
-// 3D residual replacement: r (the buffer the next SPMV_UPD reads) takes the
+// 3D residual replacement: r (the buffer the next iteration launch reads) takes the
 // FP64-checked true residual rt; partial r.r per block.
 __global__ void __launch_bounds__(kUpdThreads) k_copy_rr(int64_t n4, const float4* rt,
                                                          float4* r, double* part) {
@@ -865,6 +981,7 @@ __global__ void __launch_bounds__(1024) k_resume(const double* part, CgState* st
         st->true_rr = -1.0;
         st->final_kk = -1;
         st->done = 0;
+        st->stop = 0;
     }
 }
 
@@ -943,6 +1060,7 @@ __global__ void __launch_bounds__(1024) k_restart(const double* part,
         st->rr = a;
         st->final_kk = -1;
         st->done = a < st->thr ? 1 : 0;
+        st->stop = st->done ? kStopBefore : 0;
     }
 }
 
@@ -1020,9 +1138,13 @@ static double* part_repl(vps_context* c) {
 static double* part_misc(vps_context* c) {
     return c->partials.p + 6 * c->nbricks + 4 * kUpdBlocks;
 }
+// the second set of the 3D iteration's partials (the first: the SpMV ring)
+static double* part_iter(vps_context* c, int q) {
+    return c->partials.p + 6 * c->nbricks + 4 * kUpdBlocks + 2048 + (int64_t)q * kIterBlocks;
+}
 
 int64_t partials_size(int64_t nbricks) {
-    return 6 * nbricks + 4 * kUpdBlocks + 2048;
+    return 6 * nbricks + 4 * kUpdBlocks + 2048 + 4 * kIterBlocks;
 }
 
 static StencilArgs base_args(vps_context* c) {
@@ -1204,6 +1326,10 @@ int cg_solve(vps_context* c, vps_stats* stats) {
     float* rbuf[2] = {r, c->vr2.p};
     float* tbuf[2] = {t, c->vt2.p};
     bool resume_pending = false;
+    // 3D: persistent blocks of the iteration and its two partial sets
+    const int giter = nslots >= 8 ? (std::min(nslots, kIterBlocks) & ~7) : nslots;
+    double* setA[4] = {part_pt(c, 0), part_pt(c, 1), part_pt(c, 2), part_init(c, 1)};
+    double* setB[4] = {part_iter(c, 0), part_iter(c, 1), part_iter(c, 2), part_iter(c, 3)};
     std::vector<int> timed_iter;
     double host_launch_us = 0;
     while (true) {
@@ -1212,8 +1338,8 @@ int cg_solve(vps_context* c, vps_stats* stats) {
             if (batch > 64) batch = 64;
             const auto h0 = std::chrono::steady_clock::now();
             for (int j = 0; j < batch && three; j++) {
-                // 3D: FIRST, then per iteration the one-block reduction of the
-                // previous SpMV and the fused update + SpMV
+                // 3D: FIRST, then per iteration one launch of persistent
+                // blocks: the sums of the previous SpMV, the update, the SpMV
                 StencilArgs s = base_args(c);
                 s.kk = kk;
                 s.iter_base = iter_base;
@@ -1230,23 +1356,9 @@ int cg_solve(vps_context* c, vps_stats* stats) {
                     kk++;
                     continue;
                 }
-                // (an update resumed after a replacement has no reduction
-                // before it: not timed)
+                // (an update resumed after a replacement: not timed)
                 const bool do_time = prof && timed < max_timed && !resume_pending;
                 hipEvent_t* ev = do_time ? &c->kev[4 * timed] : nullptr;
-                if (!resume_pending) {
-                    if (ev)
-                        hipExtLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(1024), 0, c->stream,
-                                              ev[2], ev[3], 0, part_pt(c, 0), part_pt(c, 1),
-                                              part_pt(c, 2), part_init(c, 1), nslots, c->cg.p,
-                                              kk - 1, iter_base, max_iter);
-                    else
-                        hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(1024), 0, c->stream,
-                                           part_pt(c, 0), part_pt(c, 1), part_pt(c, 2),
-                                           part_init(c, 1), nslots, c->cg.p, kk - 1, iter_base,
-                                           max_iter);
-                }
-                resume_pending = false;
                 const int o = (kk - 1) & 1, nw = kk & 1;
                 s.in = pb[o];
                 s.in2 = rbuf[o];
@@ -1255,8 +1367,31 @@ int cg_solve(vps_context* c, vps_stats* stats) {
                 s.out2 = pb[nw];
                 s.out3 = rbuf[nw];
                 s.xio = x;
-                launch_op<K_SPMV_UPD, float, CoefF>(c, s, cf, ev ? ev[0] : nullptr,
-                                                    ev ? ev[1] : nullptr);
+                // partials: FIRST writes set A (one per slot); launch kk
+                // writes set B when kk is odd, A when even, and reads the other
+                IterArgs ia;
+                for (int q = 0; q < 4; q++) {
+                    ia.out[q] = nw ? setB[q] : setA[q];
+                    ia.prev[q] = nw ? setA[q] : setB[q];
+                }
+                ia.nprev = kk == 1 ? nslots : giter;
+                ia.resume = resume_pending ? 1 : 0;
+                resume_pending = false;
+                const dim3 gi((unsigned)giter), bi(kS3Threads);
+                if (c->p.op_mode == VPS_OP_ROW) {
+                    if (ev)
+                        hipExtLaunchKernelGGL(k_cg_iter3<VPS_OP_ROW>, gi, bi, 0, c->stream, ev[0],
+                                              ev[1], 0, s, cf, ia);
+                    else
+                        hipLaunchKernelGGL(k_cg_iter3<VPS_OP_ROW>, gi, bi, 0, c->stream, s, cf, ia);
+                } else {
+                    if (ev)
+                        hipExtLaunchKernelGGL(k_cg_iter3<VPS_OP_TRANSPOSE>, gi, bi, 0, c->stream,
+                                              ev[0], ev[1], 0, s, cf, ia);
+                    else
+                        hipLaunchKernelGGL(k_cg_iter3<VPS_OP_TRANSPOSE>, gi, bi, 0, c->stream, s,
+                                           cf, ia);
+                }
                 if (do_time) {
                     // an iteration of index kk - 1 (its reduction and update)
                     timed_iter.push_back(iter_base + kk - 1);
@@ -1423,7 +1558,8 @@ int cg_solve(vps_context* c, vps_stats* stats) {
     for (int64_t j = 0; j < timed; j++) {
         if (timed_iter[(size_t)j] > hs.iter) break;   // no-op launches
         spmv_ms += ms_between(c->kev[4 * j], c->kev[4 * j + 1]);
-        upd_ms += ms_between(c->kev[4 * j + 2], c->kev[4 * j + 3]);
+        // 3D: the update is part of the iteration launch
+        if (!three) upd_ms += ms_between(c->kev[4 * j + 2], c->kev[4 * j + 3]);
         counted++;
     }
     if (stats) {
