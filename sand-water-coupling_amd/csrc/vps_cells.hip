@@ -133,10 +133,14 @@ __global__ void __launch_bounds__(256) k_density(const double4* pos,
 // so the FP32 distance decides which terms enter (a pair within rounding of
 // h adds nothing measurable); particles clamped into a border cell (ofs.w <
 // 0) take the FP64 distance.
+// BND: the boundary density (own searcher, DualParticleSphLiquid.cpp:96-97)
+// into nrm.w, V_b = m / rho_b into vel.w; otherwise rho and 1 / rho, and
+// V = m / rho into vel.w when set_volume.
+template <bool BND>
 __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const float4* ofs,
                                                    const int32_t* cstart, int64_t n,
                                                    KConst k, float* rho, float* rinv,
-                                                   float4* vel, int set_volume) {
+                                                   float4* nrm, float4* vel, int set_volume) {
     const int nb = (int)((n + 255) / 256);
     const int64_t i = (int64_t)xcd_remap(blockIdx.x, nb) * 256 + threadIdx.x;
     if (i >= n) return;
@@ -188,9 +192,14 @@ __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const flo
         }
     }
     const float r = k.mass * sum;
-    rho[i] = r;
-    rinv[i] = 1.f / r;
-    if (set_volume) vel[i].w = k.mass / r;
+    if constexpr (BND) {
+        nrm[i].w = r;
+        vel[i].w = k.mass / r;
+    } else {
+        rho[i] = r;
+        rinv[i] = 1.f / r;
+        if (set_volume) vel[i].w = k.mass / r;
+    }
 }
 
 // FP32 offset of every (sorted) particle from its cell corner; w = +1, or
@@ -333,9 +342,10 @@ int real_density(vps_context* c, bool set_volume) {
     if (c->dim == 3) {
         hipLaunchKernelGGL(k_cell_offsets3, dim3(blocks(n, 256)), dim3(256), 0,
                            c->stream, c->rpos[s].p, n, c->k, c->rofs.p);
-        hipLaunchKernelGGL(k_density3f, dim3(blocks(n, 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_density3f<false>, dim3(blocks(n, 256)), dim3(256), 0,
                            c->stream, c->rpos[s].p, c->rofs.p, c->cell_start.p, n, c->k,
-                           c->rrho.p, c->rrinv.p, c->rvel[s].p, set_volume ? 1 : 0);
+                           c->rrho.p, c->rrinv.p, (float4*)nullptr, c->rvel[s].p,
+                           set_volume ? 1 : 0);
     } else
         hipLaunchKernelGGL(k_density<2>, dim3(blocks(n, 256)), dim3(256), 0,
                            c->stream, c->rpos[s].p, c->cell_start.p, n, c->k,
@@ -348,10 +358,14 @@ int boundary_density(vps_context* c) {
     const int64_t nb = c->nb;
     if (c->dim == 3)
     {
-        hipLaunchKernelGGL(k_bdensity<3>, dim3(blocks(nb, 256)), dim3(256), 0,
-                           c->stream, c->bpos.p, c->bcell_start.p, nb, c->k,
-                           c->bnrm.p, c->bvel.p);
+        // FP32 cell offsets first (w = +-1), the density walk as for the real
+        // particles, then the offsets with w = +-1 / rho_b for the virtual sums
         VPS_TRY(dalloc(c, c->bofs, (size_t)nb));
+        hipLaunchKernelGGL(k_cell_offsets3, dim3(blocks(nb, 256)), dim3(256), 0,
+                           c->stream, c->bpos.p, nb, c->k, c->bofs.p);
+        hipLaunchKernelGGL(k_density3f<true>, dim3(blocks(nb, 256)), dim3(256), 0,
+                           c->stream, c->bpos.p, c->bofs.p, c->bcell_start.p, nb, c->k,
+                           (float*)nullptr, (float*)nullptr, c->bnrm.p, c->bvel.p, 0);
         hipLaunchKernelGGL(k_boundary_offsets3, dim3(blocks(nb, 256)), dim3(256), 0,
                            c->stream, c->bpos.p, c->bnrm.p, nb, c->k, c->bofs.p);
     }
