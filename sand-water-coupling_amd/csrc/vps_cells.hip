@@ -354,6 +354,15 @@ int real_density(vps_context* c, bool set_volume) {
     return VPS_OK;
 }
 
+// FP32 cell offsets of the (sorted) particles (3D).
+int real_offsets(vps_context* c) {
+    const int64_t n = c->n;
+    hipLaunchKernelGGL(k_cell_offsets3, dim3(blocks(n, 256)), dim3(256), 0, c->stream,
+                       c->rpos[c->cur].p, n, c->k, c->rofs.p);
+    VPS_HIP(c, hipGetLastError());
+    return VPS_OK;
+}
+
 int boundary_density(vps_context* c) {
     const int64_t nb = c->nb;
     if (c->dim == 3)

@@ -516,6 +516,7 @@ int virtual_generate(vps_context* c);
 int virtual_diag(vps_context* c, bool set_alpha0);
 int cg_solve(vps_context* c, vps_stats* st);
 int64_t partials_size(int64_t nbricks);
+int real_offsets(vps_context* c);
 int apply_operator_f32(vps_context* c, const float* in, float* out,
                        int op_mode);
 int pressure_gradient(vps_context* c);
