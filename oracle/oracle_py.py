@@ -106,6 +106,7 @@ def lib():
     pr("or_kernel_d1", C.c_double, [C.c_int32, C.c_double, C.c_double])
     pr("or_packed_kernel_sum", C.c_double, [C.c_int32, C.c_double])
     pr("or_hash_key", C.c_int64, [C.c_int32, I32, C.c_int64])
+    pr("or_set_mutation", None, [C.c_int32])
     _lib = L
     return L
 
@@ -323,6 +324,32 @@ def cg_csr(rowptr, col, val, b, x0=None, tol=1e-6, max_iter=-1):
     rc = L.or_cg_csr(n, _i64(rowptr), _i32(col), _d(val), _d(b), _d(x), tol,
                      max_iter, C.byref(it), C.byref(err))
     return x, it.value, err.value, rc
+
+
+# Planted mistakes (or_set_mutation): each check of the pressure gradient and
+# the Laplacian must fail on the ones it targets.
+MUT_NONE = 0
+MUT_GRAD_SIGN = 1
+MUT_GRAD_NO_VOL = 2
+MUT_GRAD_KERNEL_H = 3
+MUT_COEF_HALF = 4
+MUT_COEF_SIGN = 5
+MUT_COEF_KERNEL_H = 6
+
+
+class mutation:
+    """Context manager planting mistake m in the oracle for its body."""
+
+    def __init__(self, m):
+        self.m = m
+
+    def __enter__(self):
+        lib().or_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_mutation(MUT_NONE)
+        return False
 
 
 def kernel(dim, radius, dist):

@@ -12,9 +12,15 @@
  *   hash searcher            brute force (same sets), hand-computed keys
  *   densities / sums         brute-force O(N^2) sums, lattice invariants
  *   virtual generation       brute force over all nodes, threshold cases
- *   Laplacian assembly       interior row sums, A vs A^T on a tiny system
+ *   Laplacian assembly       interior row sums, A vs A^T on a tiny system,
+ *                            the quadratic pressure |x|^2/2 (interior
+ *                            A p = -Dim V_I / s^Dim: sign and scale)
  *   CG                       dense solves, 1D Poisson, stopping rule
- *   pressure gradient        constant pressure, hand-built boundary cases
+ *   pressure gradient        constant pressure, hand-built boundary cases,
+ *                            a linear pressure (interior dv = -grad p),
+ *                            one pressurised node (radial push out to H)
+ * or_set_mutation() plants one of a list of plausible transcription
+ * mistakes, so the tests can show that each of these checks fails on it.
  * The last-bit FP64 values of the reference depend on Eigen's SIMD reduction
  * order inside CG (dot/squaredNorm over long vectors); that part is "parity
  * unpinned" -- only compile of the reference against Eigen would fix it and
@@ -40,6 +46,19 @@
 #define OR_INV_PI 0.31830988618379067154
 #define OR_SQRT2 1.41421356237309504880
 #define OR_SQRT3 1.73205080756887729353
+
+/* Planted mistakes (tests only; 0 = the reference's formulas). */
+enum {
+    OR_MUT_NONE = 0,
+    OR_MUT_GRAD_SIGN = 1,      /* virtual gradient term with the wrong sign */
+    OR_MUT_GRAD_NO_VOL = 2,    /* ... without V_J */
+    OR_MUT_GRAD_KERNEL_H = 3,  /* ... with the real kernel h instead of H */
+    OR_MUT_COEF_HALF = 4,      /* alpha_IJ without the factor 2 */
+    OR_MUT_COEF_SIGN = 5,      /* alpha_IJ with the wrong sign */
+    OR_MUT_COEF_KERNEL_H = 6   /* alpha_IJ from W'_h instead of W'_H */
+};
+static int g_mutation = OR_MUT_NONE;
+void or_set_mutation(int32_t m) { g_mutation = m; }
 
 static double now_s(void) {
     struct timespec t;
@@ -622,8 +641,9 @@ static void cb_coef(void* c, int32_t J, const double* pJ) {
     coef_ctx* t = (coef_ctx*)c;
     double d[3] = {t->q[0] - pJ[0], t->q[1] - pJ[1], t->q[2] - pJ[2]};
     const double r_ij = sqrt(sqnorm(t->s->dim, d));
-    const double alpha_ij =
-        2. * t->s->virt.vol[J] * kern_dW(&t->s->virt.k, r_ij) / (r_ij + 1e-6);
+    const or_kern* kk = g_mutation == OR_MUT_COEF_KERNEL_H ? &t->s->real.k : &t->s->virt.k;
+    const double two = g_mutation == OR_MUT_COEF_HALF ? 1. : (g_mutation == OR_MUT_COEF_SIGN ? -2. : 2.);
+    const double alpha_ij = two * t->s->virt.vol[J] * kern_dW(kk, r_ij) / (r_ij + 1e-6);
     t->sum -= alpha_ij;
     row_buf* rb = t->rb;
     if (rb->n == rb->cap) {
@@ -864,8 +884,10 @@ static void cb_grad_virtual(void* c, int32_t J, const double* pJ) {
     const or_sim* s = t->s;
     double d[3] = {t->q[0] - pJ[0], t->q[1] - pJ[1], t->q[2] - pJ[2]};
     double g[3];
-    kern_grad(&s->virt.k, d, g);
-    const double sc = s->virt.vol[J] * s->pres[J];
+    kern_grad(g_mutation == OR_MUT_GRAD_KERNEL_H ? &s->real.k : &s->virt.k, d, g);
+    double sc = s->virt.vol[J] * s->pres[J];
+    if (g_mutation == OR_MUT_GRAD_SIGN) sc = -sc;
+    else if (g_mutation == OR_MUT_GRAD_NO_VOL) sc = s->pres[J];
     for (int a = 0; a < s->dim; a++) t->f[a] += sc * g[a];
 }
 /* force -= V_b * min(n_b.(v_i - v_b), 0) * n_b * kernel_b(p_i - p_b)

@@ -94,6 +94,8 @@ int64_t or_matrix_nnz(or_sim* s);
 int or_get_matrix(or_sim* s, int64_t* rowptr, int32_t* col, double* val);
 /* out = M p, FP64, in reference virtual order */
 int or_apply_operator(or_sim* s, const double* p, double* out);
+/* Plants mistake m in the oracle's formulas (tests only; 0 clears). */
+void or_set_mutation(int32_t m);
 
 /* Eigen ConjugateGradient restated on a CSR matrix (identity
  * preconditioner).  Returns 0 on success, 2 on no convergence. */
