@@ -42,7 +42,9 @@ enum vps_status {
                                   reference divides by zero there,
                                   ParticlesNearbySearcher.h:71 with .cpp:13) */
     VPS_ERR_NO_CONVERGENCE = 2,/* CG hit max_iterations (reference: exit(-1)) */
-    VPS_ERR_NONFINITE = 3,     /* NaN/Inf in the right-hand side or solution */
+    VPS_ERR_NONFINITE = 3,     /* NaN/Inf in the right-hand side or solution
+                                  (a non-finite warm start is not an error:
+                                  the solve falls back to the plain start) */
     VPS_ERR_OOM = 4,           /* device allocation failed */
     VPS_ERR_HIP = 5,           /* any other HIP runtime error */
     VPS_ERR_STATE = 6,         /* call order violated (e.g. project before
@@ -145,7 +147,11 @@ typedef struct vps_stats {
     int64_t spmv_launches;
     int64_t reserved[4];       /* [0] residual replacements of the last solve,
                                   [1] host microseconds spent launching its CG
-                                  kernels */
+                                  kernels, [2] 1 when the warm start gave a
+                                  non-finite initial residual (b finite) and
+                                  the solve ran again from the plain start
+                                  (the index-carried pressures; zero when
+                                  that start was the one that failed) */
 } vps_stats;
 
 typedef struct vps_context vps_context;
@@ -458,7 +464,8 @@ int vps_mlp_forward(vps_context* ctx, int64_t n, const float* features,
 
 /* Loss of the last in-situ Adam step (mean squared error of the scaled
  * pressure change on its batch, before the step) and the number of Adam
- * steps taken (1 after vps_mlp_set_parameters on an untrained network). */
+ * steps taken.  A step whose loss or reduced gradient is non-finite is
+ * skipped (parameters and moments unchanged) and not counted. */
 int vps_mlp_training_state(vps_context* ctx, double* last_loss, int64_t* steps);
 
 /* Per-virtual-particle features of the current step in reference order

@@ -504,6 +504,7 @@ int vps_set_particles(vps_context* c, int64_t n, const double* x,
     } else {
         c->n = n;
         c->initialized = false;
+        c->cg_drift = 1.0;   // a new particle set: no drift measured on it yet
         VPS_TRY(ensure_particle_capacity(c, n));
         c->cur = 0;
         hipLaunchKernelGGL(k_upload, dim3(blocks(n, 256)), dim3(256), 0,
@@ -582,6 +583,7 @@ int vps_initialize(vps_context* c, vps_stats* st) {
     c->last_solve_buf = -1;
     c->last_solve_nvirt = 0;
     c->prev_iters = 0;
+    c->cg_drift = 1.0;
     c->nvirt = 0;
     VPS_TRY(virtual_generate(c));
     VPS_TRY(virtual_diag(c, true));
@@ -636,7 +638,22 @@ static int project_impl(vps_context* c, vps_stats* st, int e0) {
     }
     vps_stats local{};
     vps_stats* s = st ? st : &local;
-    const int rc = cg_solve(c, s);
+    int rc = cg_solve(c, s);
+    if (rc == kStartNonfinite) {
+        // The start gave a non-finite residual while b is finite (a learned
+        // or extrapolated x0 that overflowed): solve again from the plain
+        // start, the reference's index-carried pressures (zero when that was
+        // the start already), and count the event.
+        const int plain = mode == VPS_WARM_INDEX_CARRY ? VPS_WARM_ZERO : VPS_WARM_INDEX_CARRY;
+        VPS_TRY(warm_start(c, plain, x0));
+        c->start_fallbacks++;
+        rc = cg_solve(c, s);
+        s->reserved[2] = 1;
+        if (rc == kStartNonfinite) {
+            c->err = "non-finite initial residual from the plain start";
+            return VPS_ERR_NONFINITE;
+        }
+    }
     if (rc != VPS_OK && rc != VPS_ERR_NO_CONVERGENCE) return rc;
     // The network's history and training read this solve's pressures in the
     // slot map of this solve (index `which`), which after_solve keeps; they

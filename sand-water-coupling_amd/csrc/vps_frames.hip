@@ -34,7 +34,11 @@
 namespace vps {
 namespace {
 
-constexpr char kMagic[8] = {'V', 'P', 'S', 'P', 'R', 'O', 'J', '1'};
+// version 2 adds the 3D CG's measured FP32 drift (it sets the recursion's
+// stopping threshold, so a resumed run needs it to stop where the original
+// run stops); version 1 files load with the default drift
+constexpr char kMagic[8] = {'V', 'P', 'S', 'P', 'R', 'O', 'J', '2'};
+constexpr char kMagicV1[8] = {'V', 'P', 'S', 'P', 'R', 'O', 'J', '1'};
 
 struct Blob {
     std::vector<char> b;
@@ -200,6 +204,7 @@ int save_frame(vps_context* c, const std::string& dir) {
     o.put(c->rho0);
     o.put(c->kappa);
     o.put(c->alpha0);
+    o.put(c->cg_drift);
     {   // frozen real volumes, caller order
         std::vector<double> vol((size_t)n);
         VPS_TRY(vps_get_real_fields(c, nullptr, vol.data()));
@@ -258,10 +263,13 @@ int load_frame(vps_context* c, const std::string& dir) {
     char magic[8];
     int32_t dim = 0, warm = 0, init = 0;
     int64_t ns_state = 0;
-    double rho0 = 0, kappa = 0, alpha0 = 0;
-    if (!r.get_array(magic, 8) || std::memcmp(magic, kMagic, 8) != 0 || !r.get(dim) ||
-        !r.get(warm) || !r.get(init) || !r.get(ns_state) || !r.get(rho0) ||
-        !r.get(kappa) || !r.get(alpha0))
+    double rho0 = 0, kappa = 0, alpha0 = 0, drift = 1.0;
+    if (!r.get_array(magic, 8))
+        return fail_io(c, dir + "/projection.sav: not a projection checkpoint");
+    const bool v2 = std::memcmp(magic, kMagic, 8) == 0;
+    if ((!v2 && std::memcmp(magic, kMagicV1, 8) != 0) || !r.get(dim) || !r.get(warm) ||
+        !r.get(init) || !r.get(ns_state) || !r.get(rho0) || !r.get(kappa) ||
+        !r.get(alpha0) || (v2 && !r.get(drift)) || !std::isfinite(drift) || !(drift > 0))
         return fail_io(c, dir + "/projection.sav: not a projection checkpoint");
     if (dim != D || ns_state != n)
         return fail_io(c, dir + "/projection.sav: dimension or particle count differs "
@@ -284,6 +292,7 @@ int load_frame(vps_context* c, const std::string& dir) {
     c->rho0 = rho0;
     c->kappa = kappa;
     c->alpha0 = alpha0;
+    c->cg_drift = drift;   // after vps_set_particles, which resets it
     c->k.rho0 = (float)rho0;
     c->k.kappa = (float)kappa;
     c->last_solve_buf = -1;

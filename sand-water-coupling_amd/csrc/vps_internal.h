@@ -341,7 +341,8 @@ struct CgState {
     double abs0;      // r.r at the (re)start of the iteration
     float alpha, beta;
     int iter;         // Eigen iteration counter
-    int done;         // 0 running, 1 converged, 2 max iterations, 3 nonfinite
+    int done;         // 0 running, 1 converged, 2 max iterations, 3 nonfinite,
+                      // 4 b = 0, 5 b finite but the initial residual not
     int first;        // 1 before the first SpMV of a (re)start
     int pad;
     unsigned counter[4];
@@ -355,6 +356,9 @@ struct CgState {
     int stop;
 };
 constexpr int kStopBefore = 0x7fffffff;
+// cg_solve's status when b is finite but the initial residual is not (the
+// warm start overflowed); the caller retries from the plain start
+constexpr int kStartNonfinite = 100;
 
 }  // namespace vps
 
@@ -441,6 +445,7 @@ struct vps_context {
     // the FP32 recursion runs to tol^2 b.b divided by it (at most 4x) so the
     // FP64 check usually passes without a replacement
     double cg_drift = 1.0;
+    int64_t start_fallbacks = 0;   // solves restarted from the plain start
 
     // scratch
     vps::DBuf<double> partials;

@@ -586,13 +586,12 @@ __global__ void __launch_bounds__(256) k_mlp_train(
     }
 }
 
-// Sum of the per-block partials (fixed order), optional Adam step.  Block =
-// 64 parameters x 4 row groups; each thread sums rows g, g + 4, ... of its
-// parameter, the 4 groups are added in order.
+// Sum of the per-block partials (fixed order).  Block = 64 parameters x 4
+// row groups; each thread sums rows g, g + 4, ... of its parameter, the 4
+// groups are added in order.
 __global__ void __launch_bounds__(256) k_mlp_reduce(
     const float* grad_part, int nrows, const double* loss_part, float* grad,
-    double* loss, float* P, float* m, float* v, int adam, float lr, float b1t,
-    float b2t) {
+    double* loss) {
     __shared__ float s_g[4][64];
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int p = blockIdx.x * 64 + tx;
@@ -610,9 +609,24 @@ __global__ void __launch_bounds__(256) k_mlp_reduce(
     s_g[ty][tx] = g;
     __syncthreads();
     if (ty != 0 || p >= kNParams) return;
-    g = (s_g[0][tx] + s_g[1][tx]) + (s_g[2][tx] + s_g[3][tx]);
-    grad[p] = g;
-    if (adam) {
+    grad[p] = (s_g[0][tx] + s_g[1][tx]) + (s_g[2][tx] + s_g[3][tx]);
+}
+
+// One Adam step (beta 0.9 / 0.999, eps 1e-8) on the reduced gradient, in one
+// block so that the whole step is taken or skipped: a non-finite loss or
+// gradient entry (a batch the network overflows on) leaves P, m and v as
+// they are and counts the skip.
+__global__ void __launch_bounds__(1024) k_mlp_adam(const float* grad, const double* loss,
+                                                   float* P, float* m, float* v, float lr,
+                                                   float b1t, float b2t, int32_t* skipped) {
+    bool bad = !isfinite(*loss);
+    for (int p = threadIdx.x; p < kNParams; p += 1024) bad |= !isfinite(grad[p]);
+    if (__syncthreads_or(bad)) {
+        if (threadIdx.x == 0) *skipped += 1;
+        return;
+    }
+    for (int p = threadIdx.x; p < kNParams; p += 1024) {
+        const float g = grad[p];
         const float mm = 0.9f * m[p] + 0.1f * g;
         const float vv = 0.999f * v[p] + 0.001f * g * g;
         m[p] = mm;
@@ -750,6 +764,7 @@ uint64_t splitmix(uint64_t& s) {
 struct MlpState {
     DBuf<float> P, m, v, grad, grad_part;
     DBuf<double> loss_part, loss;
+    DBuf<int32_t> skipped;       // Adam steps skipped (non-finite loss or gradient)
     DBuf<float> scale[2];        // SC_* of the current / next step
     int cur_scale = 0;           // scales of the next warm start
     int step_scale = 0;          // scales used by the last warm start
@@ -794,6 +809,8 @@ int mlp_create(vps_context* c) {
     VPS_TRY(dalloc(c, m->grad_part, (size_t)kTrainBlocks * kNParams));
     VPS_TRY(dalloc(c, m->loss_part, kTrainBlocks));
     VPS_TRY(dalloc(c, m->loss, 1));
+    VPS_TRY(dalloc(c, m->skipped, 1));
+    VPS_HIP(c, hipMemset(m->skipped.p, 0, sizeof(int32_t)));
     for (int b = 0; b < 2; b++) {
         VPS_TRY(dalloc(c, m->scale[b], SC_N));
         VPS_HIP(c, hipMemset(m->scale[b].p, 0, SC_N * sizeof(float)));
@@ -838,6 +855,7 @@ void mlp_destroy(vps_context* c) {
     dfree(m->grad_part);
     dfree(m->loss_part);
     dfree(m->loss);
+    dfree(m->skipped);
     for (int b = 0; b < 2; b++) {
         dfree(m->scale[b]);
         dfree(m->DP[b]);
@@ -942,8 +960,10 @@ int mlp_after_solve(vps_context* c, const vps_stats* st) {
                                m->grad_part.p, m->loss_part.p);
             hipLaunchKernelGGL(k_mlp_reduce, dim3(blocks(kNParams, 64)), dim3(256),
                                0, c->stream, m->grad_part.p, kTrainBlocks,
-                               m->loss_part.p, m->grad.p, m->loss.p, m->P.p, m->m.p,
-                               m->v.p, 1, (float)m->lr, b1t, b2t);
+                               m->loss_part.p, m->grad.p, m->loss.p);
+            hipLaunchKernelGGL(k_mlp_adam, dim3(1), dim3(1024), 0, c->stream, m->grad.p,
+                               m->loss.p, m->P.p, m->m.p, m->v.p, (float)m->lr, b1t, b2t,
+                               m->skipped.p);
         }
     }
     const int nxt = 1 - m->cur_scale;
@@ -1109,7 +1129,7 @@ int vps_mlp_loss_and_grad(vps_context* c, int64_t n, const float* features,
                        tpw, 0ull, m->grad_part.p, m->loss_part.p);
     hipLaunchKernelGGL(k_mlp_reduce, dim3(blocks(kNParams, 64)), dim3(256), 0,
                        c->stream, m->grad_part.p, kTrainBlocks, m->loss_part.p,
-                       m->grad.p, dl.p, m->P.p, m->m.p, m->v.p, 0, 0.f, 0.f, 0.f);
+                       m->grad.p, dl.p);
     VPS_HIP(c, hipGetLastError());
     if (grad)
         VPS_HIP(c, hipMemcpyAsync(grad, m->grad.p, kNParams * 4,
@@ -1150,12 +1170,16 @@ int vps_mlp_training_state(vps_context* c, double* last_loss, int64_t* steps) {
     if (!c) return VPS_ERR_INVALID_ARG;
     const MlpState* m = S(c);
     double l = 0;
+    int32_t skipped = 0;
     VPS_HIP(c, hipMemcpyAsync(c->host_counters + 20, m->loss.p, 8,
+                              hipMemcpyDeviceToHost, c->stream));
+    VPS_HIP(c, hipMemcpyAsync(c->host_counters + 21, m->skipped.p, 4,
                               hipMemcpyDeviceToHost, c->stream));
     VPS_HIP(c, hipStreamSynchronize(c->stream));
     std::memcpy(&l, c->host_counters + 20, 8);
+    std::memcpy(&skipped, c->host_counters + 21, 4);
     if (last_loss) *last_loss = l;
-    if (steps) *steps = m->adam_t;
+    if (steps) *steps = m->adam_t - skipped;
     return VPS_OK;
 }
 

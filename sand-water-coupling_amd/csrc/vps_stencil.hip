@@ -944,8 +944,9 @@ __global__ void __launch_bounds__(1024) k_cg_start(const double* pbb,
     const double thr = tol * tol * bb;
     st->thr = thr < DBL_MIN ? DBL_MIN : thr;
     st->thr_rec = st->thr * rec_scale;
-    if (!isfinite(bb) || !isfinite(rr)) st->done = 3;
+    if (!isfinite(bb)) st->done = 3;
     else if (bb == 0.0) st->done = 4;   // Eigen: x = 0, zero iterations
+    else if (!isfinite(rr)) st->done = 5;   // the initial guess overflowed
     else st->done = rr < st->thr ? 1 : 0;
     st->stop = st->done ? kStopBefore : 0;
 }
@@ -1306,6 +1307,10 @@ int cg_solve(vps_context* c, vps_stats* stats) {
     CgState hs{};
     if (max_iter == 0) {
         VPS_TRY(read_state(c, &hs));
+        if (hs.done == 5) {
+            c->err = "non-finite initial residual (initial guess)";
+            return kStartNonfinite;
+        }
         if (hs.done == 0) {
             c->err = "CG reached the iteration limit without converging";
             return VPS_ERR_NO_CONVERGENCE;
@@ -1463,8 +1468,12 @@ int cg_solve(vps_context* c, vps_stats* stats) {
             if (!start_checked) {
                 start_checked = true;
                 if (hs.done == 3) {
-                    c->err = "non-finite right-hand side or initial residual";
+                    c->err = "non-finite right-hand side";
                     return VPS_ERR_NONFINITE;
+                }
+                if (hs.done == 5) {
+                    c->err = "non-finite initial residual (initial guess)";
+                    return kStartNonfinite;
                 }
                 if (hs.done == 4) {   // b = 0: x = 0, zero iterations
                     VPS_HIP(c, hipMemsetAsync(x, 0, (size_t)nn * 4, c->stream));

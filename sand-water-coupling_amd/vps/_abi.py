@@ -100,6 +100,9 @@ class Stats(C.Structure):
         # its CG kernels (microseconds)
         out["cg_replacements"] = self.reserved[0]
         out["cg_host_launch_us"] = self.reserved[1]
+        # 1 when the warm start overflowed and the solve ran again from the
+        # plain (index-carried) start
+        out["warm_start_fallback"] = self.reserved[2]
         return out
 
 

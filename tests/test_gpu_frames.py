@@ -10,6 +10,7 @@
 * without projection.sav the load re-initialises as the reference does
   (volumes recomputed from the loaded positions)."""
 import os
+import struct
 
 import numpy as np
 import pytest
@@ -101,6 +102,40 @@ def test_resume_is_exact(tmp_path, warm):
         assert np.array_equal(va, vb), k
     if train:
         assert np.array_equal(a.mlp_parameters(), b.mlp_parameters())
+    a.close()
+    b.close()
+
+
+def _saved_drift(fd):
+    """The 3D CG drift stored in projection.sav (version 2 header: magic,
+    dim, warm start, initialized, n, rho0, kappa, alpha0, drift)."""
+    with open(os.path.join(fd, "projection.sav"), "rb") as f:
+        head = f.read(60)
+    assert head[:8] == b"VPSPROJ2"
+    return struct.unpack_from("<d", head, 8 + 4 + 4 + 4 + 8 + 24)[0]
+
+
+def test_resume_keeps_the_cg_drift(tmp_path):
+    """The recursion's stopping threshold depends on the FP32 drift measured
+    at the last solve: the checkpoint carries it, so the resumed run stops at
+    the same iterations with bit-equal x and v."""
+    sc = scenes.config2(dx=1.0 / 40)
+    a = Projection(sc)
+    a.initialize()
+    _run(a, 4, dt=5e-4)
+    fd = str(tmp_path / "f4")
+    os.makedirs(fd)
+    a.save_frame(fd)
+    drift = _saved_drift(fd)
+    assert drift != 1.0 and np.isfinite(drift)
+    ra = _run(a, 4, dt=5e-4)
+    b = Projection(sc)
+    b.load_frame(fd)
+    rb = _run(b, 4, dt=5e-4)
+    for k, ((xa, va, ia), (xb, vb, ib)) in enumerate(zip(ra, rb)):
+        assert ia == ib, k
+        assert np.array_equal(xa, xb), k
+        assert np.array_equal(va, vb), k
     a.close()
     b.close()
 

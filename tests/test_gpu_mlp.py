@@ -210,3 +210,40 @@ def test_learned_warm_start_cuts_iterations():
     assert trans[2] <= 0.8 * trans[1]
     # against the reference's warm start (index carry)
     assert late[2] * 2 <= late[0]
+
+
+def test_overflowing_network_falls_back_to_the_plain_start():
+    """Finite parameters whose output overflows FP32 give a non-finite x0 at
+    every node with a history.  Each such solve must run again from the
+    reference's index-carried start and match a plain context step for step
+    (same iterations, same velocities); training on the overflowing network
+    must skip every Adam step (non-finite loss) and leave it unchanged."""
+    sc = scenes.small_3d(16, 10, 8)
+    plain = Projection(sc, warm_start=_abi.WARM_INDEX_CARRY)
+    learned = Projection(sc, warm_start=_abi.WARM_LEARNED)
+    p = M.random_params(5).astype(np.float32)
+    p[4736:4801] = 3e38   # W3 and b3: the output sum overflows
+    learned.mlp_set_parameters(p)
+    learned.mlp_set_training(True, batch=4096, lr=1e-3, steps_per_solve=2, seed=3)
+    plain.initialize()
+    learned.initialize()
+    fallbacks = 0
+    for step in range(5):
+        plain.advect(1e-3, 9.8)
+        learned.advect(1e-3, 9.8)
+        sp = plain.projection_step()
+        sl = learned.projection_step()
+        fallbacks += sl["warm_start_fallback"]
+        assert sp["warm_start_fallback"] == 0
+        assert sl["cg_iterations"] == sp["cg_iterations"], step
+        assert sl["cg_true_rel_residual"] < 1e-6
+        _, vp = plain.particles()
+        _, vl = learned.particles()
+        assert np.array_equal(vl, vp), step
+    # the first solve has no history (x0 = 0 in both); every later one fell back
+    assert fallbacks == 4
+    assert np.array_equal(learned.mlp_parameters(), p)
+    _, steps = learned.mlp_training_state()
+    assert steps == 0
+    plain.close()
+    learned.close()

@@ -25,10 +25,12 @@ def scene(name):
         return scenes.small_3d(12, 8, 6)
     if name == "cfg1":
         return scenes.config1()
-    if name == "3d_tiny":    # at most 8 bricks: one per block, plain block order
-        return scenes.small_3d(4, 4, 4)
-    if name == "3d_odd":     # 12 bricks: 8 blocks, some with two bricks
-        return scenes.small_3d(6, 4, 4)
+    # slots = bricks of 8^3 nodes that hold a candidate node: those covering
+    # node indices floor((x - h - o)/s) .. floor((x + h - o)/s) + 1 of a particle
+    if name == "3d_tiny":    # 6 slots (3 x 1 x 2): one block per brick, plain block order
+        return scenes.small_3d(12, 2, 4)
+    if name == "3d_odd":     # 12 slots (3 x 2 x 2): 8 blocks, four of them with two bricks
+        return scenes.small_3d(12, 4, 4)
     raise KeyError(name)
 
 
@@ -142,6 +144,10 @@ def test_iteration_grid_shapes(name):
         g.advect(1e-3, 9.8)
         so = o.projection_step()
         sg = g.projection_step()
+        # the grid shape the case stands for: fewer than 8 slots (one block
+        # per brick), or 8 or more and not a multiple of 8 (blocks owning two)
+        ns = sg["n_active_bricks"]
+        assert (ns < 8) if name == "3d_tiny" else (ns >= 8 and ns % 8 != 0), ns
         assert abs(sg["cg_iterations"] - so["cg_iterations"]) <= 1, step
         assert sg["cg_true_rel_residual"] < 1e-6, step
         vo, vg = o.virtual(), g.virtual()
