@@ -458,6 +458,21 @@ int vps_mlp_set_parameters(vps_context* ctx, const float* in);
 int vps_mlp_loss_and_grad(vps_context* ctx, int64_t n, const float* features,
                           const float* targets, double* loss, float* grad);
 
+/* vps_mlp_loss_and_grad on device memory, allocating nothing (no
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory, such as code under HIP graph capture, tight serving or training
+ * loops and harnesses that forbid hipMalloc).  d_features (n x 8 FP32),
+ * d_targets (n FP32), d_loss (1 double) and d_grad (4,801 FP32) are device
+ * pointers on the context's device; n <= 67,108,864.  The per-block partial
+ * sums live in a fixed 4.92 MB scratch array of the library on each device
+ * (set up at vps_create), so calls on one device that overlap in time
+ * (other contexts, other threads) race on it.  The kernels are queued on the
+ * context's stream and the call returns without waiting: the results are
+ * ready after vps_synchronize(ctx).  Same kernels and results as
+ * vps_mlp_loss_and_grad. */
+int vps_mlp_loss_and_grad_device(vps_context* ctx, int64_t n, const float* d_features,
+                                 const float* d_targets, double* d_loss, float* d_grad);
+
 /* Inference over given features: out[n]. */
 int vps_mlp_forward(vps_context* ctx, int64_t n, const float* features,
                     float* out);

@@ -41,6 +41,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "vps_internal.h"
@@ -64,6 +65,9 @@ enum {
 };
 constexpr int kTrainBlocks = 256;               // one block of 4 waves per CU
 constexpr int kTrainWaves = 4 * kTrainBlocks;
+// largest batch of vps_mlp_loss_and_grad(_device): 4,096 tiles of 16 samples
+// per training wave
+constexpr int64_t kMaxLossBatch = (int64_t)16 * kTrainWaves * 4096;
 constexpr int kInferBlocks = 1024;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
@@ -799,7 +803,50 @@ static void init_params(std::vector<float>& p, uint64_t seed) {
     // output layer zero: the untrained network predicts "no change"
 }
 
+// ---- scratch of vps_mlp_loss_and_grad_device (allocates nothing) ---------
+// One call's per-block partial losses (kTrainBlocks doubles) and partial
+// gradients (kTrainBlocks x kNParams floats): 4.92 MB on every device the
+// library runs on, held until the process ends.  Every call overwrites all
+// of it (each training block writes its whole row and its loss), so nothing
+// needs clearing; calls that overlap on one device race on it.
+constexpr size_t kScratchLossBytes = (size_t)kTrainBlocks * sizeof(double);
+constexpr size_t kScratchBytes =
+    kScratchLossBytes + (size_t)kTrainBlocks * kNParams * sizeof(float);
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+static_assert(kScratchLossBytes % 256 == 0, "gradient partials stay 256-byte aligned");
+
+// Device address of `scratch` per device: the host-side name is only a
+// stand-in, so kernels get the address hipGetSymbolAddress returns, fetched
+// once per device at context creation with that device current.
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kScratchDevices] = {};
+
+static int scratch_setup(vps_context* c) {
+    if (c->device < 0 || c->device >= kScratchDevices) {
+        c->err = "device index beyond the scratch table";
+        return VPS_ERR_INVALID_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[c->device]) return VPS_OK;
+    void* ptr = nullptr;
+    VPS_HIP(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) {
+        c->err = "hipGetSymbolAddress(scratch) returned no address";
+        return VPS_ERR_HIP;
+    }
+    g_scratch_dev[c->device] = ptr;
+    return VPS_OK;
+}
+
+static void* scratch_ptr(int device) {
+    if (device < 0 || device >= kScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_dev[device];
+}
+
 int mlp_create(vps_context* c) {
+    VPS_TRY(scratch_setup(c));
     MlpState* m = new MlpState();
     c->mlp = m;
     VPS_TRY(dalloc(c, m->P, kNParams));
@@ -1110,7 +1157,7 @@ int vps_mlp_set_parameters(vps_context* c, const float* in) {
 int vps_mlp_loss_and_grad(vps_context* c, int64_t n, const float* features,
                           const float* targets, double* loss, float* grad) {
     if (!c || n <= 0 || !features || !targets) return VPS_ERR_INVALID_ARG;
-    if (n > (int64_t)16 * kTrainWaves * 4096) {
+    if (n > kMaxLossBatch) {
         c->err = "mlp loss_and_grad: too many samples";
         return VPS_ERR_INVALID_ARG;
     }
@@ -1143,6 +1190,37 @@ int vps_mlp_loss_and_grad(vps_context* c, int64_t n, const float* features,
     dfree(df);
     dfree(dt);
     dfree(dl);
+    return VPS_OK;
+}
+
+int vps_mlp_loss_and_grad_device(vps_context* c, int64_t n, const float* d_features,
+                                 const float* d_targets, double* d_loss, float* d_grad) {
+    if (!c) return VPS_ERR_INVALID_ARG;
+    if (n <= 0 || !d_features || !d_targets || !d_loss || !d_grad) {
+        c->err = "mlp loss_and_grad_device: empty batch or NULL pointer";
+        return VPS_ERR_INVALID_ARG;
+    }
+    if (n > kMaxLossBatch) {
+        c->err = "mlp loss_and_grad_device: batch above the maximum of " +
+                 std::to_string(kMaxLossBatch) + " samples";
+        return VPS_ERR_INVALID_ARG;
+    }
+    char* base = (char*)scratch_ptr(c->device);
+    if (!base) {
+        c->err = "mlp loss_and_grad_device: scratch not set up on this device";
+        return VPS_ERR_STATE;
+    }
+    MlpState* m = S(c);
+    double* loss_part = (double*)base;
+    float* grad_part = (float*)(base + kScratchLossBytes);
+    const int tpw = (int)((n + 16 * kTrainWaves - 1) / (16 * kTrainWaves));
+    FeatSrc none{};
+    hipLaunchKernelGGL(k_mlp_train<true>, dim3(kTrainBlocks), dim3(256), 0, c->stream,
+                       m->P.p, none, nullptr, nullptr, n, d_features, d_targets, n, tpw,
+                       0ull, grad_part, loss_part);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(blocks(kNParams, 64)), dim3(256), 0, c->stream,
+                       grad_part, kTrainBlocks, loss_part, d_grad, d_loss);
+    VPS_HIP(c, hipGetLastError());
     return VPS_OK;
 }
 

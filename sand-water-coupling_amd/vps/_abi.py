@@ -238,6 +238,8 @@ def declare(lib):
     _proto(lib, "vps_mlp_get_parameters", C.c_int, [_P, _F])
     _proto(lib, "vps_mlp_set_parameters", C.c_int, [_P, _F])
     _proto(lib, "vps_mlp_loss_and_grad", C.c_int, [_P, C.c_int64, _F, _F, _D, _F])
+    _proto(lib, "vps_mlp_loss_and_grad_device", C.c_int,
+           [_P, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p])
     _proto(lib, "vps_mlp_forward", C.c_int, [_P, C.c_int64, _F, _F])
     _proto(lib, "vps_mlp_get_features", C.c_int, [_P, _F])
     _proto(lib, "vps_mlp_training_state", C.c_int, [_P, _D, _I64])
@@ -273,7 +275,8 @@ EXPORTED = [
     "vps_get_real_fields", "vps_get_boundary_fields", "vps_get_scalars",
     "vps_query_neighbors", "vps_mlp_set_training", "vps_mlp_parameter_count",
     "vps_mlp_get_parameters", "vps_mlp_set_parameters",
-    "vps_mlp_loss_and_grad", "vps_mlp_forward", "vps_mlp_get_features",
+    "vps_mlp_loss_and_grad", "vps_mlp_loss_and_grad_device", "vps_mlp_forward",
+    "vps_mlp_get_features",
     "vps_mlp_training_state", "vps_dem_step", "vps_couple_grains",
     "vps_add_surface", "vps_generate_box", "vps_get_boundary",
     "vps_move_particles", "vps_apply_gravity", "vps_apply_viscosity", "vps_advance",

@@ -231,6 +231,13 @@ class Projection:
                                                  C.byref(loss), _f(g)))
         return loss.value, g
 
+    def mlp_loss_and_grad_device(self, n, d_features, d_targets, d_loss, d_grad):
+        """vps_mlp_loss_and_grad_device on device pointers (integers): queues
+        the kernels on the context's stream and returns; call synchronize()
+        before reading d_loss / d_grad."""
+        self._check(self.L.vps_mlp_loss_and_grad_device(self.h, int(n), d_features,
+                                                        d_targets, d_loss, d_grad))
+
     def mlp_forward(self, features):
         f = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, 8)
         out = np.zeros(len(f), dtype=np.float32)
