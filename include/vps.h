@@ -43,8 +43,9 @@ enum vps_status {
                                   ParticlesNearbySearcher.h:71 with .cpp:13) */
     VPS_ERR_NO_CONVERGENCE = 2,/* CG hit max_iterations (reference: exit(-1)) */
     VPS_ERR_NONFINITE = 3,     /* NaN/Inf in the right-hand side or solution
-                                  (a non-finite warm start is not an error:
-                                  the solve falls back to the plain start) */
+                                  (a bad learned or extrapolated warm start is
+                                  not an error: the solve falls back to the
+                                  plain start) */
     VPS_ERR_OOM = 4,           /* device allocation failed */
     VPS_ERR_HIP = 5,           /* any other HIP runtime error */
     VPS_ERR_STATE = 6,         /* call order violated (e.g. project before
@@ -78,7 +79,11 @@ enum vps_warm_start {
                                   below, tau the RMS error of that
                                   extrapolation at the last solve (no
                                   reference counterpart); an untrained
-                                  network gives the extrapolation */
+                                  network gives the extrapolation.  This
+                                  start and the next are guarded: one whose
+                                  initial residual is not finite or exceeds
+                                  ||b|| is replaced by the index-carried
+                                  start */
     VPS_WARM_EXTRAPOLATE = 4   /* x0 = p1 + e: polynomial extrapolation of the
                                   node's last three pressures, e = dp1 with two
                                   known, 2 dp1 - dp2 with three (no reference
@@ -147,11 +152,12 @@ typedef struct vps_stats {
     int64_t spmv_launches;
     int64_t reserved[4];       /* [0] residual replacements of the last solve,
                                   [1] host microseconds spent launching its CG
-                                  kernels, [2] 1 when the warm start gave a
-                                  non-finite initial residual (b finite) and
-                                  the solve ran again from the plain start
-                                  (the index-carried pressures; zero when
-                                  that start was the one that failed) */
+                                  kernels, [2] 1 when a learned or
+                                  extrapolated start was rejected (initial
+                                  residual non-finite or above ||b||, or the
+                                  iteration from it overflowed) and the solve
+                                  ran again from the plain, index-carried
+                                  start */
 } vps_stats;
 
 typedef struct vps_context vps_context;

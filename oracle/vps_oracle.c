@@ -419,6 +419,7 @@ struct or_sim {
      * pressure p1, increments dp1 = p1 - p2 and dp2 = p2 - p3, and the depth
      * h (0: node absent from the last solve, else how many are known, <= 3) */
     double* x0;          /* CG initial guess of the last project (npres) */
+    int start_fallback;  /* 1: the last guarded start gave way to the carried one */
     double* prev_node_p;
     double* prev_node_dp1;
     double* prev_node_dp2;
@@ -960,9 +961,13 @@ int or_project(or_sim* s, or_stats* st) {
     /* _pressures.resize(this): keep the first min(old,new), zero the rest
      * (ParticlesBasedData.h:25-29); other warm starts are options */
     double* p_new = (double*)calloc((size_t)(n ? n : 1), sizeof(double));
-    if (s->p.warm_start == VPS_WARM_INDEX_CARRY) {
+    double* p_carry = (double*)calloc((size_t)(n ? n : 1), sizeof(double));
+    {
         const int64_t keep = s->npres < n ? s->npres : n;
-        if (s->pres) memcpy(p_new, s->pres, (size_t)keep * sizeof(double));
+        if (s->pres) memcpy(p_carry, s->pres, (size_t)keep * sizeof(double));
+    }
+    if (s->p.warm_start == VPS_WARM_INDEX_CARRY) {
+        memcpy(p_new, p_carry, (size_t)n * sizeof(double));
     } else if (s->p.warm_start == VPS_WARM_NODE_KEYED) {
         if (s->prev_node_p)
             for (int64_t I = 0; I < n; I++)
@@ -1014,7 +1019,31 @@ int or_project(or_sim* s, or_stats* st) {
                     s->kappa * (drho < 0. ? 0. : drho) / s->rho0;
     }
     int rc = calculate_coef(s);
-    if (rc) return rc;
+    if (rc) {
+        free(p_carry);
+        return rc;
+    }
+    /* guarded starts (extrapolated or learned, no reference counterpart):
+     * one whose residual is not finite or above ||b|| gives way to the
+     * index-carried start, as in the HIP library */
+    s->start_fallback = 0;
+    if (s->p.warm_start == VPS_WARM_EXTRAPOLATE || s->p.warm_start == VPS_WARM_LEARNED) {
+        double* t = (double*)malloc((size_t)(n ? n : 1) * sizeof(double));
+        csr_mv(n, s->rowptr, s->col, s->val, s->pres, t);
+        double rr = 0, bb = 0;
+        for (I = 0; I < n; I++) {
+            const double r = s->div[I] - t[I];
+            rr += r * r;
+            bb += s->div[I] * s->div[I];
+        }
+        free(t);
+        if (!(rr <= bb)) {
+            memcpy(s->pres, p_carry, (size_t)n * sizeof(double));
+            memcpy(s->x0, p_carry, (size_t)n * sizeof(double));
+            s->start_fallback = 1;
+        }
+    }
+    free(p_carry);
     double t1 = now_s();
     /* solveLinearSystem -> IterativeSolver::solve */
     int64_t iters = 0;

@@ -201,6 +201,7 @@ void free_all(vps_context* c) {
     dfree(c->vrt);
     dfree(c->vq);
     dfree(c->vdiag64);
+    dfree(c->vxb);
     dfree(c->vrowsum);
     dfree(c->vrank);
     dfree(c->bits);
@@ -347,6 +348,7 @@ int vps_create(const vps_params* params, vps_context** out) {
     ALLOC(c->vb, cap);
     ALLOC(c->vdiag, cap);
     ALLOC(c->vdiag64, cap);
+    ALLOC(c->vxb, cap);
     ALLOC(c->vrowsum, cap);
     ALLOC(c->vr, cap);
     ALLOC(c->vt, cap);
@@ -388,6 +390,7 @@ int vps_create(const vps_params* params, vps_context** out) {
             hipMemset(c->vb.p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->vdiag.p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->vdiag64.p, 0, cap * 8) != hipSuccess ||
+            hipMemset(c->vxb.p, 0, cap * 8) != hipSuccess ||
             hipMemset(c->vrowsum.p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->vr.p, 0, cap * 4) != hipSuccess ||
             hipMemset(c->vt.p, 0, cap * 4) != hipSuccess ||
@@ -638,22 +641,20 @@ static int project_impl(vps_context* c, vps_stats* st, int e0) {
     }
     vps_stats local{};
     vps_stats* s = st ? st : &local;
+    // a learned or extrapolated start is guarded: rejected (and the solve run
+    // from the plain start) when its residual is not finite or larger than
+    // ||b||, or when the iteration from it overflows
+    c->start_guard = mode == VPS_WARM_LEARNED || mode == VPS_WARM_EXTRAPOLATE;
     int rc = cg_solve(c, s);
-    if (rc == kStartNonfinite) {
-        // The start gave a non-finite residual while b is finite (a learned
-        // or extrapolated x0 that overflowed): solve again from the plain
-        // start, the reference's index-carried pressures (zero when that was
-        // the start already), and count the event.
-        const int plain = mode == VPS_WARM_INDEX_CARRY ? VPS_WARM_ZERO : VPS_WARM_INDEX_CARRY;
-        VPS_TRY(warm_start(c, plain, x0));
+    if (rc == kStartRejected) {
+        // the plain start: the reference's index-carried pressures
+        VPS_TRY(warm_start(c, VPS_WARM_INDEX_CARRY, x0));
         c->start_fallbacks++;
+        c->start_guard = false;
         rc = cg_solve(c, s);
         s->reserved[2] = 1;
-        if (rc == kStartNonfinite) {
-            c->err = "non-finite initial residual from the plain start";
-            return VPS_ERR_NONFINITE;
-        }
     }
+    c->start_guard = false;
     if (rc != VPS_OK && rc != VPS_ERR_NO_CONVERGENCE) return rc;
     // The network's history and training read this solve's pressures in the
     // slot map of this solve (index `which`), which after_solve keeps; they

@@ -342,7 +342,7 @@ struct CgState {
     float alpha, beta;
     int iter;         // Eigen iteration counter
     int done;         // 0 running, 1 converged, 2 max iterations, 3 nonfinite,
-                      // 4 b = 0, 5 b finite but the initial residual not
+                      // 4 b = 0, 5 guarded start rejected (r0.r0 not <= b.b)
     int first;        // 1 before the first SpMV of a (re)start
     int pad;
     unsigned counter[4];
@@ -356,9 +356,11 @@ struct CgState {
     int stop;
 };
 constexpr int kStopBefore = 0x7fffffff;
-// cg_solve's status when b is finite but the initial residual is not (the
-// warm start overflowed); the caller retries from the plain start
-constexpr int kStartNonfinite = 100;
+// cg_solve's status when a guarded warm start (learned or extrapolated) is
+// rejected: its initial residual is not finite or larger than ||b|| (worse
+// than a zero start), or the iteration from it overflowed; the caller
+// retries from the plain start
+constexpr int kStartRejected = 100;
 
 }  // namespace vps
 
@@ -428,6 +430,7 @@ struct vps_context {
     // FP64-checked true residual
     vps::DBuf<float> vr2, vt2, vrt;
     vps::DBuf<double> vdiag64;
+    vps::DBuf<double> vxb;             // CG: FP64 base of the solution x
     vps::DBuf<int32_t> vrank;
     // lattice bitmask for reference (x-fastest) ranks
     int64_t nwords = 0;
@@ -446,6 +449,7 @@ struct vps_context {
     // FP64 check usually passes without a replacement
     double cg_drift = 1.0;
     int64_t start_fallbacks = 0;   // solves restarted from the plain start
+    bool start_guard = false;      // cg_solve: the start may be rejected
 
     // scratch
     vps::DBuf<double> partials;

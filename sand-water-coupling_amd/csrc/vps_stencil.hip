@@ -120,6 +120,7 @@ struct StencilArgs {
     const double* vdiag64;
     const float* vb;
     const float* in;       // p / x / x0 / in (p_old when fused)
+    const double* in_base; // K_TRUE_RES: FP64 base of x (x = in_base + in)
     const float* in2;      // r for the fused update (r_old: 3D iteration)
     const float* in3;      // t_old (3D iteration)
     float* out;            // t / r / out
@@ -285,6 +286,8 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
     // block partials after the outputs (8-byte aligned for that)
     __shared__ __align__(16) T psum[kT][4][64];
     __shared__ float own[512];      // raw own-brick input (p / x)
+    // K_TRUE_RES: the own brick's x = base + correction in FP64
+    __shared__ double own64[KIND == K_TRUE_RES ? 512 : 1];
     __shared__ uint32_t sbase[27];
     __shared__ double sred[4][4];
     __shared__ int s_done;
@@ -321,10 +324,12 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
     // row of the right brick (x = 8..11).  The raw loads do not depend on
     // beta, so they are issued before the prologue's reductions.
     float4 ra[kUnitIters], rb[kUnitIters], rv[kUnitIters];
+    double4 rx[KIND == K_TRUE_RES ? kUnitIters : 1];
 #pragma unroll
     for (int i = 0; i < kUnitIters; i++) {
         const int u = t + kS3Threads * i;
         ra[i] = rb[i] = rv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if constexpr (KIND == K_TRUE_RES) rx[i] = make_double4(0.0, 0.0, 0.0, 0.0);
         // units of the first kUnitIters - 1 rounds always exist (t < 256)
         if (i < kUnitIters - 1 || u < kUnits) {
             const int k = u & 3, line = u >> 2;
@@ -339,6 +344,7 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
             } else {
                 ra[i] = *(const float4*)(a.in + g);
                 if constexpr (KIND == K_SPMV_FUSED) rb[i] = *(const float4*)(a.in2 + g);
+                if constexpr (KIND == K_TRUE_RES) rx[i] = *(const double4*)(a.in_base + g);
                 if constexpr (OP == VPS_OP_ROW) rv[i] = *(const float4*)(a.vvol + g);
             }
         }
@@ -364,11 +370,22 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
                     p = make_float4(rb[i].x + beta * ra[i].x, rb[i].y + beta * ra[i].y,
                                     rb[i].z + beta * ra[i].z, rb[i].w + beta * ra[i].w);
                 pn[0] = p.x; pn[1] = p.y; pn[2] = p.z; pn[3] = p.w;
+                T xv[4] = {(T)p.x, (T)p.y, (T)p.z, (T)p.w};
+                if constexpr (KIND == K_TRUE_RES) {
+                    xv[0] += rx[i].x; xv[1] += rx[i].y; xv[2] += rx[i].z; xv[3] += rx[i].w;
+                }
                 if constexpr (OP == VPS_OP_ROW) {
-                    v[0] = (T)rv[i].x * (T)p.x; v[1] = (T)rv[i].y * (T)p.y;
-                    v[2] = (T)rv[i].z * (T)p.z; v[3] = (T)rv[i].w * (T)p.w;
+                    v[0] = (T)rv[i].x * xv[0]; v[1] = (T)rv[i].y * xv[1];
+                    v[2] = (T)rv[i].z * xv[2]; v[3] = (T)rv[i].w * xv[3];
                 } else {
-                    v[0] = (T)p.x; v[1] = (T)p.y; v[2] = (T)p.z; v[3] = (T)p.w;
+                    v[0] = xv[0]; v[1] = xv[1]; v[2] = xv[2]; v[3] = xv[3];
+                }
+                if constexpr (KIND == K_TRUE_RES) {
+                    if ((k == 1 || k == 2) && yt >= 3 && yt < 11 && zt >= 3 && zt < 11) {
+                        const int loc = 4 * (k - 1) + 8 * ((yt - 3) + 8 * (zt - 3));
+#pragma unroll
+                        for (int c = 0; c < 4; c++) own64[loc + c] = (double)xv[c];
+                    }
                 }
             }
             T* dst = &tile[zt * kPS + yt * kRS + 4 + 4 * k];
@@ -417,7 +434,7 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
             a.out_f[g] = (float)d;
             a.out_s[g] = on ? (float)sd : 0.f;
         } else if constexpr (KIND == K_TRUE_RES) {
-            const double xi = (double)own[loc];
+            const double xi = own64[loc];
             const double st = OP == VPS_OP_ROW ? (double)acc : (double)V * (double)acc;
             const double mx = a.vdiag64[g] * xi + st;
             const double rt = on ? (double)Bg - mx : 0.0;
@@ -759,6 +776,7 @@ template <int KIND, int OP, typename T, typename C>
 __global__ void __launch_bounds__(64) k_stencil2(StencilArgs a, C cc) {
     __shared__ T tile[kHaloIters2 * 64];
     __shared__ float own[64];
+    __shared__ double own64[KIND == K_TRUE_RES ? 64 : 1];
     __shared__ int64_t sbase[10];
     float beta = 0.f;
     if constexpr (KIND == K_SPMV_FIRST || KIND == K_SPMV_FUSED) {
@@ -790,8 +808,15 @@ __global__ void __launch_bounds__(64) k_stencil2(StencilArgs a, C cc) {
         const int v = kHalo2.v[e];
         const int64_t g = sbase[v >> 9] + (v & 511);
         float pn = 0.f;
-        tile[e] = load_input<KIND, OP, T>(a, g, beta, &pn);
-        if constexpr (KIND != K_DIAG) {
+        if constexpr (KIND == K_TRUE_RES) {
+            // x = FP64 base + FP32 correction
+            const double xs = (double)a.in[g] + a.in_base[g];
+            tile[e] = OP == VPS_OP_ROW ? (double)a.vvol[g] * xs : xs;
+            if ((v >> 9) == 4) own64[v & 511] = xs;
+        } else {
+            tile[e] = load_input<KIND, OP, T>(a, g, beta, &pn);
+        }
+        if constexpr (KIND != K_DIAG && KIND != K_TRUE_RES) {
             if ((v >> 9) == 4) {
                 own[v & 511] = pn;
                 if constexpr (KIND == K_SPMV_FUSED) a.out2[g] = pn;
@@ -813,7 +838,7 @@ __global__ void __launch_bounds__(64) k_stencil2(StencilArgs a, C cc) {
         a.out_f[g] = (float)d;
         a.out_s[g] = on ? (float)s : 0.f;
     } else if constexpr (KIND == K_TRUE_RES) {
-        const double xi = (double)own[loc];
+        const double xi = own64[loc];
         const double st = OP == VPS_OP_ROW ? (double)acc : (double)V * (double)acc;
         const double mx = a.vdiag64[g] * xi + st;
         const double rt = on ? (double)a.vb[g] - mx : 0.0;
@@ -912,7 +937,7 @@ __global__ void __launch_bounds__(kUpdThreads) k_cg_update(
 __global__ void __launch_bounds__(1024) k_cg_start(const double* pbb,
                                                    const double* prr, int n,
                                                    double tol_in, double rec_scale,
-                                                   CgState* st) {
+                                                   int guard, CgState* st) {
     __shared__ double sbuf[16];
     double va[4], vb[4];
 #pragma unroll
@@ -946,7 +971,7 @@ __global__ void __launch_bounds__(1024) k_cg_start(const double* pbb,
     st->thr_rec = st->thr * rec_scale;
     if (!isfinite(bb)) st->done = 3;
     else if (bb == 0.0) st->done = 4;   // Eigen: x = 0, zero iterations
-    else if (!isfinite(rr)) st->done = 5;   // the initial guess overflowed
+    else if (!isfinite(rr) || (guard && !(rr <= bb))) st->done = 5;   // start rejected
     else st->done = rr < st->thr ? 1 : 0;
     st->stop = st->done ? kStopBefore : 0;
 }
@@ -966,6 +991,38 @@ __global__ void __launch_bounds__(kUpdThreads) k_copy_rr(int64_t n4, const float
     }
     rr = block_sum<kUpdThreads>(rr, sbuf);
     if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+
+// x is kept in two parts during a solve: an FP64 base xb and the FP32
+// correction x that the iterations update.  A correction is small next to
+// the solution once the start is close, so its FP32 rounding stays far
+// below the tolerance even where ||b|| is small next to ||M|| ||x||.
+//   kFoldSplit  xb = x0, x = 0             (start of the solve)
+//   kFoldMerge  xb += x, x = 0             (each residual replacement)
+//   kFoldFinal  x = (float)(xb + x)        (end of the solve: the pressures)
+//   kFoldZero   x = 0, xb = 0              (b = 0: Eigen's x = 0)
+enum FoldMode { kFoldSplit = 0, kFoldMerge = 1, kFoldFinal = 2, kFoldZero = 3 };
+__global__ void __launch_bounds__(kUpdThreads) k_fold_x(int64_t n4, float4* x, double4* xb,
+                                                        int mode) {
+    for (int64_t i = (int64_t)blockIdx.x * kUpdThreads + threadIdx.x; i < n4;
+         i += (int64_t)kUpdBlocks * kUpdThreads) {
+        const float4 d = x[i];
+        if (mode == kFoldSplit) {
+            xb[i] = make_double4(d.x, d.y, d.z, d.w);
+            x[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else if (mode == kFoldMerge) {
+            const double4 b = xb[i];
+            xb[i] = make_double4(b.x + d.x, b.y + d.y, b.z + d.z, b.w + d.w);
+            x[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else if (mode == kFoldFinal) {
+            const double4 b = xb[i];
+            x[i] = make_float4((float)(b.x + d.x), (float)(b.y + d.y), (float)(b.z + d.z),
+                               (float)(b.w + d.w));
+        } else {
+            xb[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+            x[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
 }
 
 // ... and the iteration resumes from the same x: alpha = 0 (x already has
@@ -1259,6 +1316,10 @@ static double ms_between(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+// residual replacements of one solve before it fails (the FP64 base of x
+// makes the true residual reachable: one or two are the rule)
+constexpr int kMaxReplacements = 8;
+
 int cg_solve(vps_context* c, vps_stats* stats) {
     const int w = c->which;
     const int64_t nn = c->nslots * c->brick_nodes;
@@ -1282,6 +1343,17 @@ int cg_solve(vps_context* c, vps_stats* stats) {
     float* p0 = c->vp[0].p;
     float* p1 = c->vp[1].p;
     const int nslots = (int)c->nslots;
+    // x = xb (FP64) + x (FP32 correction) during the solve; every return
+    // after the split leaves the pressures in x
+    double* xb = c->vxb.p;
+    auto fold = [&](int mode) {
+        hipLaunchKernelGGL(k_fold_x, dim3(kUpdBlocks), dim3(kUpdThreads), 0, c->stream, n4,
+                           (float4*)x, (double4*)xb, mode);
+    };
+    auto finish = [&](int rc) {
+        fold(kFoldFinal);
+        return rc;
+    };
 
     {
         StencilArgs a = base_args(c);
@@ -1291,13 +1363,14 @@ int cg_solve(vps_context* c, vps_stats* stats) {
         a.part0 = part_init(c, 0);
         a.part1 = part_init(c, 1);
         launch_op<K_CG_INIT, float, CoefF>(c, a, cf);
+        fold(kFoldSplit);
         // 3D: the recursion's threshold absorbs the FP32 drift measured at
         // the last solve (with a 10% margin); the FP64 check keeps tol
         double rec_scale = 1.0;
         if (c->dim == 3 && c->cg_drift > 0.9) rec_scale = std::max(0.25, 0.9 / c->cg_drift);
         hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(1024), 0, c->stream,
                            part_init(c, 0), part_init(c, 1), nslots, tol_in, rec_scale,
-                           c->cg.p);
+                           c->start_guard ? 1 : 0, c->cg.p);
         VPS_HIP(c, hipGetLastError());
     }
     // The start state is read back together with the first batch: its
@@ -1308,12 +1381,12 @@ int cg_solve(vps_context* c, vps_stats* stats) {
     if (max_iter == 0) {
         VPS_TRY(read_state(c, &hs));
         if (hs.done == 5) {
-            c->err = "non-finite initial residual (initial guess)";
-            return kStartNonfinite;
+            c->err = "initial guess rejected (non-finite or worse than zero)";
+            return finish(kStartRejected);
         }
         if (hs.done == 0) {
             c->err = "CG reached the iteration limit without converging";
-            return VPS_ERR_NO_CONVERGENCE;
+            return finish(VPS_ERR_NO_CONVERGENCE);
         }
     }
     bool start_checked = max_iter == 0;
@@ -1454,6 +1527,7 @@ int cg_solve(vps_context* c, vps_stats* stats) {
                 // runs on the device only if this batch converged
                 StencilArgs tr = base_args(c);
                 tr.in = x;
+                tr.in_base = xb;
                 tr.out = three ? c->vrt.p : t;   // r_true
                 tr.part0 = part_true(c);
                 tr.spec = 1;
@@ -1469,14 +1543,14 @@ int cg_solve(vps_context* c, vps_stats* stats) {
                 start_checked = true;
                 if (hs.done == 3) {
                     c->err = "non-finite right-hand side";
-                    return VPS_ERR_NONFINITE;
+                    return finish(VPS_ERR_NONFINITE);
                 }
                 if (hs.done == 5) {
-                    c->err = "non-finite initial residual (initial guess)";
-                    return kStartNonfinite;
+                    c->err = "initial guess rejected (non-finite or worse than zero)";
+                    return finish(kStartRejected);
                 }
                 if (hs.done == 4) {   // b = 0: x = 0, zero iterations
-                    VPS_HIP(c, hipMemsetAsync(x, 0, (size_t)nn * 4, c->stream));
+                    fold(kFoldZero);
                     hs.done = 1;
                     hs.iter = 0;
                     hs.rr = 0;
@@ -1487,7 +1561,7 @@ int cg_solve(vps_context* c, vps_stats* stats) {
             if (hs.done == 0) {
                 if (iter_base + kk > max_iter + 8) {
                     c->err = "CG loop did not terminate";
-                    return VPS_ERR_NO_CONVERGENCE;
+                    return finish(VPS_ERR_NO_CONVERGENCE);
                 }
                 // size the next batch from the mean residual decay so far
                 const int done_it = hs.iter - iter_base;
@@ -1505,7 +1579,8 @@ int cg_solve(vps_context* c, vps_stats* stats) {
         }
         if (hs.done == 3) {
             c->err = "non-finite value in CG";
-            return VPS_ERR_NONFINITE;
+            // from a guarded start: solve again from the plain one
+            return finish(c->start_guard ? kStartRejected : VPS_ERR_NONFINITE);
         }
         if (hs.done == 2) break;
         // converged on the recursive residual; the true residual of this
@@ -1514,8 +1589,16 @@ int cg_solve(vps_context* c, vps_stats* stats) {
         if (three && replacements == 0 && hs.rr > 0 && hs.true_rr >= 0 &&
             std::isfinite(hs.true_rr / hs.rr))
             c->cg_drift = hs.true_rr / hs.rr;
-        if (hs.true_rr < hs.thr || replacements >= 4) break;
+        if (hs.true_rr < hs.thr) break;
+        if (replacements >= kMaxReplacements) {
+            c->err = "CG: the true residual stayed above the tolerance after " +
+                     std::to_string(kMaxReplacements) + " residual replacements";
+            return finish(VPS_ERR_NO_CONVERGENCE);
+        }
         replacements++;
+        // the correction so far goes into the FP64 base; the iteration goes
+        // on with a fresh FP32 correction from the true residual
+        fold(kFoldMerge);
         if (three && hs.iter > iter_base) {
             // residual replacement (van der Vorst and Ye): r takes the true
             // residual, the direction is kept and the iteration continues
@@ -1561,6 +1644,8 @@ int cg_solve(vps_context* c, vps_stats* stats) {
         VPS_TRY(read_state(c, &hs));
         predicted = iter_base + 4;
     }
+    fold(kFoldFinal);
+    VPS_HIP(c, hipGetLastError());
     c->prev_iters = hs.iter;
     double spmv_ms = 0, upd_ms = 0;
     int64_t counted = 0;
