@@ -3,16 +3,20 @@
 
 Workload: the 3D dam break of 1,012,500 fluid particles (dx = 1/150, a
 1 x 0.6 x 0.5 block in a 2 x 1 x 1 box, 3 wall layers), one GPU.  A step is
-one advection (x += v dt, gravity) plus one projection as the reference
-orders it: real neighbour rebuild + densities, virtual generation,
-right-hand side, Laplacian diagonal, CG to 1e-6, pressure gradient.  All
-state is resident in HBM before the timed region.
+DualParticleSphLiquid::advance with the demos' box collider and gravity (no
+viscosity): advection and collisions, real neighbour rebuild + densities,
+virtual generation, right-hand side, Laplacian diagonal, CG to 1e-6,
+pressure gradient.  All state is resident in HBM before the timed region.
 
-Prints ONE JSON line (see the fields below).  The roofline object is for
-the dominant kernel (the CG SpMV), timed live with HIP events on the
-library stream; cpu_baseline is the FP64 oracle in its reference-structured
-mode (hash searcher, full-lattice scan, serial triplet assembly, CG on the
-transposed CSR matrix) on a bounded sample of the same scene.
+Prints ONE JSON line.  `roofline` is the dominant kernel (the one-launch CG
+iteration), timed live with HIP events on the library stream; `config3` is
+the same scene at 8,100,000 particles; `warm_start_comparison` the learned
+CG warm start against the other starts on its own schedule (500 training
+steps, 500 measured); `field_errors` the GPU against the FP64 oracle in
+lockstep on a bounded scene; `cpu_baseline` the FP64 oracle in its
+reference-structured mode (hash searcher, full-lattice scan, serial triplet
+assembly, CG on the transposed CSR matrix) on a bounded sample of the same
+scene, with all host threads and with the reference demos' third of them.
 """
 import argparse
 import json
@@ -29,7 +33,7 @@ METRIC = "projection particle-steps/s + CG iters at 1e-6 residual, learned vs pl
 HBM_PEAK_GBS = 8000.0
 
 WARM = {"index_carry": 0, "zero": 1, "node_keyed": 2, "learned": 3,
-        "extrapolate": 4}
+        "extrapolate": 4, "learned_online": 3}
 
 
 def parse():
@@ -43,8 +47,18 @@ def parse():
     ap.add_argument("--warm-start", default="index_carry", choices=sorted(WARM),
                     help="warm start of the headline (plain) run")
     ap.add_argument("--train-steps", type=int, default=30,
-                    help="untimed steps before --warmup in every run; the "
-                         "learned run trains its network online during them")
+                    help="untimed steps before --warmup in the headline run")
+    ap.add_argument("--c4-train", type=int, default=500,
+                    help="warm-start comparison (the learned CG warm start at "
+                         "1M): untimed steps of every run; the learned run "
+                         "trains its network online during them")
+    ap.add_argument("--c4-steps", type=int, default=500,
+                    help="warm-start comparison: measured steps of every run")
+    ap.add_argument("--c4-modes",
+                    default="learned,learned_online,extrapolate,index_carry,node_keyed,zero",
+                    help="warm starts of the comparison (learned_online: the learned start "
+                         "that keeps training, one Adam step per solve, during the "
+                         "measured steps)")
     ap.add_argument("--train-per-solve", type=int, default=4,
                     help="Adam steps per solve while training")
     ap.add_argument("--batch", type=int, default=65536)
@@ -55,6 +69,17 @@ def parse():
     ap.add_argument("--cpu-dx", type=float, default=1.0 / 60,
                     help="lattice spacing of the CPU-baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="also time the CPU baseline at the headline size "
+                         "(1,012,500 particles; several minutes)")
+    ap.add_argument("--no-config3", action="store_true",
+                    help="skip the 8,100,000-particle run (dx = 1/300)")
+    ap.add_argument("--c3-steps", type=int, default=5)
+    ap.add_argument("--no-field-errors", action="store_true",
+                    help="skip the GPU vs FP64 oracle lockstep")
+    ap.add_argument("--fe-dx", type=float, default=1.0 / 60,
+                    help="spacing of the lockstep scene of the field errors")
+    ap.add_argument("--fe-steps", type=int, default=3)
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--workload", default="dam_break", choices=["dam_break", "sand_water"],
@@ -109,47 +134,168 @@ def traffic_from_profiles(workload):
     return None
 
 
-def cpu_baseline(args):
-    """Reference-structured FP64 oracle on the same scene at a coarser
-    spacing (bounded sample), median step time -> particle-steps/s."""
+def _host_threads():
+    """Host threads of this process: OMP_NUM_THREADS when set (the CPU share
+    of the machine), else the CPU count."""
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+
+
+def cpu_time_steps(sc, args, threads, steps):
+    """Median time of `steps` projection steps (after 1 warm-up) of the
+    reference-structured FP64 oracle on scene sc with `threads` OpenMP
+    threads; each step is advection (untimed) + projection_step."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
-    from vps import scenes, _abi
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    sc = scenes.config2(seed=args.seed, dx=args.cpu_dx)
+    from vps import _abi
     o = oracle_py.Oracle(sc, search=oracle_py.SEARCH_HASH, threads=threads,
                          rest_density_mode=_abi.FROM_PARAMS,
                          kappa_mode=_abi.FROM_PARAMS)
     o.initialize()
-    times = []
-    iters = []
-    for _ in range(args.cpu_steps + 1):
+    times, iters = [], []
+    for _ in range(steps + 1):
         o.advect(args.dt, 9.8)
         t0 = time.perf_counter()
         st = o.projection_step()
         times.append(time.perf_counter() - t0)
         iters.append(st["cg_iterations"])
-    times = times[1:]   # first step is the warm-up
-    med = float(np.median(times))
     o.close()
+    return float(np.median(times[1:])), iters[1:]
+
+
+def cpu_baseline(args, dx=None, steps=None):
+    """Reference-structured FP64 oracle on the same scene (at dx, by default
+    the bounded sample's spacing): particle-steps/s with all host threads
+    and with the reference demos' max(threads / 3, 1)
+    (Demos/SmthPartHydrodLiquidTest/SmthPartHydrodLiquidTest.cpp:28)."""
+    from vps import scenes
+    dx = dx or args.cpu_dx
+    steps = steps or args.cpu_steps
+    sc = scenes.config2(seed=args.seed, dx=dx)
+    n = len(sc["x"])
+    all_t = _host_threads()
+    demo_t = max(all_t // 3, 1)
+    med_all, it_all = cpu_time_steps(sc, args, all_t, steps)
+    med_demo, it_demo = cpu_time_steps(sc, args, demo_t, steps)
     return {
-        "value": len(sc["x"]) / med,
+        "value": n / med_all,
         "unit": "particle-steps/s",
-        "cores": threads,
+        "cores": all_t,
         "kind": "port",
         "sample": "FP64 oracle, reference-structured (OR-hash searcher, full "
                   "lattice scan, serial triplet assembly, CG on A^T CSR), "
                   "same dam-break scene at dx=%.5g (%d particles), median of "
                   "%d steps after 1 warm-up, %d OpenMP threads; CG iters %s"
-                  % (args.cpu_dx, len(sc["x"]), len(times), threads, iters[1:]),
+                  % (dx, n, steps, all_t, it_all),
+        "n_particles": n,
+        "ms_per_step": round(1e3 * med_all, 1),
+        "demo_threads": {"cores": demo_t, "value": n / med_demo,
+                         "ms_per_step": round(1e3 * med_demo, 1),
+                         "cg_iters": it_demo},
+        "cpu_model": _cpu_model(),
+        "nproc": os.cpu_count(),
     }
 
 
-def run(args, sc, warm, train=False, kernel_timing=False):
-    """One run from the initial scene: train_steps + warmup untimed steps
-    (the learned run trains online during the first train_steps), then
-    `steps` timed steps, bracketed by device synchronisation."""
-    from vps import Projection, _abi
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def field_errors(args):
+    """The GPU path and the FP64 oracle in lockstep on the same dam-break
+    scene (bounded size, dx = --fe-dx) for --fe-steps steps: per output
+    field, max |gpu - oracle| / max |oracle| and RMS(gpu - oracle) /
+    RMS(oracle) after each step (the worst step is reported), for the real
+    density and frozen volume (SmoothedParticles.cpp:25-36), the virtual
+    density, volume and divergence (VirtualParticle.cpp:18-38, 171-201), the
+    CG solution (IterativeSolver.h:34) and the updated velocity
+    (VirtualParticle.cpp:110-134)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    from vps import Projection, scenes, _abi
+    sc = scenes.config2(seed=args.seed, dx=args.fe_dx)
+    kw = dict(rest_density_mode=_abi.FROM_PARAMS, kappa_mode=_abi.FROM_PARAMS)
+    g = Projection(sc, **kw)
+    o = oracle_py.Oracle(sc, threads=_host_threads(), **kw)
+    g.initialize()
+    o.initialize()
+    worst = {}
+    sets_equal = True
+    iters = []
+
+    def err(name, a, b):
+        a = np.asarray(a, dtype=np.float64).ravel()
+        b = np.asarray(b, dtype=np.float64).ravel()
+        d = a - b
+        mx = float(np.abs(d).max() / max(np.abs(b).max(), 1e-300))
+        rms = float(np.sqrt(np.mean(d * d)) / max(np.sqrt(np.mean(b * b)), 1e-300))
+        w = worst.setdefault(name, {"max_rel": 0.0, "rms_rel": 0.0})
+        w["max_rel"] = max(w["max_rel"], mx)
+        w["rms_rel"] = max(w["rms_rel"], rms)
+
+    for _ in range(args.fe_steps):
+        g.advect(args.dt, 9.8)
+        o.advect(args.dt, 9.8)
+        sg = g.projection_step()
+        so = o.projection_step()
+        iters.append([sg["cg_iterations"], so["cg_iterations"]])
+        rg, vg = g.real_fields()
+        ro, vo = o.real_fields()
+        err("rho", rg, ro)
+        err("V", vg, vo)
+        Vg, Vo = g.virtual(), o.virtual()
+        if not np.array_equal(Vg["node"], Vo["node"]):
+            sets_equal = False
+            break
+        err("rho_I", Vg["rho"], Vo["rho"])
+        err("V_I", Vg["vol"], Vo["vol"])
+        err("D_I", Vg["div"], Vo["div"])
+        err("p", Vg["p"], Vo["p"])
+        _, velg = g.particles()
+        _, velo = o.particles()
+        err("v_next", velg, velo)
+    g.close()
+    o.close()
+    return {"scene": "dam_break_3d_%d" % len(sc["x"]), "dx": args.fe_dx,
+            "steps": args.fe_steps, "virtual_sets_equal": sets_equal,
+            "cg_iters_gpu_oracle": iters,
+            "fields": {k: {q: float("%.3g" % x) for q, x in v.items()} for k, v in worst.items()}}
+
+
+def boundary_near_fluid(x, bx, h):
+    """N_B': boundary particles with a fluid particle within h."""
+    from scipy.spatial import cKDTree
+    d, _ = cKDTree(x).query(bx, k=1, distance_upper_bound=h)
+    return int(np.count_nonzero(np.isfinite(d)))
+
+
+def step(proj, args, cols):
+    """One step: DualParticleSphLiquid::advance (DualParticleSphLiquid.cpp:
+    107-116) with the box collider, gravity and no viscosity: advection and
+    collisions, neighbour rebuild and densities, gravity, virtual generation,
+    projection."""
+    return proj.advance(args.dt, gravity=9.8, viscosity=0.0, colliders=cols,
+                        enable_gravity=True)
+
+
+def run(args, sc, warm, train=False, kernel_timing=False, untimed=None, warmup=None,
+        steps=None, keep_x=False, train_measured=0):
+    """One run from the initial scene: `untimed` steps (the learned run trains
+    its network online during them), `warmup` more untimed steps with the
+    network frozen, then `steps` timed steps, bracketed by device
+    synchronisation.  train_measured > 0 keeps training during the timed
+    steps with that many Adam steps per solve (their cost is timed)."""
+    from vps import Projection, _abi, scenes
+    cols = scenes.box_colliders(sc)
+    untimed = args.train_steps if untimed is None else untimed
+    warmup = args.warmup if warmup is None else warmup
+    steps = args.steps if steps is None else steps
     proj = Projection(sc, warm_start=WARM[warm],
                       rest_density_mode=_abi.FROM_PARAMS,
                       kappa_mode=_abi.FROM_PARAMS)
@@ -162,26 +308,28 @@ def run(args, sc, warm, train=False, kernel_timing=False):
     train_iters = []
     proj.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.train_steps):
-        proj.advect(args.dt, 9.8)
-        train_iters.append(proj.projection_step()["cg_iterations"])
+    for k in range(untimed):
+        train_iters.append(step(proj, args, cols)["cg_iterations"])
     proj.synchronize()
     t_train = time.perf_counter() - t0
+    adam = 0
     if train:
         loss, adam = proj.mlp_training_state()
-        proj.mlp_set_training(False)
+        if train_measured > 0:
+            proj.mlp_set_training(True, batch=args.batch, lr=args.lr,
+                                  steps_per_solve=train_measured, seed=args.seed + 1)
+        else:
+            proj.mlp_set_training(False)
         log("%s: %d training steps, %d Adam steps, last loss %.3g"
-            % (warm, args.train_steps, adam, loss))
-    for k in range(args.warmup):
-        proj.advect(args.dt, 9.8)
-        st = proj.projection_step()
+            % (warm, untimed, adam, loss))
+    for k in range(warmup):
+        st = step(proj, args, cols)
         log("%s warmup %d: nv=%d iters=%d" % (warm, k, st["n_virtual"], st["cg_iterations"]))
     stats = []
     proj.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        proj.advect(args.dt, 9.8)
-        stats.append(proj.projection_step())
+    for k in range(steps):
+        stats.append(step(proj, args, cols))
     proj.synchronize()
     elapsed = time.perf_counter() - t0
     # per-launch HIP events on the library stream, after the timed steps
@@ -190,17 +338,18 @@ def run(args, sc, warm, train=False, kernel_timing=False):
     if kernel_timing:
         proj.set_kernel_timing(True)
         for k in range(5):
-            proj.advect(args.dt, 9.8)
-            timed_launch.append(proj.projection_step())
+            timed_launch.append(step(proj, args, cols))
         proj.set_kernel_timing(False)
     nb = proj.nb
+    x_end = proj.particles()[0] if keep_x else None
     proj.close()
     if args.verbose:
         for s in stats:
             log(json.dumps({k: (round(v, 4) if isinstance(v, float) else v)
                             for k, v in s.items()}))
-    return dict(stats=stats, elapsed=elapsed, nb=nb, t_train=t_train,
-                train_iters=train_iters, timed_launch=timed_launch)
+    return dict(stats=stats, elapsed=elapsed, nb=nb, t_train=t_train, steps=steps,
+                train_iters=train_iters, timed_launch=timed_launch, adam=adam,
+                x_end=x_end)
 
 
 def sand_water_scenes(dx, seed, jitter):
@@ -311,17 +460,7 @@ def main_sand_water(args):
     water.close()
     ms_step = 1e3 * elapsed / args.steps
     iters = [s["cg_iterations"] for s in stats]
-    spmv = [s["ms_spmv_avg"] for s in kt if s["ms_spmv_avg"] > 0]
-    roof = None
-    if spmv:
-        t_launch = float(np.mean(spmv)) * 1e-3
-        bytes_launch = SPMV_BYTES * float(np.mean([s["n_virtual"] for s in kt]))
-        ach = bytes_launch / t_launch / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": SPMV_KERNEL,
-                "avg_launch_us": round(t_launch * 1e6, 2),
-                "bytes_per_launch": bytes_launch}
+    roof = cg_roofline(kt, "sand_water_3d_%d_%d" % (n, ng))
     out = {
         "metric": METRIC,
         "value": round(n * args.steps / elapsed, 1),
@@ -359,8 +498,9 @@ def main_sand_water(args):
     return 0
 
 
-def summary(r, n, steps):
+def summary(r, n, steps=None):
     st = r["stats"]
+    steps = steps or r["steps"]
     return {
         "value": round(n * steps / r["elapsed"], 1),
         "ms_per_step": round(1e3 * r["elapsed"] / steps, 4),
@@ -373,7 +513,135 @@ def summary(r, n, steps):
                                                    for s in st])), 4),
         "true_rel_residual_max": max(s["cg_true_rel_residual"] for s in st),
         "replacements_mean": round(float(np.mean([s["cg_replacements"] for s in st])), 2),
+        "fallbacks": int(sum(s["warm_start_fallback"] for s in st)),
     }
+
+
+# Algorithmic bytes per CG iteration and virtual particle: read p, diag, V
+# and the node map and write A p (20 B), the x / r update with r.r (24 B),
+# the p update (12 B); the one-launch iteration kernel fuses them and moves
+# 40 B (read p, r, t, x, V, diag; write p, r, t, x).
+CG_BYTES = 56.0
+
+
+def cells_of(sc):
+    h = 4.0 * sc["particle_radius"]
+    ext = (np.asarray(sc["domain_hi"]) - np.asarray(sc["domain_lo"]))[:3]
+    return float(np.prod(np.ceil(ext / h)))
+
+
+def phase_bytes(n, nv, nbp, cells, k):
+    """Algorithmic bytes of each timed phase of a step (FP32 fields):
+    sort + density 48 N + 8 C + 16 N; generation (virtual marking, fused
+    rho_I, V_I, vbar, D_I) 12 N + 8 N_V + 32 N + 40 N_B' + 16 N_V; diagonal
+    12 N_V; CG 32 N_V + 56 k N_V; gradient 36 N + 8 N_V + 40 N_B'.  Total
+    144 N + 76 N_V + 80 N_B' + 8 C + 56 k N_V."""
+    return {"ms_sort": 64.0 * n + 8.0 * cells,
+            "ms_generate": 44.0 * n + 24.0 * nv + 40.0 * nbp,
+            "ms_diag": 12.0 * nv,
+            "ms_cg": 32.0 * nv + CG_BYTES * k * nv,
+            "ms_gradient": 36.0 * n + 8.0 * nv + 40.0 * nbp}
+
+
+def cg_roofline(kt, workload):
+    """Roofline of the CG iteration kernel from its per-launch HIP events."""
+    spmv = [s["ms_spmv_avg"] for s in kt if s["ms_spmv_avg"] > 0]
+    if not spmv:
+        return None
+    t_launch = float(np.mean(spmv)) * 1e-3
+    nv_kt = float(np.mean([s["n_virtual"] for s in kt]))
+    b56, b40 = CG_BYTES * nv_kt, SPMV_BYTES * nv_kt
+    a56, a40 = b56 / t_launch / 1e9, b40 / t_launch / 1e9
+    return {"bound": "hbm", "achieved": round(a56, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(a56 / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(workload),
+            "kernel": SPMV_KERNEL,
+            "avg_launch_us": round(t_launch * 1e6, 2),
+            "bytes_per_launch": b56,
+            "basis": "56 B per virtual particle per iteration (SpMV 20, x/r update 24, "
+                     "p update 12)",
+            "achieved_fused_40B": round(a40, 1),
+            "frac_fused_40B": round(a40 / HBM_PEAK_GBS, 4),
+            "n_virtual_per_launch": nv_kt,
+            "slot_nodes_per_launch": 512 * float(np.mean([s["n_active_bricks"] for s in kt]))}
+
+
+def size_record(args, r, sc, n, workload):
+    """ms/step, CG iterations, phase times with their algorithmic GB/s, the
+    CG kernel's roofline and the whole-step roofline of one run."""
+    stats = r["stats"]
+    ms_step = 1e3 * r["elapsed"] / r["steps"]
+    nvm = float(np.mean([s["n_virtual"] for s in stats]))
+    kmean = float(np.mean([s["cg_iterations"] for s in stats]))
+    cells = cells_of(sc)
+    nbp = boundary_near_fluid(r["x_end"], sc["bx"], 4.0 * sc["particle_radius"]) \
+        if r["x_end"] is not None else 0
+    pb = phase_bytes(n, nvm, nbp, cells, kmean)
+    phases = {}
+    for k, b in pb.items():
+        ms = float(np.mean([s[k] for s in stats]))
+        phases[k] = {"ms": round(ms, 4), "algorithmic_GBs": round(b / (ms * 1e-3) / 1e9, 1)}
+    b_step = sum(pb.values())
+    return {
+        "workload": workload, "n_real": n, "n_boundary": r["nb"], "n_boundary_near_fluid": nbp,
+        "n_virtual_mean": round(nvm, 1), "cells": cells,
+        "value": round(n * r["steps"] / r["elapsed"], 1),
+        "ms_per_step": round(ms_step, 4),
+        "cg_iters_mean": round(kmean, 2),
+        "cg_iters": [s["cg_iterations"] for s in stats],
+        "true_rel_residual_max": max(s["cg_true_rel_residual"] for s in stats),
+        "cg_replacements": [s["cg_replacements"] for s in stats],
+        "phases": phases,
+        "roofline": cg_roofline(r["timed_launch"], workload),
+        "step_roofline": {"bytes_per_step": b_step,
+                          "achieved": round(b_step / (ms_step * 1e-3) / 1e9, 1),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(b_step / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "basis": "144 N + 76 N_V + 80 N_B' + 8 C + 56 k N_V, step time "
+                                   "incl. advection"},
+    }
+
+
+def warm_start_comparison(args, sc, n):
+    """Learned CG warm start at 1M (its own schedule): every run starts from
+    the same scene, takes --c4-train untimed steps (the learned run trains
+    online on them: Adam, batch --batch, lr --lr), then --c4-steps measured
+    steps with the network frozen; CG always runs to 1e-6."""
+    modes = [m for m in args.c4_modes.split(",") if m]
+    runs = {}
+    for m in modes:
+        online = m == "learned_online"
+        r = run(args, sc, "learned" if online else m, train=m.startswith("learned"),
+                untimed=args.c4_train, warmup=0, steps=args.c4_steps,
+                train_measured=1 if online else 0)
+        d = summary(r, n)
+        if m.startswith("learned"):
+            d["training"] = {"steps": args.c4_train, "adam_steps": r["adam"],
+                             "adam_per_solve": args.train_per_solve,
+                             "batch": args.batch, "lr": args.lr,
+                             "ms_per_step_while_training": round(
+                                 1e3 * r["t_train"] / max(args.c4_train, 1), 4),
+                             "cg_iters_while_training_first_last_50": [
+                                 round(float(np.mean(r["train_iters"][:50])), 2),
+                                 round(float(np.mean(r["train_iters"][-50:])), 2)]}
+        runs[m] = d
+    out = {"schedule": {"train_steps": args.c4_train, "measured_steps": args.c4_steps,
+                        "dt": args.dt, "tolerance": 1e-6},
+           "runs": runs}
+
+    def ratio(a, b):
+        A, B = runs[a], runs[b]
+        return {"iters": round(B["cg_iters_mean"] / max(A["cg_iters_mean"], 1e-9), 3),
+                "cg_ms": round(B["cg_ms_mean"] / max(A["cg_ms_mean"], 1e-9), 3),
+                "projection_ms": round(B["projection_ms_mean"] / max(A["projection_ms_mean"], 1e-9), 3),
+                "step_ms": round(B["ms_per_step"] / A["ms_per_step"], 3)}
+    for a in modes:
+        for b in modes:
+            if a.startswith("learned") and not b.startswith("learned"):
+                out[a + "_speedup_vs_" + b] = ratio(a, b)
+    if "extrapolate" in runs and "index_carry" in runs:
+        out["extrapolate_speedup_vs_index_carry"] = ratio("extrapolate", "index_carry")
+    return out
 
 
 def main():
@@ -390,50 +658,17 @@ def main():
     sc = scenes.config2(seed=args.seed, dx=args.dx)
     n = len(sc["x"])
     workload = "dam_break_3d_%d" % n
-    plain = run(args, sc, args.warm_start, kernel_timing=not args.no_kernel_timing)
-    stats = plain["stats"]
-    elapsed = plain["elapsed"]
-    ms_step = 1e3 * elapsed / args.steps
-    iters = [s["cg_iterations"] for s in stats]
-    nv = [s["n_virtual"] for s in stats]
-    # roofline of the CG iteration kernel (dominant kernel of the plain step):
-    # algorithmic bytes per launch = 40 B per virtual particle (read p_old,
-    # r_old, t_old, x, V, diag; write p, r, t, x, all FP32)
-    kt = plain["timed_launch"]
-    spmv = [s["ms_spmv_avg"] for s in kt if s["ms_spmv_avg"] > 0]
-    roof = None
-    if spmv:
-        t_launch = float(np.mean(spmv)) * 1e-3
-        nv_kt = float(np.mean([s["n_virtual"] for s in kt]))
-        bytes_launch = SPMV_BYTES * nv_kt
-        ach = bytes_launch / t_launch / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles(workload),
-                "kernel": SPMV_KERNEL,
-                "avg_launch_us": round(t_launch * 1e6, 2),
-                "bytes_per_launch": bytes_launch,
-                "slot_nodes_per_launch": 512 * float(np.mean([s["n_active_bricks"] for s in kt]))}
-    # whole-step algorithmic bytes: 144 N + 76 N_V + 8 C + 40 k N_V (cell
-    # sort, densities, virtual marking and sums, diagonal, CG init, k CG
-    # iterations of the 40 B one-launch update + SpMV, gradient; the
-    # boundary term 80 N_B' is left out, so the fraction is a lower bound)
-    h = 2.0 * sc["particle_radius"] * 2.0
-    cells = float(np.prod(np.ceil((np.asarray(sc["domain_hi"]) - np.asarray(sc["domain_lo"]))[:3] / h)))
-    nvm = float(np.mean(nv))
-    kmean = float(np.mean(iters))
-    b_step = 144.0 * n + 76.0 * nvm + 8.0 * cells + SPMV_BYTES * kmean * nvm
-    step_roof = {"bytes_per_step": b_step, "achieved": round(b_step / (ms_step * 1e-3) / 1e9, 1),
-                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                 "frac": round(b_step / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    plain = run(args, sc, args.warm_start, kernel_timing=not args.no_kernel_timing,
+                keep_x=True)
+    rec = size_record(args, plain, sc, n, workload)
     out = {
         "metric": METRIC,
-        "value": round(n * args.steps / elapsed, 1),
+        "value": rec["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": rec["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -444,40 +679,49 @@ def main():
                    "warm_start": args.warm_start, "tolerance": 1e-6,
                    "untimed_steps": args.train_steps + args.warmup,
                    "parallelism": "single GPU"},
-        "cg_iters_mean": round(float(np.mean(iters)), 2),
-        "cg_iters": iters,
-        "n_virtual_mean": round(float(np.mean(nv)), 1),
-        "true_rel_residual_max": max(s["cg_true_rel_residual"] for s in stats),
-        "cg_replacements": [s["cg_replacements"] for s in stats],
-        "phase_ms_mean": {k: round(float(np.mean([s[k] for s in stats])), 4)
+        "cg_iters_mean": rec["cg_iters_mean"],
+        "cg_iters": rec["cg_iters"],
+        "n_virtual_mean": rec["n_virtual_mean"],
+        "n_boundary_near_fluid": rec["n_boundary_near_fluid"],
+        "true_rel_residual_max": rec["true_rel_residual_max"],
+        "cg_replacements": rec["cg_replacements"],
+        "phase_ms_mean": {k: round(float(np.mean([s[k] for s in plain["stats"]])), 4)
                           for k in ("ms_sort", "ms_generate", "ms_diag",
                                     "ms_cg", "ms_gradient", "ms_total")},
-        "roofline": roof,
-        "step_roofline": step_roof,
+        "phases": rec["phases"],
+        "roofline": rec["roofline"],
+        "step_roofline": rec["step_roofline"],
         "cpu_baseline": None,
     }
+    # config 2 as specified: 200 steps from the initial scene, plain start
+    r200 = run(args, sc, args.warm_start, untimed=0, warmup=0, steps=200)
+    s200 = summary(r200, n)
+    out["config2_200_steps"] = {k: s200[k] for k in ("value", "ms_per_step", "cg_iters_mean",
+                                                     "true_rel_residual_max")}
+    out["config2_200_steps"]["cg_iters_first_last_50"] = [
+        round(float(np.mean([q["cg_iterations"] for q in r200["stats"][:50]])), 2),
+        round(float(np.mean([q["cg_iterations"] for q in r200["stats"][-50:]])), 2)]
+    if not args.no_config3:
+        sc3 = scenes.config3(seed=args.seed)
+        n3 = len(sc3["x"])
+        r3 = run(args, sc3, args.warm_start, kernel_timing=not args.no_kernel_timing,
+                 untimed=3, warmup=0, steps=args.c3_steps, keep_x=True)
+        out["config3"] = size_record(args, r3, sc3, n3, "dam_break_3d_%d" % n3)
+        out["config3"]["dx"] = 1.0 / 300
+        del sc3
     if not args.no_learned:
-        base = summary(plain, n, args.steps)
-        lr = run(args, sc, "learned", train=True)
-        ex = run(args, sc, "extrapolate")
-        L = summary(lr, n, args.steps)
-        E = summary(ex, n, args.steps)
-        for d in (L, E):
-            d["iters_ratio_vs_plain"] = round(base["cg_iters_mean"] / max(d["cg_iters_mean"], 1e-9), 3)
-            d["step_speedup_vs_plain"] = round(base["ms_per_step"] / d["ms_per_step"], 3)
-            d["cg_speedup_vs_plain"] = round(base["cg_ms_mean"] / max(d["cg_ms_mean"], 1e-9), 3)
-            d["projection_speedup_vs_plain"] = round(
-                base["projection_ms_mean"] / max(d["projection_ms_mean"], 1e-9), 3)
-        L["training"] = {"steps": args.train_steps, "adam_per_solve": args.train_per_solve,
-                         "batch": args.batch, "lr": args.lr,
-                         "ms_per_step_while_training": round(1e3 * lr["t_train"] / max(args.train_steps, 1), 4),
-                         "cg_iters_while_training": lr["train_iters"]}
-        out["learned_warm_start"] = L
-        out["extrapolated_warm_start"] = E
-        out["plain_cg_ms_mean"] = base["cg_ms_mean"]
-        out["plain_projection_ms_mean"] = base["projection_ms_mean"]
+        cmp = warm_start_comparison(args, sc, n)
+        out["warm_start_comparison"] = cmp
+        if "learned" in cmp["runs"]:
+            out["learned_warm_start"] = cmp["runs"]["learned"]
+        if "extrapolate" in cmp["runs"]:
+            out["extrapolated_warm_start"] = cmp["runs"]["extrapolate"]
+    if not args.no_field_errors:
+        out["field_errors"] = field_errors(args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args)
+        if args.cpu_full:
+            out["cpu_baseline_full"] = cpu_baseline(args, dx=args.dx, steps=3)
     print(json.dumps(out), flush=True)
     return 0
 

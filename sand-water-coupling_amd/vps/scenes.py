@@ -112,8 +112,18 @@ def dam_break(dim, dx, box, block, seed=DEFAULT_SEED, wall_layers=3,
         "kappa": mass / rho0,
         "domain_lo": lo,
         "domain_hi": hi,
+        "box_hi": np.concatenate([box, np.zeros(3 - dim)]),
         "seed": seed,
     }
+
+
+def box_colliders(scene):
+    """The static collider of the reference's liquid demos: the complement
+    of the box [0, box] with restitution and friction 0
+    (Demos/SmthPartHydrodLiquidTest/SmthPartHydrodLiquidBuilder.h:47-49)."""
+    from . import _abi
+    return [_abi.collider(_abi.SURFACE_BOX, [0.0, 0.0, 0.0], list(scene["box_hi"]),
+                          complement=True)]
 
 
 def config1(seed=DEFAULT_SEED):
