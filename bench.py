@@ -72,6 +72,9 @@ def parse():
     ap.add_argument("--cpu-full", action="store_true",
                     help="also time the CPU baseline at the headline size "
                          "(1,012,500 particles; several minutes)")
+    ap.add_argument("--c2-steps", type=int, default=200,
+                    help="steps of the config-2 record run from the initial scene "
+                         "(0 skips it)")
     ap.add_argument("--no-config3", action="store_true",
                     help="skip the 8,100,000-particle run (dx = 1/300)")
     ap.add_argument("--c3-steps", type=int, default=5)
@@ -694,13 +697,16 @@ def main():
         "cpu_baseline": None,
     }
     # config 2 as specified: 200 steps from the initial scene, plain start
-    r200 = run(args, sc, args.warm_start, untimed=0, warmup=0, steps=200)
-    s200 = summary(r200, n)
-    out["config2_200_steps"] = {k: s200[k] for k in ("value", "ms_per_step", "cg_iters_mean",
-                                                     "true_rel_residual_max")}
-    out["config2_200_steps"]["cg_iters_first_last_50"] = [
-        round(float(np.mean([q["cg_iterations"] for q in r200["stats"][:50]])), 2),
-        round(float(np.mean([q["cg_iterations"] for q in r200["stats"][-50:]])), 2)]
+    if args.c2_steps > 0:
+        r200 = run(args, sc, args.warm_start, untimed=0, warmup=0, steps=args.c2_steps)
+        s200 = summary(r200, n)
+        rec200 = {k: s200[k] for k in ("value", "ms_per_step", "cg_iters_mean",
+                                       "true_rel_residual_max")}
+        rec200["steps"] = args.c2_steps
+        rec200["cg_iters_first_last_50"] = [
+            round(float(np.mean([q["cg_iterations"] for q in r200["stats"][:50]])), 2),
+            round(float(np.mean([q["cg_iterations"] for q in r200["stats"][-50:]])), 2)]
+        out["config2_200_steps"] = rec200
     if not args.no_config3:
         sc3 = scenes.config3(seed=args.seed)
         n3 = len(sc3["x"])

@@ -166,7 +166,12 @@ __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const flo
     const float qo1 = (float)(q.y - cell_corner(k, 1, cc1));
     const float qo2 = (float)(q.z - cell_corner(k, 2, cc2));
     const bool qfar = !(fabsf(qo0) <= far && fabsf(qo1) <= far && fabsf(qo2) <= far);
-    const float h2 = (float)k.h2;
+    // W / c0 = 2 max(1 - x, 0)^3 - max(1 - 2x, 0)^3: the two pieces of the
+    // cubic spline in one expression (for x < 1/2 it expands to
+    // 6 x^2 (x - 1) + 1), zero from x = 1 on, so no support test is needed;
+    // c0 m is applied once to the sum.  One loop per row of three cells (the
+    // lanes of a wave sit in different cells: per-cell loops would run the
+    // longest cell of every lane in turn).
     float sum = 0.f;
     for (int z = zlo; z <= zhi; z++) {
         const float tz = (float)(q.z - cell_corner(k, 2, z));
@@ -186,12 +191,13 @@ __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const flo
                     d2 = (float)sq_dist3(pj.x, pj.y, pj.z, q.x, q.y, q.z);
                 }
                 // v_sqrt_f32 (1 ulp): W is rounded to FP32 anyway
-                const float w = spline_w(__builtin_amdgcn_sqrtf(d2) * k.inv_h, k.c0h);
-                sum += d2 < h2 ? w : 0.f;
+                const float x = __builtin_amdgcn_sqrtf(d2) * k.inv_h;
+                const float a = fmaxf(1.f - x, 0.f), c = fmaxf(fmaf(-2.f, x, 1.f), 0.f);
+                sum += fmaf(2.f * a, a * a, -(c * c * c));
             }
         }
     }
-    const float r = k.mass * sum;
+    const float r = k.mass * k.c0h * sum;
     if constexpr (BND) {
         nrm[i].w = r;
         vel[i].w = k.mass / r;

@@ -528,6 +528,22 @@ struct IterArgs {
     int resume;              // alpha, beta from the state (after a replacement)
 };
 
+// Block sums of four per-thread values v[0..3] in a fixed order: the 4 x 256
+// values go through LDS scratch (8 KB), wave w adds the four values of lane
+// l, l + 64, l + 128, l + 192 of sum w and reduces them with one DPP tree;
+// out[w] (LDS) holds sum w for the whole block afterwards.
+__device__ __forceinline__ void block_sums4(const double v[4], double* scratch, double* out) {
+    const int t = threadIdx.x, l = t & 63, wv = t >> 6;
+#pragma unroll
+    for (int q = 0; q < 4; q++) scratch[q * kS3Threads + t] = v[q];
+    __syncthreads();
+    const double* a = scratch + kS3Threads * wv;
+    double s = (a[l] + a[l + 64]) + (a[l + 128] + a[l + 192]);
+    s = wave_sum_all(s);
+    if (l == 0) out[wv] = s;
+    __syncthreads();
+}
+
 template <int OP>
 __global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc, IterArgs it) {
     __shared__ float tile[kT * kPS];
@@ -553,6 +569,11 @@ __global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc
     auto slot_of = [nsl, G](int v) { return (G & 7) == 0 ? xcd_remap(v, nsl) : v; };
     int slot = slot_of(blockIdx.x);
     if (t < 27) sbase[t] = (uint32_t)a.nbr_slot[(int64_t)slot * 27 + t] * 512u;
+    // the neighbour slots of the block's next brick, loaded one brick ahead
+    // so that a brick's halo loads do not wait for them
+    int nb_next = 0;
+    if (t < 27 && (int)blockIdx.x + G < nsl)
+        nb_next = a.nbr_slot[(int64_t)slot_of(blockIdx.x + G) * 27 + t];
     __syncthreads();
     // halo loads of a brick (see k_stencil3) and its epilogue operands
     float4 ra[kUnitIters], rb[kUnitIters], rv[kUnitIters], rc[kUnitIters], xv[kUnitIters];
@@ -607,17 +628,13 @@ __global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc
 #pragma unroll
             for (int q = 0; q < 4; q++) sv[q] += (v4[q][0] + v4[q][1]) + (v4[q][2] + v4[q][3]);
         }
-#pragma unroll
-        for (int q = 0; q < 4; q++) sv[q] = wave_sum_all(sv[q]);
-        if (l == 0) {
-#pragma unroll
-            for (int q = 0; q < 4; q++) sred[q][wv] = sv[q];
-        }
-        __syncthreads();
+        // the four sums through LDS (psum is free here): wave w reduces sum
+        // w with one DPP tree instead of every wave reducing all four
+        block_sums4(sv, reinterpret_cast<double*>(&psum[0][0][0]), sred[0]);
         if (t == 0) {
             double sum[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) sum[q] = (sred[q][0] + sred[q][1]) + (sred[q][2] + sred[q][3]);
+            for (int q = 0; q < 4; q++) sum[q] = sred[0][q];
             const double pt = sum[0], rt = sum[1], tt = sum[2], rr = sum[3];
             const int k = a.iter_base + kk - 1;   // Eigen's index of the reduced SpMV
             const double ald = rr / pt;
@@ -676,7 +693,10 @@ __global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc
             slot = slot_of(v);
             base = (int64_t)slot * 512;
             __syncthreads();   // the previous brick's LDS has been read
-            if (t < 27) sbase[t] = (uint32_t)a.nbr_slot[(int64_t)slot * 27 + t] * 512u;
+            if (t < 27) {
+                sbase[t] = (uint32_t)nb_next * 512u;
+                if (v + G < nsl) nb_next = a.nbr_slot[(int64_t)slot_of(v + G) * 27 + t];
+            }
             __syncthreads();
         }
         issue();
@@ -752,22 +772,13 @@ __global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc
             red3 += rvv * rvv;
         }
     }
-    // this block's partials: DPP within waves, the waves in order
-    red0 = wave_sum_all(red0);
-    red1 = wave_sum_all(red1);
-    red2 = wave_sum_all(red2);
-    red3 = wave_sum_all(red3);
-    __syncthreads();   // sred of the reduction above has been read
-    if (l == 0) {
-        sred[0][wv] = red0;
-        sred[1][wv] = red1;
-        sred[2][wv] = red2;
-        sred[3][wv] = red3;
-    }
-    __syncthreads();
+    // this block's partials, one DPP tree per wave (see block_sums4)
+    double rsum[4] = {red0, red1, red2, red3};
+    __syncthreads();   // the last brick's plane sums have been read
+    block_sums4(rsum, reinterpret_cast<double*>(&psum[0][0][0]), sred[1]);
     if (t < 4) {
         double* dst = t == 0 ? it.out[0] : (t == 1 ? it.out[1] : (t == 2 ? it.out[2] : it.out[3]));
-        dst[blockIdx.x] = (sred[t][0] + sred[t][1]) + (sred[t][2] + sred[t][3]);
+        dst[blockIdx.x] = sred[1][t];
     }
 }
 
@@ -1316,6 +1327,30 @@ static double ms_between(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+// Persistent blocks of the 3D iteration: as many as are resident at once
+// (blocks per CU from the occupancy of k_cg_iter3 times the CUs, at most
+// kIterBlocks), so that no block waits for another to finish.
+static int iter_blocks(vps_context* c) {
+    static int cached[64][2] = {};
+    const int dev = c->device >= 0 && c->device < 64 ? c->device : 0;
+    const int op = c->p.op_mode == VPS_OP_ROW ? 1 : 0;
+    if (cached[dev][op] > 0) return cached[dev][op];
+    int per_cu = 0, cus = 0;
+    const auto kern = c->p.op_mode == VPS_OP_ROW ? (const void*)k_cg_iter3<VPS_OP_ROW>
+                                                 : (const void*)k_cg_iter3<VPS_OP_TRANSPOSE>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kS3Threads, 0) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) !=
+            hipSuccess ||
+        per_cu <= 0 || cus <= 0) {
+        (void)hipGetLastError();
+        return kIterBlocks;
+    }
+    const int n = std::max(8, std::min(kIterBlocks, per_cu * cus) & ~7);
+    cached[dev][op] = n;
+    return n;
+}
+
 // residual replacements of one solve before it fails (the FP64 base of x
 // makes the true residual reachable: one or two are the rule)
 constexpr int kMaxReplacements = 8;
@@ -1405,7 +1440,7 @@ int cg_solve(vps_context* c, vps_stats* stats) {
     float* tbuf[2] = {t, c->vt2.p};
     bool resume_pending = false;
     // 3D: persistent blocks of the iteration and its two partial sets
-    const int giter = nslots >= 8 ? (std::min(nslots, kIterBlocks) & ~7) : nslots;
+    const int giter = nslots >= 8 ? (std::min(nslots, iter_blocks(c)) & ~7) : nslots;
     double* setA[4] = {part_pt(c, 0), part_pt(c, 1), part_pt(c, 2), part_init(c, 1)};
     double* setB[4] = {part_iter(c, 0), part_iter(c, 1), part_iter(c, 2), part_iter(c, 3)};
     std::vector<int> timed_iter;
