@@ -124,17 +124,29 @@ SPMV_BYTES = 40.0
 
 def traffic_from_profiles(workload):
     """HBM bytes per launch of the CG iteration kernel measured with
-    rocprofv3 counters, if a summary for this workload and kernel is
-    committed under profiles/."""
+    rocprofv3 counters (FETCH_SIZE and WRITE_SIZE, calibrated on a plain
+    copy), if committed under profiles/ for this workload."""
     path = os.path.join(ROOT, "profiles", "spmv_traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
-        if d.get("workload") == workload and d.get("kernel") == SPMV_KERNEL:
-            return d.get("bytes_per_launch")
+        if d.get("kernel") == SPMV_KERNEL:
+            return d.get("workloads", {}).get(workload, {}).get("bytes_per_launch")
     except (OSError, ValueError):
         pass
     return None
+
+
+def cpu_full_from_profiles():
+    """The CPU baseline at the headline size (1,012,500 particles), measured
+    by `bench.py --cpu-full` on the GPU box host and committed under
+    profiles/ (it takes minutes, so the default run times a bounded sample)."""
+    path = os.path.join(ROOT, "profiles", "cpu_baseline_full.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
 
 
 def _host_threads():
@@ -155,12 +167,14 @@ def cpu_time_steps(sc, args, threads, steps):
                          kappa_mode=_abi.FROM_PARAMS)
     o.initialize()
     times, iters = [], []
-    for _ in range(steps + 1):
+    for k in range(steps + 1):
         o.advect(args.dt, 9.8)
         t0 = time.perf_counter()
         st = o.projection_step()
         times.append(time.perf_counter() - t0)
         iters.append(st["cg_iterations"])
+        log("cpu baseline: %d particles, %d threads, step %d: %.2f s, %d CG iterations"
+            % (len(sc["x"]), threads, k, times[-1], iters[-1]))
     o.close()
     return float(np.median(times[1:])), iters[1:]
 
@@ -728,6 +742,11 @@ def main():
         out["cpu_baseline"] = cpu_baseline(args)
         if args.cpu_full:
             out["cpu_baseline_full"] = cpu_baseline(args, dx=args.dx, steps=3)
+        else:
+            full = cpu_full_from_profiles()
+            if full is not None:
+                full["note"] = "measured by bench.py --cpu-full on the GPU box host, not in this run"
+                out["cpu_baseline_full_size"] = full
     print(json.dumps(out), flush=True)
     return 0
 
