@@ -163,6 +163,8 @@ void free_all(vps_context* c) {
         dfree(c->brick_slot[b]);
         dfree(c->slot_brick[b]);
         dfree(c->nbr_slot);
+        dfree(c->col_run);
+        dfree(c->col_order);
         dfree(c->vx[b]);
         dfree(c->vp[b]);
         dfree(c->p_by_rank[b]);
@@ -358,6 +360,8 @@ int vps_create(const vps_params* params, vps_context** out) {
     ALLOC(c->vq, cap);
     ALLOC(c->vrank, cap);
     ALLOC(c->nbr_slot, (size_t)(c->nbricks + 1) * 27);
+    ALLOC(c->col_run, c->nbricks + 1);
+    ALLOC(c->col_order, c->nbricks + 1);
     ALLOC(c->bits, c->nwords + 1);
     ALLOC(c->word_prefix, c->nwords + 1);
     ALLOC(c->partials, partials_size(c->nbricks));

@@ -424,6 +424,10 @@ struct vps_context {
     // slots of the 27 neighbour bricks of every slot of the current layout
     // (the zero brick nbricks where absent), built after each generation
     vps::DBuf<int32_t> nbr_slot;
+    // 3D slots in column order (every z-run of bricks bottom to top, runs in
+    // the order of their first slots) and the run lengths / offsets it is
+    // built from, after each generation
+    vps::DBuf<int32_t> col_run, col_order;
     // per-node fields, slot-major
     vps::DBuf<float> vrho, vvol, vb, vdiag, vx[2], vr, vp[2], vt, vq, vrowsum;
     // 3D CG iteration: second r and t (ping-pong with vr, vt) and the

@@ -278,6 +278,8 @@ constexpr int kPS = kRS * kT;               // tile plane stride
 constexpr int kUnits = kT * kT * 4;         // 16-byte loads per tile: 784
 constexpr int kUnitIters = (kUnits + kS3Threads - 1) / kS3Threads;   // 4
 static_assert(kS3Threads * (kUnitIters - 1) <= kUnits, "full unit rounds");
+constexpr int kChainUnit0 = 6 * kT * 4;         // first unit of tile plane 6: 336
+constexpr int kOwnX = 11 * 64;                  // own nodes of tile planes 3..13
 
 template <int KIND, int OP, typename T, typename C>
 __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
@@ -510,7 +512,8 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
 }
 
 // 3D CG iteration with persistent blocks (G <= kIterBlocks, a multiple of 8
-// when G >= 8, every block owning bricks blockIdx.x, + G, ...).  Each block
+// when G >= 8, every block owning one G-th of the slots, consecutive in
+// column order: it walks up z-runs of bricks).  Each block
 // first forms the global sums of the previous launch's block partials itself
 // -- the same loads, order and arithmetic in every block, so every block
 // reaches the same alpha_(kk-1) = r.r / p.t, the r.r after the update
@@ -519,7 +522,11 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
 // without a reduction launch in between (about 1K partials per set, one
 // round of loads).  Then per brick: x += alpha p_old, r = r_old - alpha t_old,
 // p = r + beta p_old, t = M p with k_stencil3's tile and plane sums, and one partial
-// per block of p.t, r.t, t.t and r.r.  The launch that ends the solve
+// per block of p.t, r.t, t.t and r.r.  A brick that sits on the block's
+// previous brick shares 6 of its 14 halo planes with it: their plane sums
+// and the updated p, r of its z = 0..2 (the previous brick's halo planes
+// 11..13, stored by it) stay in LDS, and only planes 6..13 are loaded and
+// summed.  The launch that ends the solve
 // applies only the last x update; block 0 records the state.
 struct IterArgs {
     const double* prev[4];   // the previous launch's partials (p.t, r.t, t.t, r.r)
@@ -545,12 +552,16 @@ __device__ __forceinline__ void block_sums4(const double v[4], double* scratch, 
 }
 
 template <int OP>
-__global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc, IterArgs it) {
+__global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc, IterArgs it,
+                                                          const int32_t* __restrict__ order) {
     __shared__ float tile[kT * kPS];
     __shared__ __align__(16) float psum[kT][4][64];
-    __shared__ float own[512];
-    __shared__ float own_r[512];
-    __shared__ uint32_t sbase[27];
+    // p and r of the own nodes by tile plane 3..13: planes 3..10 are the
+    // brick's own z = 0..7, planes 11..13 the next brick's z = 0..2 when it
+    // is the dz = +1 neighbour
+    __shared__ __align__(16) float own[kOwnX];
+    __shared__ __align__(16) float own_r[kOwnX];
+    __shared__ uint32_t sbase[2][27];
     __shared__ double sred[4][4];
     __shared__ float s_ab[2];
     __shared__ int s_mode;
@@ -566,44 +577,50 @@ __global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc
     const int x = l & 7, y = l >> 3;
     const int nsl = (int)a.nslots;
     const int G = (int)gridDim.x;
-    auto slot_of = [nsl, G](int v) { return (G & 7) == 0 ? xcd_remap(v, nsl) : v; };
-    int slot = slot_of(blockIdx.x);
-    if (t < 27) sbase[t] = (uint32_t)a.nbr_slot[(int64_t)slot * 27 + t] * 512u;
-    // the neighbour slots of the block's next brick, loaded one brick ahead
-    // so that a brick's halo loads do not wait for them
-    int nb_next = 0;
-    if (t < 27 && (int)blockIdx.x + G < nsl)
-        nb_next = a.nbr_slot[(int64_t)slot_of(blockIdx.x + G) * 27 + t];
-    __syncthreads();
-    // halo loads of a brick (see k_stencil3) and its epilogue operands
-    float4 ra[kUnitIters], rb[kUnitIters], rv[kUnitIters], rc[kUnitIters], xv[kUnitIters];
-    float Vz[2], Dz[2];
-    int64_t base = (int64_t)slot * 512;
-    auto issue = [&]() {
+    // this block's bricks: positions [lo, hi) of the column order, so that a
+    // block walks up z-runs (the blocks of one XCD hold consecutive ranges)
+    const int rbk = (G & 7) == 0 ? xcd_remap(blockIdx.x, G) : (int)blockIdx.x;
+    const int lo = __builtin_amdgcn_readfirstlane((int)((int64_t)rbk * nsl / G));
+    const int hi = __builtin_amdgcn_readfirstlane((int)((int64_t)(rbk + 1) * nsl / G));
+    // (loads without conditions: a loaded value merged at a branch makes
+    // the compiler wait for every load in flight)
+    int cur = order[min(lo, nsl - 1)];
+    const int nx0 = order[min(lo + 1, nsl - 1)];
+    int nxt = lo + 1 < hi ? nx0 : -1;
+    {
+        const int nb0 = a.nbr_slot[(int64_t)cur * 27 + min(t, 26)];
+        if (t < 27) sbase[0][t] = (uint32_t)nb0 * 512u;
+    }
+    // A brick's loads: the epilogue's operands at its two own nodes, and the
+    // halo (see k_stencil3) from unit u0 on -- a chained brick loads tile
+    // planes 6..13 only: planes 0..5 are the previous brick's 8..13, whose
+    // plane sums it copies.
+    float4 ra[kUnitIters], rb[kUnitIters], rv[kUnitIters], rc[kUnitIters];
+    float Vz[2], Dz[2], Pz[2], Xz[2];
+    auto issue = [&](int64_t base, const uint32_t* sbc, int u0) {
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const int64_t g = base + x + 8 * y + 64 * (2 * wv + q);
             Vz[q] = a.vvol[g];
             Dz[q] = a.vdiag[g];
+            Pz[q] = a.in[g];
+            Xz[q] = a.xio[g];
         }
 #pragma unroll
         for (int i = 0; i < kUnitIters; i++) {
-            const int u = t + kS3Threads * i;
-            ra[i] = rb[i] = rv[i] = rc[i] = xv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (i < kUnitIters - 1 || u < kUnits) {
+            const int u = u0 + t + kS3Threads * i;
+            ra[i] = rb[i] = rv[i] = rc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (u < kUnits) {
                 const int k = u & 3, line = u >> 2;
                 const int yt = line % kT, zt = line / kT;
                 const int yy = yt + 5, zz = zt + 5;
                 const int bxo = k == 0 ? 0 : (k == 3 ? 2 : 1);
                 const int nb = (zz >> 3) * 9 + (yy >> 3) * 3 + bxo;
                 const int xl = (k == 0 || k == 2) ? 4 : 0;
-                const uint32_t g = sbase[nb] + (uint32_t)(xl + 8 * ((yy & 7) + 8 * (zz & 7)));
+                const uint32_t g = sbc[nb] + (uint32_t)(xl + 8 * ((yy & 7) + 8 * (zz & 7)));
                 ra[i] = *(const float4*)(a.in + g);
                 rb[i] = *(const float4*)(a.in2 + g);
                 rc[i] = *(const float4*)(a.in3 + g);
-                if ((k == 1 || k == 2) && yt >= 3 && yt < 11 && zt >= 3 && zt < 11)
-                    xv[i] = *(const float4*)(a.xio + base + 4 * (k - 1) +
-                                             8 * ((yt - 3) + 8 * (zt - 3)));
                 if constexpr (OP == VPS_OP_ROW) rv[i] = *(const float4*)(a.vvol + g);
             }
         }
@@ -681,29 +698,35 @@ __global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc
     if (mode == 1) {
         // the solve ended: the last x update only (own nodes of every brick)
         for (int v = blockIdx.x; v < nsl; v += G) {
-            const int64_t b0 = (int64_t)slot_of(v) * 512;
+            const int64_t b0 = (int64_t)((G & 7) == 0 ? xcd_remap(v, nsl) : v) * 512;
             for (int e = t; e < 512; e += kS3Threads)
                 a.xio[b0 + e] += al * a.in[b0 + e];
         }
         return;
     }
     double red0 = 0, red1 = 0, red2 = 0, red3 = 0;
-    for (int v = blockIdx.x; v < nsl; v += G) {
-        if (v != (int)blockIdx.x) {   // uniform over the block
-            slot = slot_of(v);
-            base = (int64_t)slot * 512;
-            __syncthreads();   // the previous brick's LDS has been read
-            if (t < 27) {
-                sbase[t] = (uint32_t)nb_next * 512u;
-                if (v + G < nsl) nb_next = a.nbr_slot[(int64_t)slot_of(v + G) * 27 + t];
-            }
-            __syncthreads();
+    int sb = 0;             // which sbase row is the current brick's
+    bool chained = false;   // the current brick sits on the previous one
+    for (int p = lo; p < hi; p++) {
+        const int64_t base = (int64_t)cur * 512;
+        if (p != lo) __syncthreads();   // the previous brick's psum and own have been read
+        const uint32_t* sbc = sbase[sb];
+        const int u0 = chained ? kChainUnit0 : 0;
+        issue(base, sbc, u0);
+        // the next brick is the dz = +1 neighbour: it is chained to this one
+        const uint32_t up = __builtin_amdgcn_readfirstlane(sbc[22]);
+        const bool has_next = nxt >= 0 && up == (uint32_t)nxt * 512u;
+        // the brick after next (its slot) and the next brick's neighbour slots
+        const int nn2 = order[min(p + 2, nsl - 1)];
+        const int nbv = a.nbr_slot[(int64_t)max(nxt, 0) * 27 + min(t, 26)];
+        if (chained) {
+            float4* ps4 = reinterpret_cast<float4*>(&psum[0][0][0]);
+            for (int e = t; e < 6 * 64; e += kS3Threads) ps4[e] = ps4[e + 8 * 64];
         }
-        issue();
 #pragma unroll
         for (int i = 0; i < kUnitIters; i++) {
-            const int u = t + kS3Threads * i;
-            if (i < kUnitIters - 1 || u < kUnits) {
+            const int u = u0 + t + kS3Threads * i;
+            if (u < kUnits) {
                 const int k = u & 3, line = u >> 2;
                 const int yt = line % kT, zt = line / kT;
                 float rn[4], pn[4];
@@ -725,29 +748,37 @@ __global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc
 #pragma unroll
                     for (int c = 0; c < 4; c++) dst[c] = pn[c];
                 }
-                if ((k == 1 || k == 2) && yt >= 3 && yt < 11 && zt >= 3 && zt < 11) {
-                    const int loc = 4 * (k - 1) + 8 * ((yt - 3) + 8 * (zt - 3));
+                if ((k == 1 || k == 2) && yt >= 3 && yt < 11 && zt >= 3) {
+                    // p and r of the own nodes, and of the next brick's
+                    // z = 0..2 in planes 11..13
+                    const int loc = 4 * (k - 1) + 8 * (yt - 3) + 64 * (zt - 3);
+                    if (chained && zt >= 11) {
+                        // the previous brick left this brick's z = 0..2 here;
+                        // the same thread moves them before overwriting them
+#pragma unroll
+                        for (int c = 0; c < 4; c++) {
+                            own[loc - 512 + c] = own[loc + c];
+                            own_r[loc - 512 + c] = own_r[loc + c];
+                        }
+                    }
 #pragma unroll
                     for (int c = 0; c < 4; c++) {
                         own[loc + c] = pn[c];
                         own_r[loc + c] = rn[c];
                     }
-                    store_nt4(a.xio + base + loc, xv[i].x + al * ra[i].x,
-                              xv[i].y + al * ra[i].y, xv[i].z + al * ra[i].z,
-                              xv[i].w + al * ra[i].w);
-                    store_nt4(a.out2 + base + loc, pn[0], pn[1], pn[2], pn[3]);
-                    store_nt4(a.out3 + base + loc, rn[0], rn[1], rn[2], rn[3]);
                 }
             }
         }
+        if (t < 27 && nxt >= 0) sbase[sb ^ 1][t] = (uint32_t)nbv * 512u;
         __syncthreads();
-        for (int zp = wv; zp < kT; zp += 4) {
+        for (int zp = (chained ? 6 : 0) + wv; zp < kT; zp += 4) {
             float P[4];
             plane4<kRS, float>(&tile[zp * kPS + (y + 3) * kRS + (x + 8)], cc, P);
 #pragma unroll
             for (int q = 0; q < 4; q++) psum[zp][q][l] = P[q];
         }
         __syncthreads();
+        // t = M p, and x, p, r of the own nodes
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const int z = 2 * wv + q;
@@ -761,16 +792,24 @@ __global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc
             acc += psum[z + 6][3][l];
             const int loc = x + 8 * y + 64 * z;
             const float V = Vz[q], Dg = Dz[q];
-            const float pi = own[loc];
+            const float pi = own[loc], ri = own_r[loc];
             const float stv = OP == VPS_OP_ROW ? acc : V * acc;
             const float mp = V > 0.f ? Dg * pi + stv : 0.f;
             __builtin_nontemporal_store(mp, &a.out[base + loc]);
-            const double rvv = (double)own_r[loc];
+            __builtin_nontemporal_store(pi, &a.out2[base + loc]);
+            __builtin_nontemporal_store(ri, &a.out3[base + loc]);
+            __builtin_nontemporal_store(Xz[q] + al * Pz[q], &a.xio[base + loc]);
+            const double rvv = (double)ri;
             red0 += (double)pi * (double)mp;
             red1 += rvv * (double)mp;
             red2 += (double)mp * (double)mp;
             red3 += rvv * rvv;
         }
+        const int nn = p + 2 < hi ? nn2 : -1;
+        cur = nxt;
+        nxt = nn;
+        sb ^= 1;
+        chained = has_next;
     }
     // this block's partials, one DPP tree per wave (see block_sums4)
     double rsum[4] = {red0, red1, red2, red3};
@@ -1483,6 +1522,7 @@ int cg_solve(vps_context* c, vps_stats* stats) {
                 // partials: FIRST writes set A (one per slot); launch kk
                 // writes set B when kk is odd, A when even, and reads the other
                 IterArgs ia;
+                const int32_t* corder = c->col_order.p;
                 for (int q = 0; q < 4; q++) {
                     ia.out[q] = nw ? setB[q] : setA[q];
                     ia.prev[q] = nw ? setA[q] : setB[q];
@@ -1494,16 +1534,17 @@ int cg_solve(vps_context* c, vps_stats* stats) {
                 if (c->p.op_mode == VPS_OP_ROW) {
                     if (ev)
                         hipExtLaunchKernelGGL(k_cg_iter3<VPS_OP_ROW>, gi, bi, 0, c->stream, ev[0],
-                                              ev[1], 0, s, cf, ia);
+                                              ev[1], 0, s, cf, ia, corder);
                     else
-                        hipLaunchKernelGGL(k_cg_iter3<VPS_OP_ROW>, gi, bi, 0, c->stream, s, cf, ia);
+                        hipLaunchKernelGGL(k_cg_iter3<VPS_OP_ROW>, gi, bi, 0, c->stream, s, cf, ia,
+                                           corder);
                 } else {
                     if (ev)
                         hipExtLaunchKernelGGL(k_cg_iter3<VPS_OP_TRANSPOSE>, gi, bi, 0, c->stream,
-                                              ev[0], ev[1], 0, s, cf, ia);
+                                              ev[0], ev[1], 0, s, cf, ia, corder);
                     else
                         hipLaunchKernelGGL(k_cg_iter3<VPS_OP_TRANSPOSE>, gi, bi, 0, c->stream, s,
-                                           cf, ia);
+                                           cf, ia, corder);
                 }
                 if (do_time) {
                     // an iteration of index kk - 1 (its reduction and update)

@@ -633,6 +633,37 @@ __global__ void __launch_bounds__(256) k_nbr_slots(KConst k,
     nbr[slot * 27 + t] = s;
 }
 
+// Column order of the 3D slots, for the CG iteration's walk up z-runs of
+// bricks.  A run starts at a slot whose dz = -1 neighbour (index 4 of the
+// 27) is the zero brick and continues through the dz = +1 neighbours (index
+// 22); run[s] is the run's length at its first slot and 0 elsewhere.
+__global__ void __launch_bounds__(256) k_col_runs(const int32_t* nbr, int64_t nslots,
+                                                  int32_t zero, int32_t* run) {
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= nslots) return;
+    int32_t len = 0;
+    if (nbr[s * 27 + 4] == zero) {
+        int32_t b = (int32_t)s;
+        len = 1;
+        while ((b = nbr[(int64_t)b * 27 + 22]) != zero) len++;
+    }
+    run[s] = len;
+}
+
+// order[off[first slot of the run] + height in the run] = slot
+__global__ void __launch_bounds__(256) k_col_place(const int32_t* nbr, int64_t nslots,
+                                                   int32_t zero, const int32_t* off,
+                                                   int32_t* order) {
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= nslots) return;
+    int32_t b = (int32_t)s, d = 0, n;
+    while ((n = nbr[(int64_t)b * 27 + 4]) != zero) {
+        b = n;
+        d++;
+    }
+    order[off[b] + d] = (int32_t)s;
+}
+
 __global__ void __launch_bounds__(256) k_popc(const uint32_t* bits,
                                               int64_t nwords, int32_t* cnt) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -809,6 +840,15 @@ int virtual_generate(vps_context* c) {
     }
     c->nslots = nslots;
     c->which = w;
+    if (D == 3 && nslots > 0) {
+        const unsigned nb = blocks(nslots, 256);
+        hipLaunchKernelGGL(k_col_runs, dim3(nb), dim3(256), 0, c->stream, c->nbr_slot.p,
+                           nslots, (int32_t)c->nbricks, c->col_run.p);
+        VPS_TRY(scan_exclusive_i32(c, c->col_run.p, c->col_run.p, nslots, nullptr));
+        hipLaunchKernelGGL(k_col_place, dim3(nb), dim3(256), 0, c->stream, c->nbr_slot.p,
+                           nslots, (int32_t)c->nbricks, c->col_run.p, c->col_order.p);
+        VPS_HIP(c, hipGetLastError());
+    }
     c->nvirt = nslots ? *(int32_t*)(c->host_counters + 1) : 0;
     c->grad_nchunks = *(int32_t*)(c->host_counters + 2);
     return VPS_OK;
