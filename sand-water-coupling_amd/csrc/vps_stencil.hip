@@ -528,6 +528,7 @@ __global__ void __launch_bounds__(kS3Threads) k_stencil3(StencilArgs a, C cc) {
 // 11..13, stored by it) stay in LDS, and only planes 6..13 are loaded and
 // summed.  The launch that ends the solve
 // applies only the last x update; block 0 records the state.
+static_assert(kIterBlocks <= 4 * kS3Threads, "one round of partial loads per thread");
 struct IterArgs {
     const double* prev[4];   // the previous launch's partials (p.t, r.t, t.t, r.r)
     double* out[4];          // this launch's partials, one per block
@@ -567,12 +568,23 @@ __global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc
     __shared__ int s_mode;
     CgState* st = a.st;
     const int kk = a.kk;
-    {
-        // a solve ended by an earlier launch (or before any): nothing to do
-        const int stop = __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (stop != 0 && stop != kk + 1) return;   // uniform over the grid
-    }
     const int t = threadIdx.x;
+    // Everything the prologue reads goes out at once: the stop tag and the
+    // state (written by earlier launches only, so plain loads), the previous
+    // launch's partials (at most 4 x 256 per sum) and this block's first
+    // slots.  Every lane loads (indices past the partials read the last
+    // one): loads under a condition make the compiler's waits for them
+    // cover every later load.
+    const int stop = st->stop;
+    const double thr_rec = st->thr_rec;
+    const float st_alpha = st->alpha, st_beta = st->beta;
+    double v4[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int i = min(t + kS3Threads * u, it.nprev - 1);
+#pragma unroll
+        for (int q = 0; q < 4; q++) v4[q][u] = it.prev[q][i];
+    }
     const int l = t & 63, wv = t >> 6;
     const int x = l & 7, y = l >> 3;
     const int nsl = (int)a.nslots;
@@ -582,11 +594,11 @@ __global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc
     const int rbk = (G & 7) == 0 ? xcd_remap(blockIdx.x, G) : (int)blockIdx.x;
     const int lo = __builtin_amdgcn_readfirstlane((int)((int64_t)rbk * nsl / G));
     const int hi = __builtin_amdgcn_readfirstlane((int)((int64_t)(rbk + 1) * nsl / G));
-    // (loads without conditions: a loaded value merged at a branch makes
-    // the compiler wait for every load in flight)
     int cur = order[min(lo, nsl - 1)];
     const int nx0 = order[min(lo + 1, nsl - 1)];
     int nxt = lo + 1 < hi ? nx0 : -1;
+    // a solve ended by an earlier launch (or before any): nothing to do
+    if (stop != 0 && stop != kk + 1) return;   // uniform over the grid
     {
         const int nb0 = a.nbr_slot[(int64_t)cur * 27 + min(t, 26)];
         if (t < 27) sbase[0][t] = (uint32_t)nb0 * 512u;
@@ -629,21 +641,17 @@ __global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc
     if (it.resume) {
         if (t == 0) {
             s_mode = 0;
-            s_ab[0] = st->alpha;
-            s_ab[1] = st->beta;
+            s_ab[0] = st_alpha;
+            s_ab[1] = st_beta;
         }
     } else {
-        double sv[4] = {0, 0, 0, 0};
-        for (int i0 = 0; i0 < it.nprev; i0 += 4 * kS3Threads) {
-            double v4[4][4];
+        double sv[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int i = i0 + t + kS3Threads * u;
+        for (int q = 0; q < 4; q++) {
+            double w[4];
 #pragma unroll
-                for (int q = 0; q < 4; q++) v4[q][u] = i < it.nprev ? it.prev[q][i] : 0.0;
-            }
-#pragma unroll
-            for (int q = 0; q < 4; q++) sv[q] += (v4[q][0] + v4[q][1]) + (v4[q][2] + v4[q][3]);
+            for (int u = 0; u < 4; u++) w[u] = t + kS3Threads * u < it.nprev ? v4[q][u] : 0.0;
+            sv[q] = (w[0] + w[1]) + (w[2] + w[3]);
         }
         // the four sums through LDS (psum is free here): wave w reduces sum
         // w with one DPP tree instead of every wave reducing all four
@@ -663,7 +671,7 @@ __global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc
             if (!isfinite(ald) || !isfinite(rn)) {
                 done = 3;
                 iter = k;
-            } else if (rn < st->thr_rec) {
+            } else if (rn < thr_rec) {
                 done = 1;
                 iter = k;
             } else if (k + 1 >= a.max_iter) {
@@ -1112,6 +1120,26 @@ __global__ void __launch_bounds__(1024) k_sum_partials(const double* p, int n,
     if (threadIdx.x == 0) *out = a;
 }
 
+// The first SpMV leaves one partial per brick; the first iteration launch
+// reads them folded to G: partial g of sum q is the sum, in index order, of
+// bricks [g n / G, (g + 1) n / G), so that no block of that launch reads all
+// n (every block of an iteration launch sums every partial it is given).
+__global__ void __launch_bounds__(256) k_fold_partials(const double* p0, const double* p1,
+                                                       const double* p2, const double* p3,
+                                                       int n, int G, double* out0,
+                                                       double* out1, double* out2,
+                                                       double* out3) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= 4 * G) return;
+    const int q = i / G, g = i - q * G;
+    const double* p = q == 0 ? p0 : (q == 1 ? p1 : (q == 2 ? p2 : p3));
+    double* out = q == 0 ? out0 : (q == 1 ? out1 : (q == 2 ? out2 : out3));
+    const int a = (int)((int64_t)g * n / G), b = (int)((int64_t)(g + 1) * n / G);
+    double s = 0.0;
+    for (int j = a; j < b; j++) s += p[j];
+    out[g] = s;
+}
+
 // Residual replacement: r = rt; p = rt (restart direction); partial r.r.
 __global__ void __launch_bounds__(kUpdThreads) k_replace(int64_t n4,
                                                          const float4* rt,
@@ -1250,9 +1278,14 @@ static double* part_misc(vps_context* c) {
 static double* part_iter(vps_context* c, int q) {
     return c->partials.p + 6 * c->nbricks + 4 * kUpdBlocks + 2048 + (int64_t)q * kIterBlocks;
 }
+// the first SpMV's per-brick partials folded to one per iteration block
+static double* part_fold(vps_context* c, int q) {
+    return c->partials.p + 6 * c->nbricks + 4 * kUpdBlocks + 2048 +
+           (int64_t)(4 + q) * kIterBlocks;
+}
 
 int64_t partials_size(int64_t nbricks) {
-    return 6 * nbricks + 4 * kUpdBlocks + 2048 + 4 * kIterBlocks;
+    return 6 * nbricks + 4 * kUpdBlocks + 2048 + 8 * kIterBlocks;
 }
 
 static StencilArgs base_args(vps_context* c) {
@@ -1504,6 +1537,10 @@ int cg_solve(vps_context* c, vps_stats* stats) {
                     s.in = p0;
                     s.out = t;
                     launch_op<K_SPMV_FIRST, float, CoefF>(c, s, cf);
+                    hipLaunchKernelGGL(k_fold_partials, dim3((unsigned)((4 * giter + 255) / 256)),
+                                       dim3(256), 0, c->stream, setA[0], setA[1], setA[2],
+                                       setA[3], nslots, giter, part_fold(c, 0), part_fold(c, 1),
+                                       part_fold(c, 2), part_fold(c, 3));
                     spmv_launches++;
                     kk++;
                     continue;
@@ -1519,15 +1556,16 @@ int cg_solve(vps_context* c, vps_stats* stats) {
                 s.out2 = pb[nw];
                 s.out3 = rbuf[nw];
                 s.xio = x;
-                // partials: FIRST writes set A (one per slot); launch kk
-                // writes set B when kk is odd, A when even, and reads the other
+                // partials: FIRST writes set A (one per slot), folded to one
+                // per block for launch 1; launch kk writes set B when kk is
+                // odd, A when even, and reads the other
                 IterArgs ia;
                 const int32_t* corder = c->col_order.p;
                 for (int q = 0; q < 4; q++) {
                     ia.out[q] = nw ? setB[q] : setA[q];
-                    ia.prev[q] = nw ? setA[q] : setB[q];
+                    ia.prev[q] = kk == 1 ? part_fold(c, q) : (nw ? setA[q] : setB[q]);
                 }
-                ia.nprev = kk == 1 ? nslots : giter;
+                ia.nprev = giter;
                 ia.resume = resume_pending ? 1 : 0;
                 resume_pending = false;
                 const dim3 gi((unsigned)giter), bi(kS3Threads);
