@@ -629,11 +629,15 @@ __global__ void __launch_bounds__(kS3Threads) k_cg_iter3(StencilArgs a, CoefF cc
                 const int bxo = k == 0 ? 0 : (k == 3 ? 2 : 1);
                 const int nb = (zz >> 3) * 9 + (yy >> 3) * 3 + bxo;
                 const int xl = (k == 0 || k == 2) ? 4 : 0;
-                const uint32_t g = sbc[nb] + (uint32_t)(xl + 8 * ((yy & 7) + 8 * (zz & 7)));
-                ra[i] = *(const float4*)(a.in + g);
-                rb[i] = *(const float4*)(a.in2 + g);
-                rc[i] = *(const float4*)(a.in3 + g);
-                if constexpr (OP == VPS_OP_ROW) rv[i] = *(const float4*)(a.vvol + g);
+                // byte offset in 32 bits (the node arrays stay below 4 GB):
+                // the loads take the array base from scalar registers
+                const uint32_t g =
+                    4u * (sbc[nb] + (uint32_t)(xl + 8 * ((yy & 7) + 8 * (zz & 7))));
+                ra[i] = *(const float4*)((const char*)a.in + g);
+                rb[i] = *(const float4*)((const char*)a.in2 + g);
+                rc[i] = *(const float4*)((const char*)a.in3 + g);
+                if constexpr (OP == VPS_OP_ROW)
+                    rv[i] = *(const float4*)((const char*)a.vvol + g);
             }
         }
     };
