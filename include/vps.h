@@ -171,7 +171,9 @@ int64_t vps_sizeof_stats(void);
 
 /* Creates a context on the current HIP device.  Replaces the constructors
  * DualParticleSphLiquid(grid, particleRadius) and
- * VirtualParticle(grid, grid.spacing()) (DualParticleSphLiquid.h:27-29). */
+ * VirtualParticle(grid, grid.spacing()) (DualParticleSphLiquid.h:27-29).
+ * VPS_ERR_INVALID_ARG for a domain of more than 2^30 cells, a lattice of
+ * 2^31 nodes or more, or (3D) more than 2^21 - 1 bricks of 8^3 nodes. */
 int vps_create(const vps_params* params, vps_context** out);
 void vps_destroy(vps_context* ctx);
 

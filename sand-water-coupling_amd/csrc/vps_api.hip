@@ -329,6 +329,12 @@ int vps_create(const vps_params* params, vps_context** out) {
         delete c;
         return fail(nullptr, VPS_ERR_INVALID_ARG, "domain or lattice too large");
     }
+    // the 3D CG iteration addresses the per-node arrays (bricks and the zero
+    // brick, 2 KB each) with 32-bit byte offsets
+    if (p.dim == 3 && (c->nbricks + 1) * 2048 > (int64_t)UINT32_MAX) {
+        delete c;
+        return fail(nullptr, VPS_ERR_INVALID_ARG, "lattice too large (more than 2^21 - 1 bricks)");
+    }
     // every per-node array has one extra all-zero brick at index nbricks,
     // read by the stencil in place of absent neighbour bricks
     const size_t cap = (size_t)(c->nbricks + 1) * c->brick_nodes;
