@@ -119,7 +119,11 @@ void fill_kconst(vps_context* c) {
         k.bdims[a] = a < p.dim ? (k.ndims[a] + kBrick - 1) / kBrick : 1;
     }
     k.spacing = p.lattice_spacing;
-    const double cs = k.h * (1 + 1e-6);
+    // cells a hair above h: every neighbour within h stays within one cell
+    // of the FP64-rounded cell of a point, while a lattice node on a cell
+    // face (nodes every h / 2 on a grid aligned with the cells) stays within
+    // 1e-9 h of it over any domain, so the sums can skip the cell behind it
+    const double cs = k.h * (1 + 1e-12);
     k.cell_inv = 1 / cs;
     k.cell_size = cs;
     for (int a = 0; a < 3; a++) {

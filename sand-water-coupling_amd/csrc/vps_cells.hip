@@ -5,7 +5,7 @@
 // (SmoothedParticles.cpp:25-36).  The reference hashes cells into 2N
 // buckets with OR-combined primes and walks 27 sorted bucket keys per query;
 // here particles are counting-sorted by a dense x-fastest cell id (cell size
-// h (1 + 1e-6)), so the 3 cells of an x-row are one contiguous range and a
+// h (1 + 1e-12)), so the 3 cells of an x-row are one contiguous range and a
 // query reads 9 ranges (3 in 2D).  Membership is the reference's exact FP64
 // test (x_j - q).squaredNorm() < h^2 (.cpp:32), so the neighbour sets are
 // identical; only the visiting order differs.  Within a cell particles are

@@ -8,7 +8,7 @@
 //                    rid   int32                      caller index
 //   boundary         same layout, sorted once (bpos, bvel = (v, V),
 //                    bnrm = (n, rho))
-//   cell grid        uniform, cell size h (1 + 1e-6), x-fastest linear id,
+//   cell grid        uniform, cell size h (1 + 1e-12), x-fastest linear id,
 //                    cell_start[C + 1]; the 3 cells of an x-row are
 //                    contiguous in the sorted arrays, so a query walks
 //                    3^(D-1) contiguous ranges

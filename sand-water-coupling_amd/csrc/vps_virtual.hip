@@ -398,6 +398,7 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
     const int slot = xcd_remap(blockIdx.x, nslots);
     const int brick = brick_of_slot[slot];
     const int t = threadIdx.x;
+    // a wave: the 8 x 8 nodes of one z-plane of the brick
     const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
     const int bx = brick % k.bdims[0];
     const int by = (brick / k.bdims[0]) % k.bdims[1];
@@ -535,13 +536,18 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
     // the exact test decides, so a fast member's accurate d2f is > 0.
     const uint32_t lo_min = __float_as_uint(k.h2f_hi - k.h2f_lo);
     const uint32_t lo_span = __float_as_uint(k.h2f_lo) - lo_min - 1u;
-    // Cells whose nearest point lies beyond h are skipped: a whole row by
-    // its y and z distance, the end cells of a row by their x distance.
-    // Only interior cells are culled (a border cell may hold particles
-    // clamped into it from outside its extent), with a margin of 1e-4 h^2
-    // far above the FP32 error of these distances.
+    // Cells whose nearest point lies beyond h (1 - 2e-6) are skipped: a
+    // whole row by its y and z distance, the end cells of a row by their x
+    // distance.  Only interior cells are culled (a border cell may hold
+    // particles clamped into it from outside its extent).  The margin is
+    // twenty times the FP32 error of these distances; a member it drops lies
+    // within 2.1e-6 h of the cutoff, where W and W' are below 1e-17 and
+    // 5e-12 of their peaks, far below FP32 resolution of the sums.  On a
+    // lattice aligned with the cell grid (a node every half cell) it skips
+    // the cells behind a node on a cell face; the nodes of a wave share
+    // their z, so the rows behind them in z are skipped by the whole wave.
     const float csf = (float)k.cell_size;
-    const float h2c = (float)k.h2 * 1.0001f;
+    const float h2c = (float)k.h2 * (1.f - 4e-6f);
     auto gap = [csf](float t) { return fmaxf(fmaxf(-t, t - csf), 0.f); };   // q - C -> distance
     const float txh = xhi - xlo == 2 ? tx2 : (xhi - xlo == 1 ? tx1 : tx0);
     const bool cut_l = xlo < xhi && xlo > 0, cut_r = xhi > xlo && xhi < k.cdims[0] - 1;
