@@ -1510,7 +1510,10 @@ int cg_solve(vps_context* c, vps_stats* stats) {
     int kk = 0;          // iteration since the (re)start
     int iter_base = 0;   // Eigen iterations before the (re)start
     int replacements = 0;
-    int predicted = c->prev_iters > 0 ? c->prev_iters + 1 : 8;
+    // the first batch: the last solve's count and three more (a launch past
+    // convergence returns at its prologue in a few us; a batch that stops
+    // short costs a host round trip of about 30 us)
+    int predicted = c->prev_iters > 0 ? c->prev_iters + 3 : 8;
     // 3D: ping-pong vectors of the one-launch iteration; resume_pending: the
     // next launch is an update after a replacement (no reduction before it)
     const bool three = c->dim == 3;

@@ -643,10 +643,18 @@ __global__ void __launch_bounds__(256) k_nbr_slots(KConst k,
 // bricks.  A run starts at a slot whose dz = -1 neighbour (index 4 of the
 // 27) is the zero brick and continues through the dz = +1 neighbours (index
 // 22); run[s] is the run's length at its first slot and 0 elsewhere.
-__global__ void __launch_bounds__(256) k_col_runs(const int32_t* nbr, int64_t nslots,
-                                                  int32_t zero, int32_t* run) {
+__global__ void __launch_bounds__(256) k_col_runs(const int32_t* nbr, const int32_t* nslots_p,
+                                                  int64_t grid, int32_t zero, int32_t* run) {
     const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (s >= nslots) return;
+    if (s >= grid) return;
+    // the grid may exceed the slot count; a count beyond the grid (the
+    // neighbour table is then incomplete) leaves the order to the rerun on
+    // the exact grid
+    const int32_t nsl = *nslots_p;
+    if (s >= nsl || nsl > grid) {
+        run[s] = 0;
+        return;
+    }
     int32_t len = 0;
     if (nbr[s * 27 + 4] == zero) {
         int32_t b = (int32_t)s;
@@ -657,11 +665,12 @@ __global__ void __launch_bounds__(256) k_col_runs(const int32_t* nbr, int64_t ns
 }
 
 // order[off[first slot of the run] + height in the run] = slot
-__global__ void __launch_bounds__(256) k_col_place(const int32_t* nbr, int64_t nslots,
-                                                   int32_t zero, const int32_t* off,
-                                                   int32_t* order) {
+__global__ void __launch_bounds__(256) k_col_place(const int32_t* nbr, const int32_t* nslots_p,
+                                                   int64_t grid, int32_t zero,
+                                                   const int32_t* off, int32_t* order) {
     const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (s >= nslots) return;
+    const int32_t nsl = *nslots_p;
+    if (s >= nsl || nsl > grid) return;
     int32_t b = (int32_t)s, d = 0, n;
     while ((n = nbr[(int64_t)b * 27 + 4]) != zero) {
         b = n;
@@ -765,11 +774,19 @@ static int generate_slots(vps_context* c, int w, int64_t grid) {
                            c->cell_start.p, c->bpos.p, c->bvel.p, c->bnrm.p,
                            c->bcell_start.p, c->vrho.p, c->vvol.p, c->vb.p,
                            c->bits.p);
-    if (D == 3)
+    if (D == 3) {
         hipLaunchKernelGGL(k_nbr_slots<3>, dim3(blocks(grid * 27, 256)), dim3(256),
                            0, c->stream, c->k, c->slot_brick[w].p, c->brick_slot[w].p,
                            nsl, (int32_t)c->nbricks, c->nbr_slot.p);
-    else
+        // the column order, on the grid (before the read-back below, so
+        // that it runs while the host waits)
+        hipLaunchKernelGGL(k_col_runs, dim3(blocks(grid, 256)), dim3(256), 0, c->stream,
+                           c->nbr_slot.p, nsl, grid, (int32_t)c->nbricks, c->col_run.p);
+        VPS_TRY(scan_exclusive_i32(c, c->col_run.p, c->col_run.p, grid, nullptr));
+        hipLaunchKernelGGL(k_col_place, dim3(blocks(grid, 256)), dim3(256), 0, c->stream,
+                           c->nbr_slot.p, nsl, grid, (int32_t)c->nbricks, c->col_run.p,
+                           c->col_order.p);
+    } else
         hipLaunchKernelGGL(k_nbr_slots<2>, dim3(blocks(grid * 9, 256)), dim3(256),
                            0, c->stream, c->k, c->slot_brick[w].p, c->brick_slot[w].p,
                            nsl, (int32_t)c->nbricks, c->nbr_slot.p);
@@ -846,15 +863,6 @@ int virtual_generate(vps_context* c) {
     }
     c->nslots = nslots;
     c->which = w;
-    if (D == 3 && nslots > 0) {
-        const unsigned nb = blocks(nslots, 256);
-        hipLaunchKernelGGL(k_col_runs, dim3(nb), dim3(256), 0, c->stream, c->nbr_slot.p,
-                           nslots, (int32_t)c->nbricks, c->col_run.p);
-        VPS_TRY(scan_exclusive_i32(c, c->col_run.p, c->col_run.p, nslots, nullptr));
-        hipLaunchKernelGGL(k_col_place, dim3(nb), dim3(256), 0, c->stream, c->nbr_slot.p,
-                           nslots, (int32_t)c->nbricks, c->col_run.p, c->col_order.p);
-        VPS_HIP(c, hipGetLastError());
-    }
     c->nvirt = nslots ? *(int32_t*)(c->host_counters + 1) : 0;
     c->grad_nchunks = *(int32_t*)(c->host_counters + 2);
     return VPS_OK;
