@@ -192,7 +192,10 @@ __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const flo
                 }
                 // v_sqrt_f32 (1 ulp): W is rounded to FP32 anyway
                 const float x = __builtin_amdgcn_sqrtf(d2) * k.inv_h;
-                const float a = fmaxf(1.f - x, 0.f), c = fmaxf(fmaf(-2.f, x, 1.f), 0.f);
+                // max(., 0) of values <= 1 as the clamp to [0, 1], which
+                // folds into the producing add and fma
+                const float a = __builtin_amdgcn_fmed3f(1.f - x, 0.f, 1.f);
+                const float c = __builtin_amdgcn_fmed3f(fmaf(-2.f, x, 1.f), 0.f, 1.f);
                 sum += fmaf(2.f * a, a * a, -(c * c * c));
             }
         }

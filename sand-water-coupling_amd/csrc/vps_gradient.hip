@@ -322,13 +322,13 @@ __global__ void __launch_bounds__(kGradThreads, 4) k_gradient3s(
         }
     }
     __syncthreads();
-    // spline of H as a function of d2 (S = -W'(r) / r):
-    //   r < H/2:     S = c1 / H (2 - 3 r / H)
-    //   H/2 <= r < H: S = c1 (1 - r/H)^2 / r = rinv (c1 + c1 d2 / H^2) - 2 c1 / H
-    const float c1 = k.c1H, iH = k.inv_H;
-    const float kIn0 = 2.f * c1 * iH, kIn1 = 3.f * c1 * iH * iH;
-    const float kOut1 = c1 * iH * iH, kOut0 = -2.f * c1 * iH;
-    const float H2 = (float)k.H2, H2q = 0.25f * (float)k.H2;
+    // spline of H (S = -W'(r) / r, q = r / H): with a = max(1 - q, 0) and
+    // b = max(1 - 2 q, 0), S = c1 (a^2 - b^2) / r on every piece:
+    // c1 (2 - 3 q) / H for q < 1/2, c1 (1 - q)^2 / r up to q = 1, and 0
+    // beyond (a = b = 0).  1 - q <= 1, so the clamp to [0, 1] is the max;
+    // it folds into the fma.  c1 is applied once per particle.
+    const float c1 = k.c1H, iH = k.inv_H, iH2 = 2.f * k.inv_H;
+    const float H2 = (float)k.H2;
     for (int p0 = ch.x; p0 < ch.y; p0 += kGradThreads) {
         const int p = p0 + t;
         if (p >= ch.y) break;
@@ -380,10 +380,10 @@ __global__ void __launch_bounds__(kGradThreads, 4) k_gradient3s(
                         if (c < cut || c > 7 - cut) continue;   // uniform
                         const float d2 = dx2[c] + dyz;
                         const float rinv = __builtin_amdgcn_rsqf(d2);
-                        const float so = fmaf(fmaf(d2, kOut1, c1), rinv, kOut0);
-                        const float si = fmaf(-kIn1, d2 * rinv, kIn0);
-                        const float S = d2 < H2q ? si : (d2 < H2 ? so : 0.f);
-                        const float sc = row[c] * S;
+                        const float r = d2 * rinv;
+                        const float a = __builtin_amdgcn_fmed3f(fmaf(-iH, r, 1.f), 0.f, 1.f);
+                        const float b = __builtin_amdgcn_fmed3f(fmaf(-iH2, r, 1.f), 0.f, 1.f);
+                        const float sc = row[c] * rinv * fmaf(a, a, -(b * b));
                         sx = fmaf(sc, dxf[c], sx);
                         ss += sc;
                     }
@@ -392,6 +392,9 @@ __global__ void __launch_bounds__(kGradThreads, 4) k_gradient3s(
                     fz -= ss * dzfc;
                 }
             }
+            fx *= c1;
+            fy *= c1;
+            fz *= c1;
         }
         if (s_walls) boundary_term<3>(k, q, vi, bpos, bvel, bnrm, bcstart, fx, fy, fz);
         float4 v = vi;

@@ -166,6 +166,27 @@ __device__ __forceinline__ float spline_s(float x, float rinv, float c1,
     return x < 0.5f ? in : out;
 }
 
+// The same spline with no select: from a = max(1 - x, 0) and
+// b = max(1 - 2x, 0), W = c0 (2 a^3 - b^3) and S = c1 (a^2 - b^2) / r on
+// every piece (below x = 1/2, 2 a^3 - b^3 = 6 x^2 (x - 1) + 1 and
+// a^2 - b^2 = x (2 - 3x); above, b = 0; from x = 1 on, a = b = 0).  Both
+// values are <= 1, so each max is the clamp to [0, 1], which folds into the
+// fma that forms it.  The differences lose at most one ulp of 1 to
+// cancellation, below the rounding of the sums they enter.
+struct SplineAB {
+    float a, b;
+};
+__device__ __forceinline__ SplineAB spline_ab(float r, float inv_kr) {
+    return {__builtin_amdgcn_fmed3f(fmaf(-inv_kr, r, 1.f), 0.f, 1.f),
+            __builtin_amdgcn_fmed3f(fmaf(-2.f * inv_kr, r, 1.f), 0.f, 1.f)};
+}
+__device__ __forceinline__ float spline_w_ab(SplineAB s, float c0) {
+    return c0 * fmaf(s.a + s.a, s.a * s.a, -(s.b * s.b * s.b));
+}
+__device__ __forceinline__ float spline_s_ab(SplineAB s, float rinv, float c1) {
+    return c1 * rinv * fmaf(s.a, s.a, -(s.b * s.b));
+}
+
 // Candidate masks of an FP32 pre-test on cell-relative offsets (w < 0 marks
 // a particle clamped into a border cell): `fast` inside h by the FP32 band,
 // `exact` where the FP64 test decides (the band around h^2, r = 0, clamped

@@ -119,13 +119,13 @@ struct RealAcc {
                                              float inv_rhoj) {
         const float rinv = __builtin_amdgcn_rsqf(d2f);
         const float r = d2f * rinv;
-        const float xh = r * k.inv_h, xH = r * k.inv_H;
-        const float wh = spline_w(xh, k.c0h);
-        const float sh = spline_s(xh, rinv, k.c1h, k.inv_h);
+        const SplineAB sp = spline_ab(r, k.inv_h);
+        const float wh = spline_w_ab(sp, k.c0h);
+        const float sh = spline_s_ab(sp, rinv, k.c1h);
         const float Vj = vj.w;
         const float wr = wh * inv_rhoj;
         w += wh;
-        sH += spline_w(xH, k.c0H);
+        sH += spline_w_ab(spline_ab(r, k.inv_H), k.c0H);
         sV += Vj * Vj * wh;
         den += Vj * wh;
         nvx += vj.x * wr;
@@ -172,9 +172,9 @@ struct BAcc {
     __device__ __forceinline__ void add_fast(const KConst& k, float d2f, float dx, float dy,
                                              float dz, const float4& vj, float inv_rho_b) {
         const float rinv = __builtin_amdgcn_rsqf(d2f);
-        const float xh = d2f * rinv * k.inv_h;
-        const float wh = spline_w(xh, k.c0h);
-        const float sh = spline_s(xh, rinv, k.c1h, k.inv_h);
+        const SplineAB sp = spline_ab(d2f * rinv, k.inv_h);
+        const float wh = spline_w_ab(sp, k.c0h);
+        const float sh = spline_s_ab(sp, rinv, k.c1h);
         const float Vj = vj.w;
         const float wr = wh * inv_rho_b;
         den += Vj * wh;
