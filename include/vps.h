@@ -174,7 +174,7 @@ int64_t vps_sizeof_stats(void);
  * VirtualParticle(grid, grid.spacing()) (DualParticleSphLiquid.h:27-29).
  * VPS_ERR_INVALID_ARG for a domain of more than 2^30 cells, a lattice of
  * more than 2^31 - 1024 nodes, or (3D) more than 2^21 - 1 bricks of 8^3
- * nodes. */
+ * nodes or 2^24 or more cells (of size h) along x. */
 int vps_create(const vps_params* params, vps_context** out);
 void vps_destroy(vps_context* ctx);
 

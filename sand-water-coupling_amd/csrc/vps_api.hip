@@ -333,6 +333,11 @@ int vps_create(const vps_params* params, vps_context** out) {
         delete c;
         return fail(nullptr, VPS_ERR_INVALID_ARG, "domain or lattice too large");
     }
+    // the 3D cell offsets carry the cell's x index + 1 exactly in FP32
+    if (p.dim == 3 && k.cdims[0] >= (1 << 24)) {
+        delete c;
+        return fail(nullptr, VPS_ERR_INVALID_ARG, "domain too large (2^24 or more cells along x)");
+    }
     // the 3D CG iteration addresses the per-node arrays (bricks and the zero
     // brick, 2 KB each) with 32-bit byte offsets
     if (p.dim == 3 && (c->nbricks + 1) * 2048 > (int64_t)UINT32_MAX) {

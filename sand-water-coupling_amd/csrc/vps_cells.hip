@@ -136,7 +136,9 @@ __global__ void __launch_bounds__(256) k_density(const double4* pos,
 // BND: the boundary density (own searcher, DualParticleSphLiquid.cpp:96-97)
 // into nrm.w, V_b = m / rho_b into vel.w; otherwise rho and 1 / rho, and
 // V = m / rho into vel.w when set_volume.
-template <bool BND>
+// Off: the type of the candidates' byte offsets (uint32_t when the set's
+// offsets fit, so the loads take a scalar base and a 32-bit lane offset).
+template <bool BND, typename Off>
 __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const float4* ofs,
                                                    const int32_t* cstart, int64_t n,
                                                    KConst k, float* rho, float* rinv,
@@ -152,14 +154,13 @@ __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const flo
     const int x1 = cc0 + 1 < k.cdims[0] ? cc0 + 1 : k.cdims[0] - 1;
     const int zlo = cc2 > 0 ? cc2 - 1 : 0, zhi = cc2 + 1 < k.cdims[2] ? cc2 + 1 : k.cdims[2] - 1;
     const int ylo = cc1 > 0 ? cc1 - 1 : 0, yhi = cc1 + 1 < k.cdims[1] ? cc1 + 1 : k.cdims[1] - 1;
-    // q - C_c of the (up to) three cells x0, x0 + 1, x0 + 2 of each row,
-    // rounded once from FP64 (the staged x_j - C_j complete q - x_j)
-    float tx0 = (float)(q.x - cell_corner(k, 0, x0));
-    float tx1 = (float)(q.x - cell_corner(k, 0, x0 + 1));
-    float tx2 = (float)(q.x - cell_corner(k, 0, x0 + 2));
-    // kept in registers: without this the compiler selects the cell index in
-    // the candidate loop and recomputes the FP64 corner per candidate
-    asm volatile("" : "+v"(tx0), "+v"(tx1), "+v"(tx2));
+    // q - C_x0, rounded once from FP64 (the staged x_j - C_j complete
+    // q - x_j).  A candidate in cell x0 + d (d = 0, 1, 2; the offset's w is
+    // its cell x + 1) takes q - C_x0 - d cells, formed by one fma: exact
+    // when q - C_x0 lies within [1, 2] cells (Sterbenz), else rounded once.
+    const float tx0 = (float)(q.x - cell_corner(k, 0, x0));
+    const float csf = (float)k.cell_size;
+    const float x0w = (float)(x0 + 1);
     // a particle clamped into a border cell takes the FP64 distance throughout
     const float far = 2.f * (float)k.cell_size;
     const float qo0 = (float)(q.x - cell_corner(k, 0, cc0));
@@ -179,15 +180,14 @@ __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const flo
             const float ty = (float)(q.y - cell_corner(k, 1, y));
             const int64_t row = ((int64_t)z * k.cdims[1] + y) * k.cdims[0];
             const int b = cstart[row + x0], e = cstart[row + x1 + 1];
-            const int b1 = x0 + 1 <= x1 ? cstart[row + x0 + 1] : e;
-            const int b2 = x0 + 2 <= x1 ? cstart[row + x0 + 2] : e;
             for (int j = b; j < e; j++) {
-                const float4 p = ofs[j];
-                const float tx = j < b1 ? tx0 : (j < b2 ? tx1 : tx2);
+                const Off o = (Off)j * 16u;
+                const float4 p = *(const float4*)((const char*)ofs + o);
+                const float tx = fmaf(x0w - p.w, csf, tx0);
                 const float dx = tx - p.x, dy = ty - p.y, dz = tz - p.z;
                 float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                 if (p.w < 0.f || qfar) {   // rare: FP64 distance
-                    const double4 pj = pos[j];
+                    const double4 pj = *(const double4*)((const char*)pos + (uint64_t)o * 2u);
                     d2 = (float)sq_dist3(pj.x, pj.y, pj.z, q.x, q.y, q.z);
                 }
                 // v_sqrt_f32 (1 ulp): W is rounded to FP32 anyway
@@ -211,8 +211,9 @@ __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const flo
     }
 }
 
-// FP32 offset of every (sorted) particle from its cell corner; w = +1, or
-// -1 when the particle is clamped into a border cell (|offset| > 2 cells).
+// FP32 offset of every (sorted) particle from its cell corner; w = its cell
+// x + 1, negated when the particle is clamped into a border cell (|offset|
+// > 2 cells).  The walks read the sign; the density also the cell.
 __global__ void __launch_bounds__(256) k_cell_offsets3(const double4* pos,
                                                        int64_t n, KConst k,
                                                        float4* ofs) {
@@ -223,13 +224,16 @@ __global__ void __launch_bounds__(256) k_cell_offsets3(const double4* pos,
     const float far = 2.f * (float)k.cell_size;
     float of[3];
     bool is_far = false;
+    int cx = 0;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const int c = cell_coord(pc[a], k.cell_lo[a], k.cell_inv, k.cdims[a]);
+        if (a == 0) cx = c;
         of[a] = (float)(pc[a] - cell_corner(k, a, c));
         is_far |= !(fabsf(of[a]) <= far);
     }
-    ofs[i] = make_float4(of[0], of[1], of[2], is_far ? -1.f : 1.f);
+    const float w = (float)(cx + 1);
+    ofs[i] = make_float4(of[0], of[1], of[2], is_far ? -w : w);
 }
 
 // Boundary density from boundary-only neighbours (the boundary set has its
@@ -351,10 +355,16 @@ int real_density(vps_context* c, bool set_volume) {
     if (c->dim == 3) {
         hipLaunchKernelGGL(k_cell_offsets3, dim3(blocks(n, 256)), dim3(256), 0,
                            c->stream, c->rpos[s].p, n, c->k, c->rofs.p);
-        hipLaunchKernelGGL(k_density3f<false>, dim3(blocks(n, 256)), dim3(256), 0,
-                           c->stream, c->rpos[s].p, c->rofs.p, c->cell_start.p, n, c->k,
-                           c->rrho.p, c->rrinv.p, (float4*)nullptr, c->rvel[s].p,
-                           set_volume ? 1 : 0);
+        if (n * 16 <= (int64_t)UINT32_MAX)
+            hipLaunchKernelGGL((k_density3f<false, uint32_t>), dim3(blocks(n, 256)), dim3(256),
+                               0, c->stream, c->rpos[s].p, c->rofs.p, c->cell_start.p, n,
+                               c->k, c->rrho.p, c->rrinv.p, (float4*)nullptr, c->rvel[s].p,
+                               set_volume ? 1 : 0);
+        else
+            hipLaunchKernelGGL((k_density3f<false, uint64_t>), dim3(blocks(n, 256)), dim3(256),
+                               0, c->stream, c->rpos[s].p, c->rofs.p, c->cell_start.p, n,
+                               c->k, c->rrho.p, c->rrinv.p, (float4*)nullptr, c->rvel[s].p,
+                               set_volume ? 1 : 0);
     } else
         hipLaunchKernelGGL(k_density<2>, dim3(blocks(n, 256)), dim3(256), 0,
                            c->stream, c->rpos[s].p, c->cell_start.p, n, c->k,
@@ -381,9 +391,14 @@ int boundary_density(vps_context* c) {
         VPS_TRY(dalloc(c, c->bofs, (size_t)nb));
         hipLaunchKernelGGL(k_cell_offsets3, dim3(blocks(nb, 256)), dim3(256), 0,
                            c->stream, c->bpos.p, nb, c->k, c->bofs.p);
-        hipLaunchKernelGGL(k_density3f<true>, dim3(blocks(nb, 256)), dim3(256), 0,
-                           c->stream, c->bpos.p, c->bofs.p, c->bcell_start.p, nb, c->k,
-                           (float*)nullptr, (float*)nullptr, c->bnrm.p, c->bvel.p, 0);
+        if (nb * 16 <= (int64_t)UINT32_MAX)
+            hipLaunchKernelGGL((k_density3f<true, uint32_t>), dim3(blocks(nb, 256)), dim3(256),
+                               0, c->stream, c->bpos.p, c->bofs.p, c->bcell_start.p, nb, c->k,
+                               (float*)nullptr, (float*)nullptr, c->bnrm.p, c->bvel.p, 0);
+        else
+            hipLaunchKernelGGL((k_density3f<true, uint64_t>), dim3(blocks(nb, 256)), dim3(256),
+                               0, c->stream, c->bpos.p, c->bofs.p, c->bcell_start.p, nb, c->k,
+                               (float*)nullptr, (float*)nullptr, c->bnrm.p, c->bvel.p, 0);
         hipLaunchKernelGGL(k_boundary_offsets3, dim3(blocks(nb, 256)), dim3(256), 0,
                            c->stream, c->bpos.p, c->bnrm.p, nb, c->k, c->bofs.p);
     }
