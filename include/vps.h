@@ -442,6 +442,20 @@ int vps_query_neighbors(vps_context* ctx, int32_t which, int64_t nq,
  * SparseMatrix * vector product inside Eigen's CG (IterativeSolver.h:35). */
 int vps_debug_apply_operator(vps_context* ctx, const double* p, double* out);
 
+/* Test support, no counterpart in the reference (3D): the number of terms
+ * that each FP32 neighbour walk sums over, per particle in caller order.
+ * which = 0: the real density walk (real particles within h of each real
+ * particle, itself included; SmoothedParticles.cpp:25-31), 1: the boundary
+ * density walk (boundary particles within h of each boundary particle),
+ * 2: the pressure gradient's walk over the virtual particles within H of
+ * each real particle (VirtualParticle.cpp:118-124; -1 for a particle whose
+ * span is beyond the staged path).  The walks decide membership on FP32
+ * distances: only a pair within rounding of the cutoff, whose term is zero
+ * to FP32 resolution, can differ from the exact sets of
+ * vps_query_neighbors.  Call it on the current generation (after a
+ * projection, before advection). */
+int vps_debug_member_counts(vps_context* ctx, int32_t which, int32_t* counts);
+
 /* ---- learned warm start (no reference counterpart) ------------------- */
 
 /* Online training of the warm-start MLP on the (features, solution) pairs

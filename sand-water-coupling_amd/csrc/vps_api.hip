@@ -938,6 +938,34 @@ int vps_debug_apply_operator(vps_context* c, const double* p, double* out) {
     return VPS_OK;
 }
 
+int vps_debug_member_counts(vps_context* c, int32_t which, int32_t* counts) {
+    if (!c || !counts || which < 0 || which > 2) return VPS_ERR_INVALID_ARG;
+    if (c->dim != 3) return fail(c, VPS_ERR_INVALID_ARG, "member counts are 3D only");
+    if (which == 2 && (c->nslots == 0 || c->grad_span_cells <= 0))
+        return fail(c, VPS_ERR_STATE, "no virtual set or no staged gradient");
+    const int64_t n = which == 1 ? c->nb : c->n;
+    if (n == 0) return VPS_OK;
+    DBuf<int32_t> dm;
+    VPS_TRY(dalloc(c, dm, (size_t)n));
+    VPS_HIP(c, hipMemsetAsync(dm.p, 0xff, (size_t)n * 4, c->stream));   // -1
+    const int rc = which == 2 ? gradient_member_counts(c, dm.p)
+                              : density_member_counts(c, which, dm.p);
+    if (rc != VPS_OK) {
+        dfree(dm);
+        return rc;
+    }
+    std::vector<int32_t> m((size_t)n), id((size_t)n);
+    const int32_t* did = which == 1 ? c->bid.p : c->rid[c->cur].p;
+    VPS_HIP(c, hipMemcpyAsync(m.data(), dm.p, (size_t)n * 4, hipMemcpyDeviceToHost,
+                              c->stream));
+    VPS_HIP(c, hipMemcpyAsync(id.data(), did, (size_t)n * 4, hipMemcpyDeviceToHost,
+                              c->stream));
+    VPS_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(dm);
+    for (int64_t k = 0; k < n; k++) counts[id[(size_t)k]] = m[(size_t)k];
+    return VPS_OK;
+}
+
 }  // extern "C"
 
 namespace vps {

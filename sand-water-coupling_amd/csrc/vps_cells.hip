@@ -138,11 +138,14 @@ __global__ void __launch_bounds__(256) k_density(const double4* pos,
 // V = m / rho into vel.w when set_volume.
 // Off: the type of the candidates' byte offsets (uint32_t when the set's
 // offsets fit, so the loads take a scalar base and a 32-bit lane offset).
-template <bool BND, typename Off>
+// CNT (test support): only the number of terms that enter the sum (x < 1,
+// where the spline is nonzero) into members[i].
+template <bool BND, typename Off, bool CNT = false>
 __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const float4* ofs,
                                                    const int32_t* cstart, int64_t n,
                                                    KConst k, float* rho, float* rinv,
-                                                   float4* nrm, float4* vel, int set_volume) {
+                                                   float4* nrm, float4* vel, int set_volume,
+                                                   int32_t* members = nullptr) {
     const int nb = (int)((n + 255) / 256);
     const int64_t i = (int64_t)xcd_remap(blockIdx.x, nb) * 256 + threadIdx.x;
     if (i >= n) return;
@@ -174,6 +177,7 @@ __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const flo
     // lanes of a wave sit in different cells: per-cell loops would run the
     // longest cell of every lane in turn).
     float sum = 0.f;
+    int32_t nterms = 0;
     for (int z = zlo; z <= zhi; z++) {
         const float tz = (float)(q.z - cell_corner(k, 2, z));
         for (int y = ylo; y <= yhi; y++) {
@@ -197,8 +201,13 @@ __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const flo
                 const float a = __builtin_amdgcn_fmed3f(1.f - x, 0.f, 1.f);
                 const float c = __builtin_amdgcn_fmed3f(fmaf(-2.f, x, 1.f), 0.f, 1.f);
                 sum += fmaf(2.f * a, a * a, -(c * c * c));
+                if constexpr (CNT) nterms += x < 1.f;
             }
         }
+    }
+    if constexpr (CNT) {
+        members[i] = nterms;
+        return;
     }
     const float r = k.mass * k.c0h * sum;
     if constexpr (BND) {
@@ -378,6 +387,38 @@ int real_offsets(vps_context* c) {
     const int64_t n = c->n;
     hipLaunchKernelGGL(k_cell_offsets3, dim3(blocks(n, 256)), dim3(256), 0, c->stream,
                        c->rpos[c->cur].p, n, c->k, c->rofs.p);
+    VPS_HIP(c, hipGetLastError());
+    return VPS_OK;
+}
+
+// Terms of the FP32 density walks (test support): set 0 the real density
+// (real particles), 1 the boundary density (boundary particles, from offsets
+// recomputed with the cell index); members in the sorted order.
+int density_member_counts(vps_context* c, int which, int32_t* members) {
+    if (c->dim != 3 || (which != 0 && which != 1)) return VPS_ERR_INVALID_ARG;
+    const int64_t n = which == 0 ? c->n : c->nb;
+    if (n == 0) return VPS_OK;
+    if (which == 0) {
+        const int s = c->cur;
+        hipLaunchKernelGGL(k_cell_offsets3, dim3(blocks(n, 256)), dim3(256), 0, c->stream,
+                           c->rpos[s].p, n, c->k, c->rofs.p);
+        hipLaunchKernelGGL((k_density3f<false, uint64_t, true>), dim3(blocks(n, 256)),
+                           dim3(256), 0, c->stream, c->rpos[s].p, c->rofs.p, c->cell_start.p,
+                           n, c->k, (float*)nullptr, (float*)nullptr, (float4*)nullptr,
+                           (float4*)nullptr, 0, members);
+    } else {
+        DBuf<float4> ofs;
+        VPS_TRY(dalloc(c, ofs, (size_t)n));
+        hipLaunchKernelGGL(k_cell_offsets3, dim3(blocks(n, 256)), dim3(256), 0, c->stream,
+                           c->bpos.p, n, c->k, ofs.p);
+        hipLaunchKernelGGL((k_density3f<true, uint64_t, true>), dim3(blocks(n, 256)),
+                           dim3(256), 0, c->stream, c->bpos.p, ofs.p, c->bcell_start.p, n,
+                           c->k, (float*)nullptr, (float*)nullptr, (float4*)nullptr,
+                           (float4*)nullptr, 0, members);
+        const hipError_t e = hipStreamSynchronize(c->stream);
+        dfree(ofs);
+        VPS_HIP(c, e);
+    }
     VPS_HIP(c, hipGetLastError());
     return VPS_OK;
 }

@@ -231,11 +231,15 @@ __global__ void __launch_bounds__(kGradThreads) k_gradient_overflow(
     }
 }
 
+// CNT (test support): vq holds V_J instead of q_J, and only the number of
+// virtual particles whose term enters the sum (V_J > 0, r < H in the FP32
+// spline) goes into members[p]; spans beyond the stage are left out.
+template <bool CNT = false>
 __global__ void __launch_bounds__(kGradThreads, 4) k_gradient3s(
     KConst k, const int4* chunks, int nchunks, const double4* rpos, float4* rvel,
     const int32_t* slot_of_brick, const float* vq, const double4* bpos,
     const float4* bvel, const float4* bnrm, const int32_t* bcstart, int32_t* over,
-    int32_t* nover) {
+    int32_t* nover, int32_t* members = nullptr) {
     __shared__ float s_q[kGradCap];
     __shared__ int s_box[6];
     __shared__ int s_walls;   // boundary particles in the cells around the span
@@ -296,7 +300,7 @@ __global__ void __launch_bounds__(kGradThreads, 4) k_gradient3s(
     const int64_t e2 = (int64_t)s_box[5] - s_box[2] + 8;
     if (e0 * e1 * e2 > kGradCap) {
         // particles far outside their cells: left to k_gradient_overflow
-        if (t == 0) over[atomicAdd(nover, 1)] = chunk;
+        if (!CNT && t == 0) over[atomicAdd(nover, 1)] = chunk;
         return;
     }
     {
@@ -335,6 +339,7 @@ __global__ void __launch_bounds__(kGradThreads, 4) k_gradient3s(
         const double4 q = rpos[p];
         const float4 vi = rvel[p];
         float fx = 0.f, fy = 0.f, fz = 0.f;
+        int32_t nterms = 0;
         {
             const int f0 = node_floor(q.x, k.origin[0], inv);
             const int f1 = node_floor(q.y, k.origin[1], inv);
@@ -386,6 +391,7 @@ __global__ void __launch_bounds__(kGradThreads, 4) k_gradient3s(
                         const float sc = row[c] * rinv * fmaf(a, a, -(b * b));
                         sx = fmaf(sc, dxf[c], sx);
                         ss += sc;
+                        if constexpr (CNT) nterms += (a > 0.f) & (row[c] > 0.f);
                     }
                     fx -= sx;
                     fy -= ss * dyc;
@@ -395,6 +401,10 @@ __global__ void __launch_bounds__(kGradThreads, 4) k_gradient3s(
             fx *= c1;
             fy *= c1;
             fz *= c1;
+        }
+        if constexpr (CNT) {
+            members[p] = nterms;
+            continue;
         }
         if (s_walls) boundary_term<3>(k, q, vi, bpos, bvel, bnrm, bcstart, fx, fy, fz);
         float4 v = vi;
@@ -508,6 +518,21 @@ int gradient_spans(vps_context* c) {
     return VPS_OK;
 }
 
+// Terms of the staged gradient's virtual walk per real particle (test
+// support, sorted order; -1 stays where a span exceeds the stage).
+int gradient_member_counts(vps_context* c, int32_t* members) {
+    if (c->dim != 3 || c->grad_span_cells <= 0 || c->nslots == 0) return VPS_ERR_INVALID_ARG;
+    if (c->grad_nchunks > 0)
+        hipLaunchKernelGGL(k_gradient3s<true>, dim3((unsigned)c->grad_nchunks),
+                           dim3(kGradThreads), 0, c->stream, c->k, c->grad_chunks.p,
+                           (int)c->grad_nchunks, c->rpos[c->cur].p, c->rvel[c->cur].p,
+                           c->brick_slot[c->which].p, c->vvol.p, c->bpos.p, c->bvel.p,
+                           c->bnrm.p, c->bcell_start.p, (int32_t*)nullptr,
+                           (int32_t*)nullptr, members);
+    VPS_HIP(c, hipGetLastError());
+    return VPS_OK;
+}
+
 int pressure_gradient(vps_context* c) {
     const int64_t n = c->n;
     const int s = c->cur;
@@ -516,7 +541,7 @@ int pressure_gradient(vps_context* c) {
         if (c->grad_nchunks > 0) {
             int32_t* nover = (int32_t*)(c->counters.p + 3);   // zeroed with the spans
             int32_t* over = c->grad_over.p;
-            hipLaunchKernelGGL(k_gradient3s, dim3((unsigned)c->grad_nchunks),
+            hipLaunchKernelGGL(k_gradient3s<false>, dim3((unsigned)c->grad_nchunks),
                                dim3(kGradThreads), 0, c->stream, c->k,
                                c->grad_chunks.p, (int)c->grad_nchunks, c->rpos[s].p,
                                c->rvel[s].p, c->brick_slot[w].p, c->vq.p, c->bpos.p,

@@ -261,6 +261,7 @@ def declare(lib):
     _proto(lib, "vps_write_and_save_frame", C.c_int, [_P, C.c_char_p, C.c_int32])
     _proto(lib, "vps_load_frame_number", C.c_int, [_P, C.c_char_p, C.c_int32])
     _proto(lib, "vps_debug_apply_operator", C.c_int, [_P, _D, _D])
+    _proto(lib, "vps_debug_member_counts", C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)])
     return lib
 
 
@@ -282,7 +283,7 @@ EXPORTED = [
     "vps_move_particles", "vps_apply_gravity", "vps_apply_viscosity", "vps_advance",
     "vps_write_frame", "vps_save_frame", "vps_load_frame", "vps_write_description",
     "vps_write_end_frame", "vps_write_and_save_frame", "vps_load_frame_number",
-    "vps_debug_apply_operator",
+    "vps_debug_apply_operator", "vps_debug_member_counts",
 ]
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))

@@ -196,6 +196,15 @@ class Projection:
                                                _i64(idx), len(idx), C.byref(need)))
         return offs, idx[: need.value]
 
+    def member_counts(self, which):
+        """Terms per particle of the FP32 walks (0 real density, 1 boundary
+        density, 2 the gradient's virtual walk), caller order."""
+        n = self.nb if which == 1 else self.n
+        out = np.zeros(n, dtype=np.int32)
+        self._check(self.L.vps_debug_member_counts(
+            self.h, int(which), out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out
+
     def apply_operator(self, p):
         p = np.ascontiguousarray(p, dtype=np.float64)
         out = np.zeros_like(p)
