@@ -449,7 +449,9 @@ int vps_debug_apply_operator(vps_context* ctx, const double* p, double* out);
  * density walk (boundary particles within h of each boundary particle),
  * 2: the pressure gradient's walk over the virtual particles within H of
  * each real particle (VirtualParticle.cpp:118-124; -1 for a particle whose
- * span is beyond the staged path).  The walks decide membership on FP32
+ * span is beyond the staged path), 3: the viscosity walk (real particles
+ * within h of each real particle, itself excluded).  The walks decide
+ * membership on FP32
  * distances: only a pair within rounding of the cutoff, whose term is zero
  * to FP32 resolution, can differ from the exact sets of
  * vps_query_neighbors.  Call it on the current generation (after a

@@ -939,7 +939,7 @@ int vps_debug_apply_operator(vps_context* c, const double* p, double* out) {
 }
 
 int vps_debug_member_counts(vps_context* c, int32_t which, int32_t* counts) {
-    if (!c || !counts || which < 0 || which > 2) return VPS_ERR_INVALID_ARG;
+    if (!c || !counts || which < 0 || which > 3) return VPS_ERR_INVALID_ARG;
     if (c->dim != 3) return fail(c, VPS_ERR_INVALID_ARG, "member counts are 3D only");
     if (which == 2 && (c->nslots == 0 || c->grad_span_cells <= 0))
         return fail(c, VPS_ERR_STATE, "no virtual set or no staged gradient");
@@ -948,8 +948,9 @@ int vps_debug_member_counts(vps_context* c, int32_t which, int32_t* counts) {
     DBuf<int32_t> dm;
     VPS_TRY(dalloc(c, dm, (size_t)n));
     VPS_HIP(c, hipMemsetAsync(dm.p, 0xff, (size_t)n * 4, c->stream));   // -1
-    const int rc = which == 2 ? gradient_member_counts(c, dm.p)
-                              : density_member_counts(c, which, dm.p);
+    const int rc = which == 2   ? gradient_member_counts(c, dm.p)
+                   : which == 3 ? viscosity_member_counts(c, dm.p)
+                                : density_member_counts(c, which, dm.p);
     if (rc != VPS_OK) {
         dfree(dm);
         return rc;

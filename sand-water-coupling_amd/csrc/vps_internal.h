@@ -557,6 +557,7 @@ int pressure_gradient(vps_context* c);
 // test support: terms per particle (sorted order) of the FP32 walks
 int density_member_counts(vps_context* c, int which, int32_t* members);
 int gradient_member_counts(vps_context* c, int32_t* members);
+int viscosity_member_counts(vps_context* c, int32_t* members);
 int advect(vps_context* c, double dt, double g);
 int query_neighbors(vps_context* c, int which, int64_t nq, const double* q,
                     int64_t* offsets, int64_t* idx, int64_t cap,

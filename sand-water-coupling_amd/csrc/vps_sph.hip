@@ -101,11 +101,14 @@ struct ViscAcc {
 };
 
 // 3D from the FP32 cell-corner offsets (as the density): W' is zero at h,
-// so the FP32 distance decides which terms enter.
+// so the FP32 distance decides which terms enter.  CNT (test support): only
+// the number of terms into members[i].
+template <bool CNT = false>
 __global__ void __launch_bounds__(256) k_viscosity3f(const double4* pos, const float4* ofs,
                                                      const int32_t* cstart, const float4* vel,
                                                      const float* rho, int64_t n, KConst k,
-                                                     float scale, float4* vout) {
+                                                     float scale, float4* vout,
+                                                     int32_t* members = nullptr) {
     const int nb = (int)((n + 255) / 256);
     const int64_t i = (int64_t)xcd_remap(blockIdx.x, nb) * 256 + threadIdx.x;
     if (i >= n) return;
@@ -129,6 +132,7 @@ __global__ void __launch_bounds__(256) k_viscosity3f(const double4* pos, const f
     const bool qfar = !(fabsf(qo0) <= far && fabsf(qo1) <= far && fabsf(qo2) <= far);
     const float h2 = (float)k.h2;
     ViscAcc acc;
+    int32_t nterms = 0;
     for (int z = zlo; z <= zhi; z++) {
         const float tz = (float)(q.z - cell_corner(k, 2, z));
         for (int y = ylo; y <= yhi; y++) {
@@ -153,9 +157,17 @@ __global__ void __launch_bounds__(256) k_viscosity3f(const double4* pos, const f
                 }
                 const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                 if (!(d2 > 0.f) || !(d2 < h2)) continue;   // d = 0 is skipped
+                if constexpr (CNT) {
+                    nterms++;
+                    continue;
+                }
                 acc.add(k, d2, dx, dy, dz, vi, vel[j], rho[j]);
             }
         }
+    }
+    if constexpr (CNT) {
+        members[i] = nterms;
+        return;
     }
     float4 v = vi;
     v.x += scale * acc.lx;
@@ -227,6 +239,21 @@ int apply_gravity(vps_context* c, double dt, double g) {
     return VPS_OK;
 }
 
+// Terms of the 3D viscosity walk per real particle (test support, sorted
+// order), on offsets recomputed from the current positions.
+int viscosity_member_counts(vps_context* c, int32_t* members) {
+    const int64_t n = c->n;
+    if (c->dim != 3) return VPS_ERR_INVALID_ARG;
+    if (n == 0) return VPS_OK;
+    VPS_TRY(real_offsets(c));
+    const int s = c->cur;
+    hipLaunchKernelGGL(k_viscosity3f<true>, dim3(blocks(n, 256)), dim3(256), 0, c->stream,
+                       c->rpos[s].p, c->rofs.p, c->cell_start.p, c->rvel[s].p, c->rrho.p, n,
+                       c->k, 0.f, (float4*)nullptr, members);
+    VPS_HIP(c, hipGetLastError());
+    return VPS_OK;
+}
+
 int apply_viscosity(vps_context* c, double dt, double nu) {
     const int64_t n = c->n;
     if (nu == 0 || n == 0) return VPS_OK;
@@ -238,7 +265,7 @@ int apply_viscosity(vps_context* c, double dt, double nu) {
     // v += nu lapl dt with lapl = 2 (Dim + 2) m sum(...)
     const float scale = (float)(nu * (2.0 * (c->dim + 2.0) * c->p.mass) * dt);
     if (c->dim == 3)
-        hipLaunchKernelGGL(k_viscosity3f, dim3(blocks(n, 256)), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL(k_viscosity3f<false>, dim3(blocks(n, 256)), dim3(256), 0, c->stream,
                            c->rpos[s].p, c->rofs.p, c->cell_start.p, c->rvel[s].p, c->rrho.p,
                            n, c->k, scale, c->vel_next.p);
     else

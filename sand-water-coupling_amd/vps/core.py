@@ -198,7 +198,7 @@ class Projection:
 
     def member_counts(self, which):
         """Terms per particle of the FP32 walks (0 real density, 1 boundary
-        density, 2 the gradient's virtual walk), caller order."""
+        density, 2 the gradient's virtual walk, 3 viscosity), caller order."""
         n = self.nb if which == 1 else self.n
         out = np.zeros(n, dtype=np.int32)
         self._check(self.L.vps_debug_member_counts(

@@ -1,8 +1,8 @@
 """The sets the FP32 neighbour walks actually sum over (needs an MI355X).
 
-The density walks (real and boundary, SmoothedParticles.cpp:25-31) and the
-pressure gradient's walk over the virtual particles (VirtualParticle.cpp:
-118-124) decide membership on FP32 cell-relative distances, where the
+The density walks (real and boundary, SmoothedParticles.cpp:25-31), the
+viscosity walk and the pressure gradient's walk over the virtual particles
+(VirtualParticle.cpp:118-124) decide membership on FP32 cell-relative distances, where the
 reference tests the FP64 squared distance against h^2 (H^2).  Their terms
 vanish at the cutoff, so the fields cannot show a differing pair; here the
 number of terms each walk enters per particle (vps_debug_member_counts) is
@@ -61,6 +61,10 @@ def test_walk_member_counts(name):
     # real density walk: real particles within h, the particle itself included
     offs, _ = g.query(0, x)
     check_counts(g.member_counts(0), np.diff(offs), x, x, h)
+
+    # viscosity walk: the same pairs without the particle itself (the
+    # scene holds no coincident particles)
+    check_counts(g.member_counts(3) + 1, np.diff(offs), x, x, h)
 
     # boundary density walk: boundary particles within h of each boundary particle
     bx = g.boundary()[0]
