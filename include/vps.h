@@ -283,6 +283,18 @@ int vps_set_check_true_residual(vps_context* ctx, int32_t enable);
  * execution), reported as stats.ms_spmv_avg / ms_update_avg. */
 int vps_set_kernel_timing(vps_context* ctx, int32_t enable);
 
+/* Asynchronous steps (no counterpart in the reference, whose steps run on
+ * the host): with enable != 0, vps_project, vps_projection_step and
+ * vps_advance return once the step's work is queued on the context's
+ * stream instead of waiting for its end, so the next step's kernels queue
+ * behind it with no idle gap.  Their stats then hold zero phase times
+ * (ms_*); the counts, iterations and residuals are known on the host before
+ * the step's last kernels.  Every call that returns data (vps_get_*,
+ * frames) and vps_synchronize wait for the queued work; a device error of a
+ * queued step is reported by a later call.  Default 0 (each step waits for
+ * its end). */
+int vps_set_async_steps(vps_context* ctx, int32_t enable);
+
 /* Blocks until all work on the context's stream is done. */
 int vps_synchronize(vps_context* ctx);
 

@@ -693,6 +693,12 @@ int vps_project(vps_context* c, vps_stats* st) {
     if (st) std::memset(st, 0, sizeof(*st));
     const int rc = project_impl(c, st, 4);
     if (rc != VPS_OK && rc != VPS_ERR_NO_CONVERGENCE) return rc;
+    if (c->async_steps) {
+        // the step's last kernels stay queued: no phase times
+        if (st) fill_common_stats(c, st);
+        c->generated = false;
+        return rc;
+    }
     VPS_HIP(c, hipStreamSynchronize(c->stream));
     if (st) {
         fill_common_stats(c, st);
@@ -757,6 +763,12 @@ int vps_advance(vps_context* c, const vps_step_params* sp, vps_stats* st) {
     VPS_HIP(c, hipEventRecord(c->ev[2], c->stream));
     const int rc = project_impl(c, st, 4);
     if (rc != VPS_OK && rc != VPS_ERR_NO_CONVERGENCE) return rc;
+    if (c->async_steps) {
+        // the step's last kernels stay queued: no phase times
+        if (st) fill_common_stats(c, st);
+        c->generated = false;
+        return rc;
+    }
     VPS_HIP(c, hipStreamSynchronize(c->stream));
     if (st) {
         fill_common_stats(c, st);
@@ -784,6 +796,12 @@ int vps_projection_step(vps_context* c, vps_stats* st) {
     VPS_HIP(c, hipEventRecord(c->ev[2], c->stream));
     const int rc = project_impl(c, st, 4);
     if (rc != VPS_OK && rc != VPS_ERR_NO_CONVERGENCE) return rc;
+    if (c->async_steps) {
+        // the step's last kernels stay queued: no phase times
+        if (st) fill_common_stats(c, st);
+        c->generated = false;
+        return rc;
+    }
     VPS_HIP(c, hipStreamSynchronize(c->stream));
     if (st) {
         fill_common_stats(c, st);
@@ -817,6 +835,12 @@ int vps_set_check_true_residual(vps_context* c, int32_t enable) {
 int vps_set_kernel_timing(vps_context* c, int32_t enable) {
     if (!c) return VPS_ERR_INVALID_ARG;
     c->kernel_timing = enable != 0;
+    return VPS_OK;
+}
+
+int vps_set_async_steps(vps_context* c, int32_t enable) {
+    if (!c) return VPS_ERR_INVALID_ARG;
+    c->async_steps = enable != 0;
     return VPS_OK;
 }
 

@@ -488,6 +488,7 @@ struct vps_context {
     hipEvent_t ev[16] = {};
     std::vector<hipEvent_t> kev;        // per-launch events (SpMV/update)
     bool kernel_timing = false;
+    bool async_steps = false;   // steps return without waiting for their end
 
     // learned warm start (vps_mlp.hip)
     void* mlp = nullptr;

@@ -224,6 +224,7 @@ def declare(lib):
     _proto(lib, "vps_projection_step", C.c_int, [_P, C.POINTER(Stats)])
     _proto(lib, "vps_advect", C.c_int, [_P, C.c_double, C.c_double])
     _proto(lib, "vps_set_check_true_residual", C.c_int, [_P, C.c_int32])
+    _proto(lib, "vps_set_async_steps", C.c_int, [_P, C.c_int32])
     _proto(lib, "vps_synchronize", C.c_int, [_P])
     _proto(lib, "vps_get_virtual_count", C.c_int, [_P, _I64])
     _proto(lib, "vps_get_virtual", C.c_int, [_P, _I64, _D, _D, _D, _D, _D])
@@ -272,6 +273,7 @@ EXPORTED = [
     "vps_set_particles", "vps_get_particles", "vps_initialize",
     "vps_refresh_real", "vps_generate_particles", "vps_project",
     "vps_projection_step", "vps_advect", "vps_set_check_true_residual",
+    "vps_set_async_steps",
     "vps_synchronize", "vps_get_virtual_count", "vps_get_virtual",
     "vps_get_real_fields", "vps_get_boundary_fields", "vps_get_scalars",
     "vps_query_neighbors", "vps_mlp_set_training", "vps_mlp_parameter_count",

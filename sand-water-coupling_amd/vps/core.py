@@ -149,6 +149,10 @@ class Projection:
     def set_check_true_residual(self, on):
         self._check(self.L.vps_set_check_true_residual(self.h, 1 if on else 0))
 
+    def set_async_steps(self, on):
+        """Steps return once queued (no wait for their end; zero phase times)."""
+        self._check(self.L.vps_set_async_steps(self.h, 1 if on else 0))
+
     def set_kernel_timing(self, on):
         self._check(self.L.vps_set_kernel_timing(self.h, 1 if on else 0))
 

@@ -258,6 +258,30 @@ def test_deterministic():
     assert np.array_equal(runs[0], runs[1])
 
 
+def test_async_steps_match_waiting_steps():
+    """Steps that return once queued (vps_set_async_steps) give the same
+    iteration counts and the same particles, bit for bit, as steps that wait
+    for their end; their phase times are zero."""
+    sc = scene("3d")
+    cols = scenes.box_colliders(sc)
+    runs = []
+    for queued in (False, True):
+        g = Projection(sc, **INIT)
+        g.initialize()
+        g.set_async_steps(queued)
+        st = [g.advance(1e-3, gravity=9.8, viscosity=0.0, colliders=cols,
+                        enable_gravity=True) for _ in range(4)]
+        if queued:
+            assert all(s["ms_total"] == 0 for s in st)
+        else:
+            assert all(s["ms_total"] > 0 for s in st)
+        runs.append(([s["cg_iterations"] for s in st], g.particles()))
+        g.close()
+    assert runs[0][0] == runs[1][0]
+    assert np.array_equal(runs[0][1][0], runs[1][1][0])
+    assert np.array_equal(runs[0][1][1], runs[1][1][1])
+
+
 def test_error_paths():
     sc = scene("2d")
     g = Projection(sc, **INIT)
