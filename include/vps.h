@@ -74,16 +74,17 @@ enum vps_warm_start {
     VPS_WARM_ZERO = 1,         /* x0 = 0 */
     VPS_WARM_NODE_KEYED = 2,   /* x0 = previous pressure at the same lattice
                                   node, zero for nodes that are new */
-    VPS_WARM_LEARNED = 3,      /* x0 = p1 + e + tau * MLP(features), the
-                                  learned correction to the extrapolation
-                                  below, tau the RMS error of that
-                                  extrapolation at the last solve (no
-                                  reference counterpart); an untrained
-                                  network gives the extrapolation.  This
-                                  start and the next are guarded: one whose
-                                  initial residual is not finite or exceeds
-                                  ||b|| is replaced by the index-carried
-                                  start */
+    VPS_WARM_LEARNED = 3,      /* x0 = y0 p1 + y1 e + y2 c with c = tau *
+                                  MLP(features), the learned correction to
+                                  the extrapolation below (tau the RMS error
+                                  of that extrapolation at the last solve),
+                                  and y the Galerkin weights (V^T M V) y =
+                                  V^T b, V = [p1 e c], formed on the device
+                                  per solve (no reference counterpart).
+                                  This start and the next are guarded: one
+                                  whose initial residual is not finite or
+                                  exceeds ||b|| is replaced by the
+                                  index-carried start */
     VPS_WARM_EXTRAPOLATE = 4   /* x0 = p1 + e: polynomial extrapolation of the
                                   node's last three pressures, e = dp1 with two
                                   known, 2 dp1 - dp2 with three (no reference

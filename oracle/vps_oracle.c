@@ -952,6 +952,90 @@ int or_pressure_gradient(or_sim* s) {
     return 0;
 }
 
+/* The learned start of the HIP library with an untrained network (no
+ * reference counterpart): the Galerkin projection onto the span of
+ * P = p1 and E = e (the extrapolation's increment), x0 = y0 P + y1 E with
+ * (V^T M V) y = V^T b, V = [P E], over the fields with v . M v > 0 and by
+ * elimination with partial pivoting (a pivot not above 1e-10 of its
+ * diagonal keeps the extrapolation P + E).  Nodes without a history stay
+ * zero; s->pres holds the extrapolation on entry. */
+static void learned_start(or_sim* s, int64_t n) {
+    double* V[2];
+    double* MV[2];
+    for (int k = 0; k < 2; k++) {
+        V[k] = (double*)calloc((size_t)(n ? n : 1), sizeof(double));
+        MV[k] = (double*)malloc((size_t)(n ? n : 1) * sizeof(double));
+    }
+    for (int64_t I = 0; I < n; I++) {
+        const int64_t nd = s->node[I];
+        const int h = s->prev_node_h[nd];
+        if (h >= 1) {
+            V[0][I] = s->prev_node_p[nd];
+            V[1][I] = h >= 3 ? 2 * s->prev_node_dp1[nd] - s->prev_node_dp2[nd]
+                             : (h == 2 ? s->prev_node_dp1[nd] : 0.);
+        }
+    }
+    for (int k = 0; k < 2; k++) csr_mv(n, s->rowptr, s->col, s->val, V[k], MV[k]);
+    double t[2][2] = {{0, 0}, {0, 0}}, rb[2] = {0, 0};
+    for (int64_t I = 0; I < n; I++)
+        for (int p = 0; p < 2; p++) {
+            for (int q = 0; q < 2; q++) t[p][q] += V[p][I] * MV[q][I];
+            rb[p] += V[p][I] * s->div[I];
+        }
+    int idx[2], m = 0;
+    for (int i = 0; i < 2; i++)
+        if (t[i][i] > 0 && isfinite(t[i][i])) idx[m++] = i;
+    double G[2][2], r[2], y[2] = {1, 1};
+    for (int p = 0; p < m; p++) {
+        r[p] = rb[idx[p]];
+        for (int q = 0; q < m; q++) G[p][q] = t[idx[p]][idx[q]];
+    }
+    int ok = m > 0;
+    double dg[2] = {m > 0 ? fabs(G[0][0]) : 0, m > 1 ? fabs(G[1][1]) : 0};
+    if (m == 2 && fabs(G[1][0]) > fabs(G[0][0])) {   /* partial pivoting */
+        for (int q = 0; q < 2; q++) {
+            const double tmp = G[0][q];
+            G[0][q] = G[1][q];
+            G[1][q] = tmp;
+        }
+        const double tr = r[0];
+        r[0] = r[1];
+        r[1] = tr;
+    }
+    for (int k = 0; k < m && ok; k++) {
+        if (!(fabs(G[k][k]) > 1e-10 * dg[k])) {
+            ok = 0;
+            break;
+        }
+        for (int p = k + 1; p < m; p++) {
+            const double f = G[p][k] / G[k][k];
+            for (int q = k; q < m; q++) G[p][q] -= f * G[k][q];
+            r[p] -= f * r[k];
+        }
+    }
+    if (ok) {
+        double x[2];
+        for (int p = m - 1; p >= 0; p--) {
+            double sum = r[p];
+            for (int q = p + 1; q < m; q++) sum -= G[p][q] * x[q];
+            x[p] = sum / G[p][p];
+        }
+        int fin = 1;
+        for (int p = 0; p < m; p++) fin = fin && isfinite(x[p]);
+        if (fin) {
+            y[0] = y[1] = 0;
+            for (int p = 0; p < m; p++) y[idx[p]] = x[p];
+        }
+    }
+    if (ok)
+        for (int64_t I = 0; I < n; I++) s->pres[I] = y[0] * V[0][I] + y[1] * V[1][I];
+    memcpy(s->x0, s->pres, (size_t)n * sizeof(double));
+    for (int k = 0; k < 2; k++) {
+        free(V[k]);
+        free(MV[k]);
+    }
+}
+
 /* ---- VirtualParticle::project (VirtualParticle.cpp:75-134) ------------- */
 int or_project(or_sim* s, or_stats* st) {
     if (!s->initialized) return VPS_ERR_STATE;
@@ -975,8 +1059,9 @@ int or_project(or_sim* s, or_stats* st) {
                     p_new[I] = s->prev_node_p[s->node[I]];
     } else if (s->p.warm_start == VPS_WARM_EXTRAPOLATE ||
                s->p.warm_start == VPS_WARM_LEARNED) {
-        /* polynomial extrapolation through the known history (the learned
-         * start of the HIP library with an untrained network) */
+        /* polynomial extrapolation through the known history; the learned
+         * start (of the HIP library with an untrained network) is formed
+         * from it once the matrix exists, below */
         if (s->prev_node_p)
             for (int64_t I = 0; I < n; I++) {
                 const int64_t nd = s->node[I];
@@ -1023,6 +1108,7 @@ int or_project(or_sim* s, or_stats* st) {
         free(p_carry);
         return rc;
     }
+    if (s->p.warm_start == VPS_WARM_LEARNED && s->prev_node_p) learned_start(s, n);
     /* guarded starts (extrapolated or learned, no reference counterpart):
      * one whose residual is not finite or above ||b|| gives way to the
      * index-carried start, as in the HIP library */

@@ -6,10 +6,15 @@
 // polynomial extrapolation through the known values is the prior
 //     e = 0 (h <= 1),  dp1 (h = 2),  2 dp1 - dp2 (h = 3)   (p_next ~ p1 + e)
 // and a per-virtual-particle MLP 8 -> 64 -> 64 -> 1 (tanh-form GELU, BF16 weights and
-// activations, FP32 accumulation) learns the correction to it:
-//     x0_I = p1_I + e_I + tau * net(f_I)     (nodes with h >= 1)
-//     x0_I = 0                               (nodes that are new)
-// with features
+// activations, FP32 accumulation) learns the correction c = tau * net(f)
+// to it.  The start is the Galerkin projection of the solution onto the span
+// of the three fields (zero on nodes that are new):
+//     x0 = y0 p1 + y1 e + y2 c,   (V^T M V) y = V^T b,   V = [p1 e c]
+// (the residual of x0 is orthogonal to the three fields; for a symmetric M
+// it is the start of least M-norm error in that span, which holds the
+// node-keyed start p1, the extrapolation p1 + e and the network's own
+// p1 + e + c).  It costs three applications of M and twelve dot products per
+// solve.  The network's features are
 //     f0 = p1 / ps, f1 = dp1 / dps, f2 = dp2 / dps    the node history
 //     f3 = D_I / ds                                the CG right-hand side
 //     f4 = diag_I / alpha0 - 1, f5 = s_I / alpha0 - 1   clamped / unclamped
@@ -22,8 +27,8 @@
 // network's output is in units of the error it corrects, so its BF16 and
 // Adam-step noise stays far below that error as the flow settles.  The
 // scales live on the device and are refreshed after every solve, so no host
-// round trip is added.  The output layer starts at zero: an untrained network gives the
-// extrapolation (VPS_WARM_EXTRAPOLATE) exactly.  CG always runs to the same
+// round trip is added.  The output layer starts at zero: an untrained network
+// (c = 0) gives the Galerkin start over p1 and e.  CG always runs to the same
 // tolerance, so the warm start changes the iteration count, not the answer.
 //
 // Kernels (all matrix work on MFMA v_mfma_f32_16x16x16_bf16, 64-lane
@@ -195,6 +200,7 @@ struct FeatSrc {
     const float* vvol;
     const float* scale;   // device scales (SC_*)
     float inv_a0, inv_vs;
+    float* corr;          // the network's correction c per node, or null
 };
 
 // Polynomial extrapolation of the node history (the prior of the network).
@@ -355,14 +361,159 @@ __global__ void __launch_bounds__(256) k_mlp_infer_pipe(const float* P, FeatSrc 
             Fwd F;
             forward<false>(W, nullptr, xb, F);
             // lanes 0..15 hold node r's output and (q = 0) its raw history
-            if (lane < 16 && cur.g < nn)
+            if (lane < 16 && cur.g < nn) {
                 out[cur.g] = ok ? cur.v[0] + prior(cur.h, cur.v[1], cur.v[2]) + tau * F.out
                                 : 0.f;
+                if (fs.corr) fs.corr[cur.g] = ok ? tau * F.out : 0.f;
+            }
         } else if (lane < 16 && cur.g < nn) {
             out[cur.g] = 0.f;
+            if (fs.corr) fs.corr[cur.g] = 0.f;
         }
         cur = nxt;
     }
+}
+
+// The fields of the Galerkin start: P = p1 and E = e on the nodes with a
+// history, zero elsewhere (C, the network's correction, is written by the
+// inference).
+__global__ void __launch_bounds__(256) k_galerkin_fields(FeatSrc fs, int64_t nn, float* P,
+                                                         float* E) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= nn) return;
+    const int h = fs.hh[g];
+    const bool ok = fs.vvol[g] > 0.f && h >= 1;
+    P[g] = ok ? fs.p1[g] : 0.f;
+    E[g] = ok ? prior(h, fs.dp1[g], fs.dp2[g]) : 0.f;
+}
+
+// Block partials of v_i . M v_j (i, j = 0..2) and v_i . b for V = [P E C].
+constexpr int kNGram = 12;
+__global__ void __launch_bounds__(256) k_galerkin_gram(int64_t nn, const float* P,
+                                                       const float* E, const float* Cc,
+                                                       const float* MP, const float* ME,
+                                                       const float* MC, const float* b,
+                                                       double* part) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double a[kNGram];
+#pragma unroll
+    for (int j = 0; j < kNGram; j++) a[j] = 0;
+    if (g < nn) {
+        const double v[3] = {P[g], E[g], Cc[g]};
+        const double mv[3] = {MP[g], ME[g], MC[g]};
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+#pragma unroll
+            for (int j = 0; j < 3; j++) a[3 * i + j] = v[i] * mv[j];
+            a[9 + i] = v[i] * (double)b[g];
+        }
+    }
+    __shared__ double sw[kNGram][4];
+#pragma unroll
+    for (int j = 0; j < kNGram; j++) {
+        a[j] = wave_sum(a[j]);
+        if ((threadIdx.x & 63) == 0) sw[j][threadIdx.x >> 6] = a[j];
+    }
+    __syncthreads();
+    if (threadIdx.x < kNGram) {
+        const double* w = sw[threadIdx.x];
+        part[(int64_t)blockIdx.x * kNGram + threadIdx.x] = (w[0] + w[1]) + (w[2] + w[3]);
+    }
+}
+
+// Sums the partials (fixed order) and solves the 3 x 3 Galerkin system
+// (V^T M V) y = V^T b by elimination with partial pivoting over the fields
+// that are present (a field with v . M v = 0 takes weight 0); y = (1, 1, 1),
+// the network's own start, if the system is singular or the weights are not
+// finite.
+__global__ void __launch_bounds__(1024) k_galerkin_solve(const double* part, int nblk,
+                                                         double* y) {
+    __shared__ double sw[kNGram][16];
+    double a[kNGram];
+#pragma unroll
+    for (int j = 0; j < kNGram; j++) a[j] = 0;
+    for (int i = threadIdx.x; i < nblk; i += 1024)
+#pragma unroll
+        for (int j = 0; j < kNGram; j++) a[j] += part[(int64_t)i * kNGram + j];
+#pragma unroll
+    for (int j = 0; j < kNGram; j++) {
+        a[j] = wave_sum(a[j]);
+        if ((threadIdx.x & 63) == 0) sw[j][threadIdx.x >> 6] = a[j];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double t[kNGram];
+    for (int j = 0; j < kNGram; j++) {
+        t[j] = 0;
+        for (int i = 0; i < 16; i++) t[j] += sw[j][i];
+    }
+    double G[3][3], r[3];
+    int idx[3], m = 0;
+    for (int i = 0; i < 3; i++)
+        if (t[4 * i] > 0 && std::isfinite(t[4 * i])) idx[m++] = i;   // v_i . M v_i
+    for (int p = 0; p < m; p++) {
+        r[p] = t[9 + idx[p]];
+        for (int q = 0; q < m; q++) G[p][q] = t[3 * idx[p] + idx[q]];
+    }
+    // Gaussian elimination with partial pivoting; a pivot below 1e-10 of its
+    // row's diagonal (a field nearly in the span of the others) rejects
+    bool ok = true;
+    int perm[3] = {0, 1, 2};
+    double dg[3];
+    for (int p = 0; p < m; p++) dg[p] = fabs(G[p][p]);
+    for (int k = 0; k < m && ok; k++) {
+        int piv = k;
+        for (int p = k + 1; p < m; p++)
+            if (fabs(G[p][k]) > fabs(G[piv][k])) piv = p;
+        if (piv != k) {
+            for (int q = 0; q < m; q++) {
+                const double tmp = G[k][q];
+                G[k][q] = G[piv][q];
+                G[piv][q] = tmp;
+            }
+            const double tr = r[k];
+            r[k] = r[piv];
+            r[piv] = tr;
+            const int tp = perm[k];
+            perm[k] = perm[piv];
+            perm[piv] = tp;
+        }
+        if (!(fabs(G[k][k]) > 1e-10 * dg[k])) {
+            ok = false;
+            break;
+        }
+        for (int p = k + 1; p < m; p++) {
+            const double f = G[p][k] / G[k][k];
+            for (int q = k; q < m; q++) G[p][q] -= f * G[k][q];
+            r[p] -= f * r[k];
+        }
+    }
+    (void)perm;
+    double out[3] = {1, 1, 1};
+    if (ok && m > 0) {
+        double x[3];
+        for (int p = m - 1; p >= 0; p--) {
+            double sum = r[p];
+            for (int q = p + 1; q < m; q++) sum -= G[p][q] * x[q];
+            x[p] = sum / G[p][p];
+        }
+        bool fin = true;
+        for (int p = 0; p < m; p++) fin = fin && std::isfinite(x[p]);
+        if (fin) {
+            out[0] = out[1] = out[2] = 0;
+            for (int p = 0; p < m; p++) out[idx[p]] = x[p];
+        }
+    }
+    for (int i = 0; i < 3; i++) y[i] = out[i];
+}
+
+__global__ void __launch_bounds__(256) k_galerkin_combine(int64_t nn, const float* P,
+                                                          const float* E, const float* Cc,
+                                                          const double* y, float* x0) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= nn) return;
+    const float y0 = (float)y[0], y1 = (float)y[1], y2 = (float)y[2];
+    x0[g] = fmaf(y2, Cc[g], fmaf(y1, E[g], y0 * P[g]));
 }
 
 // x0 = p1 + e (the untrained network's answer, without the MLP).
@@ -776,6 +927,7 @@ struct MlpState {
     DBuf<uint8_t> H[2];
     DBuf<float> hp1, hdp1, hdp2;
     DBuf<uint8_t> hh;
+    DBuf<double> gpart, gy;      // Galerkin start: block partials, weights
     DBuf<int32_t> slot_of_rank;
     DBuf<double> part;
     bool training = false;
@@ -889,6 +1041,8 @@ static int ensure_history(vps_context* c) {
     VPS_TRY(dalloc(c, m->hh, cap));
     VPS_TRY(dalloc(c, m->slot_of_rank, cap));
     VPS_TRY(dalloc(c, m->part, kNPart * (cap / 256 + 16)));
+    VPS_TRY(dalloc(c, m->gpart, kNGram * (cap / 256 + 16)));
+    VPS_TRY(dalloc(c, m->gy, 3));
     return VPS_OK;
 }
 
@@ -913,6 +1067,8 @@ void mlp_destroy(vps_context* c) {
     dfree(m->hdp1);
     dfree(m->hdp2);
     dfree(m->hh);
+    dfree(m->gpart);
+    dfree(m->gy);
     dfree(m->slot_of_rank);
     dfree(m->part);
     delete m;
@@ -963,10 +1119,30 @@ int mlp_warm_start(vps_context* c, int mode, float* x0) {
                        m->hp1.p, m->hdp1.p, m->hdp2.p, m->hh.p);
     VPS_HIP(c, hipGetLastError());
     if (mode == VPS_WARM_LEARNED) {
+        // The Galerkin start over [p1 e c], with the CG's scratch vectors:
+        // inputs of M in vp[0], vp[1] and vr2 (their zero brick stays zero),
+        // products in vt, vt2 and vr.  The CG overwrites all of them.
+        float* P = c->vp[0].p;
+        float* E = c->vp[1].p;
+        float* Cc = c->vr2.p;
+        FeatSrc fs = feat_src(c, m->step_scale);
+        fs.corr = Cc;
         // about 8 tiles of 16 nodes per wave
         const int64_t nblk = std::min<int64_t>(std::max<int64_t>((nn + 511) / 512, 1), 8192);
         hipLaunchKernelGGL(k_mlp_infer_pipe, dim3((unsigned)nblk), dim3(256), 0,
-                           c->stream, m->P.p, feat_src(c, m->step_scale), nn, x0);
+                           c->stream, m->P.p, fs, nn, x0);
+        hipLaunchKernelGGL(k_galerkin_fields, dim3(blocks(nn, 256)), dim3(256), 0, c->stream,
+                           fs, nn, P, E);
+        VPS_TRY(apply_operator_f32(c, P, c->vt.p, c->p.op_mode));
+        VPS_TRY(apply_operator_f32(c, E, c->vt2.p, c->p.op_mode));
+        VPS_TRY(apply_operator_f32(c, Cc, c->vr.p, c->p.op_mode));
+        const int nb = (int)blocks(nn, 256);
+        hipLaunchKernelGGL(k_galerkin_gram, dim3(nb), dim3(256), 0, c->stream, nn, P, E, Cc,
+                           c->vt.p, c->vt2.p, c->vr.p, c->vb.p, m->gpart.p);
+        hipLaunchKernelGGL(k_galerkin_solve, dim3(1), dim3(1024), 0, c->stream, m->gpart.p,
+                           nb, m->gy.p);
+        hipLaunchKernelGGL(k_galerkin_combine, dim3(nb), dim3(256), 0, c->stream, nn, P, E,
+                           Cc, m->gy.p, x0);
     } else if (mode == VPS_WARM_EXTRAPOLATE) {
         hipLaunchKernelGGL(k_extrapolate, dim3(blocks(nn, 256)), dim3(256), 0,
                            c->stream, feat_src(c, m->step_scale), nn, x0);

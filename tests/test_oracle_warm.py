@@ -59,3 +59,54 @@ def test_node_keyed_and_depth_one_agree():
     # second solve: every node has at most one known pressure
     assert np.array_equal(g_n[1], g_e[1])
     assert it_n == it_e
+
+
+def test_learned_start_is_the_galerkin_projection():
+    """The learned start of an untrained network: x0 lies in the span of the
+    node-keyed pressures P and the extrapolation's increments E, and its
+    residual b - M x0 is orthogonal to both (the Galerkin condition
+    V^T (b - M V y) = 0 with the CG's own matrix M)."""
+    import scipy.sparse as sp
+    sc = scenes.small_3d(12, 8, 6)
+    o = O.Oracle(sc, warm_start=_abi.WARM_LEARNED, rest_density_mode=_abi.FROM_INIT,
+                 kappa_mode=_abi.FROM_INIT)
+    o.initialize()
+    hist = []
+    checked = 0
+    for k in range(6):
+        o.advect(1e-3, 9.8)
+        st = o.projection_step()
+        v = o.virtual()
+        x0 = o.initial_guess()
+        rp, col, val = o.matrix()
+        n = len(v["node"])
+        M = sp.csr_matrix((val, col, rp), shape=(n, n))
+        P = np.zeros(n)
+        E = np.zeros(n)
+        for I, nd in enumerate(v["node"].tolist()):
+            past = []
+            for j in range(len(hist) - 1, max(len(hist) - 4, -1), -1):
+                if nd not in hist[j]:
+                    break
+                past.append(hist[j][nd])
+            if past:
+                P[I] = past[0]
+                E[I] = (2 * past[0] - 3 * past[1] + past[2] - 0.0) if len(past) == 3 \
+                    else (past[0] - past[1] if len(past) == 2 else 0.0)
+        hist.append(dict(zip(v["node"].tolist(), v["p"].tolist())))
+        if k < 2 or not np.any(E):
+            continue
+        # E above is the polynomial's increment over the last value (2 dp1 -
+        # dp2 with three known pressures).  A start rejected by the guard is
+        # the index-carried one, outside the span: not this check's case.
+        y, *_ = np.linalg.lstsq(np.stack([P, E], 1), x0, rcond=None)
+        if np.linalg.norm(x0 - y[0] * P - y[1] * E) > 1e-10 * np.linalg.norm(x0):
+            continue
+        r0 = v["div"] - M @ x0
+        for f in (P, E):
+            assert abs(r0 @ f) <= 1e-8 * np.linalg.norm(r0) * np.linalg.norm(f)
+        # and the weights are not the plain extrapolation's (1, 1)
+        assert abs(y[0] - 1) + abs(y[1] - 1) > 1e-6
+        checked += 1
+    o.close()
+    assert checked >= 3
