@@ -387,25 +387,28 @@ __global__ void __launch_bounds__(256) k_galerkin_fields(FeatSrc fs, int64_t nn,
     E[g] = ok ? prior(h, fs.dp1[g], fs.dp2[g]) : 0.f;
 }
 
-// Block partials of v_i . M v_j (i, j = 0..2) and v_i . b for V = [P E C].
+// Block partials of v_i . M v_j (i, j = 0..2) and v_i . b for V = [P E C]
+// (grid-stride over the nodes: kGramBlocks partials for the solve to sum).
 constexpr int kNGram = 12;
+constexpr int kGramBlocks = 512;
 __global__ void __launch_bounds__(256) k_galerkin_gram(int64_t nn, const float* P,
                                                        const float* E, const float* Cc,
                                                        const float* MP, const float* ME,
                                                        const float* MC, const float* b,
                                                        double* part) {
-    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
     double a[kNGram];
 #pragma unroll
     for (int j = 0; j < kNGram; j++) a[j] = 0;
-    if (g < nn) {
+    for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < nn;
+         g += (int64_t)gridDim.x * 256) {
         const double v[3] = {P[g], E[g], Cc[g]};
         const double mv[3] = {MP[g], ME[g], MC[g]};
+        const double bg = b[g];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
 #pragma unroll
-            for (int j = 0; j < 3; j++) a[3 * i + j] = v[i] * mv[j];
-            a[9 + i] = v[i] * (double)b[g];
+            for (int j = 0; j < 3; j++) a[3 * i + j] += v[i] * mv[j];
+            a[9 + i] += v[i] * bg;
         }
     }
     __shared__ double sw[kNGram][4];
@@ -827,9 +830,10 @@ __global__ void __launch_bounds__(256) k_history_store(
     int64_t nn, const float* vvol, const int32_t* vrank, const float* x,
     const float* hp1, const float* hdp1, const float* hdp2, const uint8_t* hh,
     float* DP, float* DP2, uint8_t* H, int32_t* slot_of_rank, double* part) {
-    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
     double a[kNPart] = {0, 0, 0, 0};
-    if (g < nn) {
+    // grid-stride: kGramBlocks partials for k_scales to sum
+    for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < nn;
+         g += (int64_t)gridDim.x * 256) {
         const bool on = vvol[g] > 0.f;
         const int h = hh[g];
         const float xv = x[g];
@@ -839,12 +843,12 @@ __global__ void __launch_bounds__(256) k_history_store(
         DP2[g] = on && h >= 2 ? hdp1[g] : 0.f;
         if (on) {
             slot_of_rank[vrank[g]] = (int32_t)g;
-            a[0] = (double)xv * (double)xv;
+            a[0] += (double)xv * (double)xv;
             if (h >= 1) {
                 const double r = (double)(dp - prior(h, hdp1[g], hdp2[g]));
-                a[1] = (double)dp * (double)dp;
-                a[2] = r * r;
-                a[3] = 1;
+                a[1] += (double)dp * (double)dp;
+                a[2] += r * r;
+                a[3] += 1;
             }
         }
     }
@@ -1041,7 +1045,7 @@ static int ensure_history(vps_context* c) {
     VPS_TRY(dalloc(c, m->hh, cap));
     VPS_TRY(dalloc(c, m->slot_of_rank, cap));
     VPS_TRY(dalloc(c, m->part, kNPart * (cap / 256 + 16)));
-    VPS_TRY(dalloc(c, m->gpart, kNGram * (cap / 256 + 16)));
+    VPS_TRY(dalloc(c, m->gpart, kNGram * kGramBlocks));
     VPS_TRY(dalloc(c, m->gy, 3));
     return VPS_OK;
 }
@@ -1137,10 +1141,11 @@ int mlp_warm_start(vps_context* c, int mode, float* x0) {
         VPS_TRY(apply_operator_f32(c, E, c->vt2.p, c->p.op_mode));
         VPS_TRY(apply_operator_f32(c, Cc, c->vr.p, c->p.op_mode));
         const int nb = (int)blocks(nn, 256);
-        hipLaunchKernelGGL(k_galerkin_gram, dim3(nb), dim3(256), 0, c->stream, nn, P, E, Cc,
+        const int ng = std::min(nb, kGramBlocks);
+        hipLaunchKernelGGL(k_galerkin_gram, dim3(ng), dim3(256), 0, c->stream, nn, P, E, Cc,
                            c->vt.p, c->vt2.p, c->vr.p, c->vb.p, m->gpart.p);
         hipLaunchKernelGGL(k_galerkin_solve, dim3(1), dim3(1024), 0, c->stream, m->gpart.p,
-                           nb, m->gy.p);
+                           ng, m->gy.p);
         hipLaunchKernelGGL(k_galerkin_combine, dim3(nb), dim3(256), 0, c->stream, nn, P, E,
                            Cc, m->gy.p, x0);
     } else if (mode == VPS_WARM_EXTRAPOLATE) {
@@ -1163,7 +1168,8 @@ int mlp_after_solve(vps_context* c, const vps_stats* st) {
     const int64_t nn = c->nslots * c->brick_nodes;
     if (nn == 0) return VPS_OK;
     const int nb = (int)blocks(nn, 256);
-    hipLaunchKernelGGL(k_history_store, dim3(nb), dim3(256), 0, c->stream, nn,
+    const int nh = std::min(nb, kGramBlocks);
+    hipLaunchKernelGGL(k_history_store, dim3(nh), dim3(256), 0, c->stream, nn,
                        c->vvol.p, c->vrank.p, c->vx[w].p, m->hp1.p, m->hdp1.p,
                        m->hdp2.p, m->hh.p, m->DP[w].p, m->DP2[w].p, m->H[w].p,
                        m->slot_of_rank.p, m->part.p);
@@ -1190,7 +1196,7 @@ int mlp_after_solve(vps_context* c, const vps_stats* st) {
         }
     }
     const int nxt = 1 - m->cur_scale;
-    hipLaunchKernelGGL(k_scales, dim3(1), dim3(1024), 0, c->stream, m->part.p, nb,
+    hipLaunchKernelGGL(k_scales, dim3(1), dim3(1024), 0, c->stream, m->part.p, nh,
                        c->cg.p, c->nvirt, m->scale[nxt].p);
     VPS_HIP(c, hipGetLastError());
     m->cur_scale = nxt;
