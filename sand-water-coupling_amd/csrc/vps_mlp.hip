@@ -446,67 +446,69 @@ __global__ void __launch_bounds__(1024) k_galerkin_solve(const double* part, int
     __syncthreads();
     if (threadIdx.x != 0) return;
     double t[kNGram];
+#pragma unroll
     for (int j = 0; j < kNGram; j++) {
         t[j] = 0;
+#pragma unroll
         for (int i = 0; i < 16; i++) t[j] += sw[j][i];
     }
-    double G[3][3], r[3];
-    int idx[3], m = 0;
-    for (int i = 0; i < 3; i++)
-        if (t[4 * i] > 0 && std::isfinite(t[4 * i])) idx[m++] = i;   // v_i . M v_i
-    for (int p = 0; p < m; p++) {
-        r[p] = t[9 + idx[p]];
-        for (int q = 0; q < m; q++) G[p][q] = t[3 * idx[p] + idx[q]];
+    // a field that is absent (v . M v not positive) gets the row and column
+    // of the identity and a zero right-hand side, so its weight is 0; all
+    // indices below are compile-time constants (registers, no scratch)
+    double G[3][3], r[3], dg[3];
+    bool act[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) act[i] = t[4 * i] > 0 && isfinite(t[4 * i]);
+    const bool any = act[0] || act[1] || act[2];
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        r[p] = act[p] ? t[9 + p] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+            G[p][q] = act[p] && act[q] ? t[3 * p + q] : (p == q ? 1.0 : 0.0);
+        dg[p] = fabs(G[p][p]);
     }
-    // Gaussian elimination with partial pivoting; a pivot below 1e-10 of its
-    // row's diagonal (a field nearly in the span of the others) rejects
-    bool ok = true;
-    int perm[3] = {0, 1, 2};
-    double dg[3];
-    for (int p = 0; p < m; p++) dg[p] = fabs(G[p][p]);
-    for (int k = 0; k < m && ok; k++) {
-        int piv = k;
-        for (int p = k + 1; p < m; p++)
-            if (fabs(G[p][k]) > fabs(G[piv][k])) piv = p;
-        if (piv != k) {
-            for (int q = 0; q < m; q++) {
-                const double tmp = G[k][q];
-                G[k][q] = G[piv][q];
-                G[piv][q] = tmp;
+    // Gaussian elimination with partial pivoting; a pivot not above 1e-10 of
+    // its row's diagonal (a field nearly in the span of the others) rejects
+    bool ok = any;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+#pragma unroll
+        for (int p = k + 1; p < 3; p++) {
+            if (fabs(G[p][k]) > fabs(G[k][k])) {
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    const double tmp = G[k][q];
+                    G[k][q] = G[p][q];
+                    G[p][q] = tmp;
+                }
+                const double tr = r[k];
+                r[k] = r[p];
+                r[p] = tr;
             }
-            const double tr = r[k];
-            r[k] = r[piv];
-            r[piv] = tr;
-            const int tp = perm[k];
-            perm[k] = perm[piv];
-            perm[piv] = tp;
         }
-        if (!(fabs(G[k][k]) > 1e-10 * dg[k])) {
-            ok = false;
-            break;
-        }
-        for (int p = k + 1; p < m; p++) {
-            const double f = G[p][k] / G[k][k];
-            for (int q = k; q < m; q++) G[p][q] -= f * G[k][q];
+        ok = ok && fabs(G[k][k]) > 1e-10 * dg[k];
+        const double piv = ok ? G[k][k] : 1.0;
+#pragma unroll
+        for (int p = k + 1; p < 3; p++) {
+            const double f = G[p][k] / piv;
+#pragma unroll
+            for (int q = 0; q < 3; q++) G[p][q] -= f * G[k][q];
             r[p] -= f * r[k];
         }
     }
-    (void)perm;
-    double out[3] = {1, 1, 1};
-    if (ok && m > 0) {
-        double x[3];
-        for (int p = m - 1; p >= 0; p--) {
-            double sum = r[p];
-            for (int q = p + 1; q < m; q++) sum -= G[p][q] * x[q];
-            x[p] = sum / G[p][p];
-        }
-        bool fin = true;
-        for (int p = 0; p < m; p++) fin = fin && std::isfinite(x[p]);
-        if (fin) {
-            out[0] = out[1] = out[2] = 0;
-            for (int p = 0; p < m; p++) out[idx[p]] = x[p];
-        }
+    double x[3];
+#pragma unroll
+    for (int p = 2; p >= 0; p--) {
+        double sum = r[p];
+#pragma unroll
+        for (int q = p + 1; q < 3; q++) sum -= G[p][q] * x[q];
+        x[p] = sum / (ok ? G[p][p] : 1.0);
     }
+    const bool fin = isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]);
+    double out[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) out[i] = ok && fin ? (act[i] ? x[i] : 0.0) : 1.0;
     for (int i = 0; i < 3; i++) y[i] = out[i];
 }
 
@@ -830,10 +832,9 @@ __global__ void __launch_bounds__(256) k_history_store(
     int64_t nn, const float* vvol, const int32_t* vrank, const float* x,
     const float* hp1, const float* hdp1, const float* hdp2, const uint8_t* hh,
     float* DP, float* DP2, uint8_t* H, int32_t* slot_of_rank, double* part) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
     double a[kNPart] = {0, 0, 0, 0};
-    // grid-stride: kGramBlocks partials for k_scales to sum
-    for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < nn;
-         g += (int64_t)gridDim.x * 256) {
+    if (g < nn) {
         const bool on = vvol[g] > 0.f;
         const int h = hh[g];
         const float xv = x[g];
@@ -843,12 +844,12 @@ __global__ void __launch_bounds__(256) k_history_store(
         DP2[g] = on && h >= 2 ? hdp1[g] : 0.f;
         if (on) {
             slot_of_rank[vrank[g]] = (int32_t)g;
-            a[0] += (double)xv * (double)xv;
+            a[0] = (double)xv * (double)xv;
             if (h >= 1) {
                 const double r = (double)(dp - prior(h, hdp1[g], hdp2[g]));
-                a[1] += (double)dp * (double)dp;
-                a[2] += r * r;
-                a[3] += 1;
+                a[1] = (double)dp * (double)dp;
+                a[2] = r * r;
+                a[3] = 1;
             }
         }
     }
@@ -1168,8 +1169,7 @@ int mlp_after_solve(vps_context* c, const vps_stats* st) {
     const int64_t nn = c->nslots * c->brick_nodes;
     if (nn == 0) return VPS_OK;
     const int nb = (int)blocks(nn, 256);
-    const int nh = std::min(nb, kGramBlocks);
-    hipLaunchKernelGGL(k_history_store, dim3(nh), dim3(256), 0, c->stream, nn,
+    hipLaunchKernelGGL(k_history_store, dim3(nb), dim3(256), 0, c->stream, nn,
                        c->vvol.p, c->vrank.p, c->vx[w].p, m->hp1.p, m->hdp1.p,
                        m->hdp2.p, m->hh.p, m->DP[w].p, m->DP2[w].p, m->H[w].p,
                        m->slot_of_rank.p, m->part.p);
@@ -1196,7 +1196,7 @@ int mlp_after_solve(vps_context* c, const vps_stats* st) {
         }
     }
     const int nxt = 1 - m->cur_scale;
-    hipLaunchKernelGGL(k_scales, dim3(1), dim3(1024), 0, c->stream, m->part.p, nh,
+    hipLaunchKernelGGL(k_scales, dim3(1), dim3(1024), 0, c->stream, m->part.p, nb,
                        c->cg.p, c->nvirt, m->scale[nxt].p);
     VPS_HIP(c, hipGetLastError());
     m->cur_scale = nxt;
