@@ -664,6 +664,10 @@ static int project_impl(vps_context* c, vps_stats* st, int e0) {
     // from the plain start) when its residual is not finite or larger than
     // ||b||, or when the iteration from it overflows
     c->start_guard = mode == VPS_WARM_LEARNED || mode == VPS_WARM_EXTRAPOLATE;
+    // cg_solve queues the fold, after_solve and the gradient behind its
+    // batches (guarded on the device) and records ev[e0 + 2] before the
+    // gradient; spec_done says the accepted batch's copy stands
+    c->spec_post = e0 == 4;
     int rc = cg_solve(c, s);
     if (rc == kStartRejected) {
         // the plain start: the reference's index-carried pressures
@@ -674,15 +678,24 @@ static int project_impl(vps_context* c, vps_stats* st, int e0) {
         s->reserved[2] = 1;
     }
     c->start_guard = false;
-    if (rc != VPS_OK && rc != VPS_ERR_NO_CONVERGENCE) return rc;
+    c->spec_post = false;
+    if (rc != VPS_OK && rc != VPS_ERR_NO_CONVERGENCE) {
+        c->spec_done = false;
+        return rc;
+    }
     // The network's history and training read this solve's pressures in the
     // slot map of this solve (index `which`), which after_solve keeps; they
     // touch neither the particles nor V p, so they are queued behind the
     // gradient, which then starts without waiting for the host to launch
     // the short history kernels.
-    VPS_TRY(after_solve(c));
-    VPS_HIP(c, hipEventRecord(c->ev[e0 + 2], c->stream));
-    if (rc == VPS_OK) VPS_TRY(pressure_gradient(c));
+    if (c->spec_done) {
+        after_solve_commit(c);   // its kernel and the gradient are queued
+        c->spec_done = false;
+    } else {
+        VPS_TRY(after_solve(c));
+        VPS_HIP(c, hipEventRecord(c->ev[e0 + 2], c->stream));
+        if (rc == VPS_OK) VPS_TRY(pressure_gradient(c));
+    }
     if (mlp_active(c) && rc == VPS_OK) VPS_TRY(mlp_after_solve(c, s));
     VPS_HIP(c, hipEventRecord(c->ev[e0 + 3], c->stream));
     return rc;

@@ -156,10 +156,10 @@ template <int D>
 __global__ void __launch_bounds__(256) k_gradient(
     KConst k, int64_t n, const double4* rpos, float4* rvel,
     const int32_t* slot_of_brick, const float* vq, const double4* bpos,
-    const float4* bvel, const float4* bnrm, const int32_t* bcstart) {
+    const float4* bvel, const float4* bnrm, const int32_t* bcstart, SolveGuard guard) {
     const int nblk = (int)((n + 255) / 256);
     const int64_t i = (int64_t)xcd_remap(blockIdx.x, nblk) * 256 + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || !solve_accepted(guard)) return;
     const double4 q = rpos[i];
     float4 vi = rvel[i];
     float fx = 0.f, fy = 0.f, fz = 0.f;
@@ -212,7 +212,9 @@ __device__ __forceinline__ int node_floor(double x, double o, double inv) {
 __global__ void __launch_bounds__(kGradThreads) k_gradient_overflow(
     KConst k, const int4* chunks, const int32_t* over, const int32_t* nover,
     const double4* rpos, float4* rvel, const int32_t* slot_of_brick, const float* vq,
-    const double4* bpos, const float4* bvel, const float4* bnrm, const int32_t* bcstart) {
+    const double4* bpos, const float4* bvel, const float4* bnrm, const int32_t* bcstart,
+    SolveGuard guard) {
+    if (!solve_accepted(guard)) return;
     const int no = *nover;
     for (int o = blockIdx.x; o < no; o += gridDim.x) {
         const int4 ch = chunks[over[o]];
@@ -239,7 +241,8 @@ __global__ void __launch_bounds__(kGradThreads, 4) k_gradient3s(
     KConst k, const int4* chunks, int nchunks, const double4* rpos, float4* rvel,
     const int32_t* slot_of_brick, const float* vq, const double4* bpos,
     const float4* bvel, const float4* bnrm, const int32_t* bcstart, int32_t* over,
-    int32_t* nover, int32_t* members = nullptr) {
+    int32_t* nover, SolveGuard guard, int32_t* members = nullptr) {
+    if (!solve_accepted(guard)) return;   // uniform over the block
     __shared__ float s_q[kGradCap];
     __shared__ int s_box[6];
     __shared__ int s_walls;   // boundary particles in the cells around the span
@@ -528,12 +531,12 @@ int gradient_member_counts(vps_context* c, int32_t* members) {
                            (int)c->grad_nchunks, c->rpos[c->cur].p, c->rvel[c->cur].p,
                            c->brick_slot[c->which].p, c->vvol.p, c->bpos.p, c->bvel.p,
                            c->bnrm.p, c->bcell_start.p, (int32_t*)nullptr,
-                           (int32_t*)nullptr, members);
+                           (int32_t*)nullptr, SolveGuard{}, members);
     VPS_HIP(c, hipGetLastError());
     return VPS_OK;
 }
 
-int pressure_gradient(vps_context* c) {
+int pressure_gradient(vps_context* c, SolveGuard g) {
     const int64_t n = c->n;
     const int s = c->cur;
     const int w = c->which;
@@ -545,22 +548,22 @@ int pressure_gradient(vps_context* c) {
                                dim3(kGradThreads), 0, c->stream, c->k,
                                c->grad_chunks.p, (int)c->grad_nchunks, c->rpos[s].p,
                                c->rvel[s].p, c->brick_slot[w].p, c->vq.p, c->bpos.p,
-                               c->bvel.p, c->bnrm.p, c->bcell_start.p, over, nover);
+                               c->bvel.p, c->bnrm.p, c->bcell_start.p, over, nover, g);
             hipLaunchKernelGGL(k_gradient_overflow, dim3(64), dim3(kGradThreads), 0,
                                c->stream, c->k, c->grad_chunks.p, over, nover,
                                c->rpos[s].p, c->rvel[s].p, c->brick_slot[w].p, c->vq.p,
-                               c->bpos.p, c->bvel.p, c->bnrm.p, c->bcell_start.p);
+                               c->bpos.p, c->bvel.p, c->bnrm.p, c->bcell_start.p, g);
         }
     } else if (c->dim == 3)
         hipLaunchKernelGGL(k_gradient<3>, dim3(blocks(n, 256)), dim3(256), 0,
                            c->stream, c->k, n, c->rpos[s].p, c->rvel[s].p,
                            c->brick_slot[w].p, c->vq.p, c->bpos.p, c->bvel.p,
-                           c->bnrm.p, c->bcell_start.p);
+                           c->bnrm.p, c->bcell_start.p, g);
     else
         hipLaunchKernelGGL(k_gradient<2>, dim3(blocks(n, 256)), dim3(256), 0,
                            c->stream, c->k, n, c->rpos[s].p, c->rvel[s].p,
                            c->brick_slot[w].p, c->vq.p, c->bpos.p, c->bvel.p,
-                           c->bnrm.p, c->bcell_start.p);
+                           c->bnrm.p, c->bcell_start.p, g);
     VPS_HIP(c, hipGetLastError());
     return VPS_OK;
 }

@@ -377,6 +377,20 @@ struct CgState {
     int stop;
 };
 constexpr int kStopBefore = 0x7fffffff;
+
+// Work after a solve that is queued before the host has read the CG state:
+// it runs only if the solve ended accepted.  mode 0: always (the host has
+// decided); 1: converged (done == 1); 2: converged with the FP64 true
+// residual below the threshold (the host's test when the check is on).
+struct SolveGuard {
+    const CgState* st = nullptr;
+    int mode = 0;
+};
+__device__ __forceinline__ bool solve_accepted(const SolveGuard& g) {
+    if (g.mode == 0) return true;
+    const CgState* s = g.st;
+    return s->done == 1 && (g.mode == 1 || s->true_rr < s->thr);
+}
 // cg_solve's status when a guarded warm start (learned or extrapolated) is
 // rejected: its initial residual is not finite or larger than ||b|| (worse
 // than a zero start), or the iteration from it overflowed; the caller
@@ -475,6 +489,10 @@ struct vps_context {
     double cg_drift = 1.0;
     int64_t start_fallbacks = 0;   // solves restarted from the plain start
     bool start_guard = false;      // cg_solve: the start may be rejected
+    // cg_solve queues the post-solve work (fold, after_solve, gradient)
+    // behind each batch, guarded, before reading the state back; spec_done:
+    // the accepted batch's queued work stands (the caller skips its own)
+    bool spec_post = false, spec_done = false;
 
     // scratch
     vps::DBuf<double> partials;
@@ -554,7 +572,7 @@ int64_t partials_size(int64_t nbricks);
 int real_offsets(vps_context* c);
 int apply_operator_f32(vps_context* c, const float* in, float* out,
                        int op_mode);
-int pressure_gradient(vps_context* c);
+int pressure_gradient(vps_context* c, SolveGuard g = SolveGuard{});
 // test support: terms per particle (sorted order) of the FP32 walks
 int density_member_counts(vps_context* c, int which, int32_t* members);
 int gradient_member_counts(vps_context* c, int32_t* members);
@@ -567,7 +585,10 @@ int reduce_max_f32(vps_context* c, const float* in, int64_t n, double* out);
 int gather_virtual(vps_context* c, int64_t* node, double* rho, double* vol,
                    double* div, double* diag, double* pressure);
 int warm_start(vps_context* c, int mode, float* x0);
-int after_solve(vps_context* c);
+// after_solve: the kernel (guarded) and, with an unguarded call, the host
+// bookkeeping; after_solve_commit: the bookkeeping alone
+int after_solve(vps_context* c, SolveGuard g = SolveGuard{});
+void after_solve_commit(vps_context* c);
 
 // learned warm start
 int mlp_create(vps_context* c);

@@ -1065,7 +1065,8 @@ __global__ void __launch_bounds__(kUpdThreads) k_copy_rr(int64_t n4, const float
 //   kFoldZero   x = 0, xb = 0              (b = 0: Eigen's x = 0)
 enum FoldMode { kFoldSplit = 0, kFoldMerge = 1, kFoldFinal = 2, kFoldZero = 3 };
 __global__ void __launch_bounds__(kUpdThreads) k_fold_x(int64_t n4, float4* x, double4* xb,
-                                                        int mode) {
+                                                        int mode, SolveGuard guard) {
+    if (!solve_accepted(guard)) return;
     for (int64_t i = (int64_t)blockIdx.x * kUpdThreads + threadIdx.x; i < n4;
          i += (int64_t)kUpdBlocks * kUpdThreads) {
         const float4 d = x[i];
@@ -1460,9 +1461,22 @@ int cg_solve(vps_context* c, vps_stats* stats) {
     // x = xb (FP64) + x (FP32 correction) during the solve; every return
     // after the split leaves the pressures in x
     double* xb = c->vxb.p;
-    auto fold = [&](int mode) {
+    auto fold = [&](int mode, SolveGuard g = SolveGuard{}) {
         hipLaunchKernelGGL(k_fold_x, dim3(kUpdBlocks), dim3(kUpdThreads), 0, c->stream, n4,
-                           (float4*)x, (double4*)xb, mode);
+                           (float4*)x, (double4*)xb, mode, g);
+    };
+    // the post-solve work behind a batch, run only if the batch ends the
+    // solve accepted (the state is read back before it, so the host learns
+    // the outcome while the gradient runs)
+    const bool spec = c->spec_post;
+    c->spec_done = false;
+    auto queue_post = [&]() -> int {
+        const SolveGuard g{c->cg.p, c->check_true ? 2 : 1};
+        fold(kFoldFinal, g);
+        VPS_TRY(after_solve(c, g));
+        VPS_HIP(c, hipEventRecord(c->ev[6], c->stream));
+        VPS_TRY(pressure_gradient(c, g));
+        return VPS_OK;
     };
     auto finish = [&](int rc) {
         fold(kFoldFinal);
@@ -1527,6 +1541,7 @@ int cg_solve(vps_context* c, vps_stats* stats) {
     double* setB[4] = {part_iter(c, 0), part_iter(c, 1), part_iter(c, 2), part_iter(c, 3)};
     std::vector<int> timed_iter;
     double host_launch_us = 0;
+    bool post_queued = false;   // the guarded post-solve work follows this state
     while (true) {
         if (hs.done == 0) {
             int batch = iter_base + kk < predicted ? predicted - (iter_base + kk) : 4;
@@ -1662,7 +1677,15 @@ int cg_solve(vps_context* c, vps_stats* stats) {
             VPS_HIP(c, hipGetLastError());
             host_launch_us += std::chrono::duration<double, std::micro>(
                                   std::chrono::steady_clock::now() - h0).count();
-            VPS_TRY(read_state(c, &hs));
+            // the state read-back, then (spec) the guarded post-solve work
+            // behind it; the host waits for the read-back only
+            VPS_HIP(c, hipMemcpyAsync(c->host_cg, c->cg.p, sizeof(CgState),
+                                      hipMemcpyDeviceToHost, c->stream));
+            VPS_HIP(c, hipEventRecord(c->ev[8], c->stream));
+            if (spec) VPS_TRY(queue_post());
+            VPS_HIP(c, hipEventSynchronize(c->ev[8]));
+            hs = *c->host_cg;
+            post_queued = spec;
             if (!start_checked) {
                 start_checked = true;
                 if (hs.done == 3) {
@@ -1709,11 +1732,17 @@ int cg_solve(vps_context* c, vps_stats* stats) {
         if (hs.done == 2) break;
         // converged on the recursive residual; the true residual of this
         // state was computed right after the batch
-        if (!c->check_true) break;
+        if (!c->check_true) {
+            c->spec_done = post_queued;
+            break;
+        }
         if (three && replacements == 0 && hs.rr > 0 && hs.true_rr >= 0 &&
             std::isfinite(hs.true_rr / hs.rr))
             c->cg_drift = hs.true_rr / hs.rr;
-        if (hs.true_rr < hs.thr) break;
+        if (hs.true_rr < hs.thr) {
+            c->spec_done = post_queued;
+            break;
+        }
         if (replacements >= kMaxReplacements) {
             c->err = "CG: the true residual stayed above the tolerance after " +
                      std::to_string(kMaxReplacements) + " residual replacements";
@@ -1766,9 +1795,10 @@ int cg_solve(vps_context* c, vps_stats* stats) {
                            part_repl(c), c->cg.p);
         VPS_HIP(c, hipGetLastError());
         VPS_TRY(read_state(c, &hs));
+        post_queued = false;
         predicted = iter_base + 4;
     }
-    fold(kFoldFinal);
+    if (!c->spec_done) fold(kFoldFinal);
     VPS_HIP(c, hipGetLastError());
     c->prev_iters = hs.iter;
     double spmv_ms = 0, upd_ms = 0;

@@ -742,9 +742,10 @@ __global__ void __launch_bounds__(256) k_after_solve(int64_t nnode,
                                                      const float* vvol,
                                                      const float* x,
                                                      const int32_t* rank,
-                                                     float* p_rank, float* q) {
+                                                     float* p_rank, float* q,
+                                                     SolveGuard guard) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= nnode) return;
+    if (i >= nnode || !solve_accepted(guard)) return;
     const float V = vvol[i];
     const float p = x[i];
     q[i] = V * p;
@@ -889,17 +890,22 @@ int warm_start(vps_context* c, int mode, float* x0) {
     return VPS_OK;
 }
 
-int after_solve(vps_context* c) {
+int after_solve(vps_context* c, SolveGuard g) {
     const int w = c->which;
     const int64_t nn = c->nslots * c->brick_nodes;
     if (nn == 0) return VPS_OK;
     hipLaunchKernelGGL(k_after_solve, dim3(blocks(nn, 256)), dim3(256), 0,
                        c->stream, nn, c->vvol.p, c->vx[w].p, c->vrank.p,
-                       c->p_by_rank[w].p, c->vq.p);
+                       c->p_by_rank[w].p, c->vq.p, g);
     VPS_HIP(c, hipGetLastError());
-    c->last_solve_buf = w;
-    c->last_solve_nvirt = c->nvirt;
+    if (g.mode == 0) after_solve_commit(c);
     return VPS_OK;
+}
+
+void after_solve_commit(vps_context* c) {
+    if (c->nslots * c->brick_nodes == 0) return;
+    c->last_solve_buf = c->which;
+    c->last_solve_nvirt = c->nvirt;
 }
 
 }  // namespace vps
