@@ -185,6 +185,7 @@ __global__ void __launch_bounds__(256) k_gradient(
 // global per-particle path.
 constexpr int kGradThreads = 256;
 constexpr int kGradCap = 7168;   // staged nodes (28 KB: five blocks per CU)
+constexpr int kGradBoxBricks = 128;   // bricks a staged box may touch
 
 __global__ void __launch_bounds__(256) k_spans(KConst k, int span_cells, int nspan,
                                                const int32_t* cstart, int4* chunks,
@@ -244,6 +245,7 @@ __global__ void __launch_bounds__(kGradThreads, 4) k_gradient3s(
     int32_t* nover, SolveGuard guard, int32_t* members = nullptr) {
     if (!solve_accepted(guard)) return;   // uniform over the block
     __shared__ float s_q[kGradCap];
+    __shared__ int s_bslot[kGradBoxBricks];   // slots of the box's bricks
     __shared__ int s_box[6];
     __shared__ int s_walls;   // boundary particles in the cells around the span
     const int chunk = xcd_remap(blockIdx.x, nchunks);
@@ -306,25 +308,60 @@ __global__ void __launch_bounds__(kGradThreads, 4) k_gradient3s(
         if (!CNT && t == 0) over[atomicAdd(nover, 1)] = chunk;
         return;
     }
+    // the bricks the box touches (floor division by 8, also below zero)
+    const int bx0 = lo0 >> 3, by0 = lo1 >> 3, bz0 = lo2 >> 3;
+    const int nbx = ((lo0 + (int)e0 - 1) >> 3) - bx0 + 1;
+    const int nby = ((lo1 + (int)e1 - 1) >> 3) - by0 + 1;
+    const int nbz = ((lo2 + (int)e2 - 1) >> 3) - bz0 + 1;
+    if (nbx * nby * nbz > kGradBoxBricks) {   // (a box of at most 108 bricks fits)
+        if (!CNT && t == 0) over[atomicAdd(nover, 1)] = chunk;
+        return;
+    }
+    // their slots (-1 off the virtual set or off the lattice) in one round
+    for (int i = t; i < nbx * nby * nbz; i += kGradThreads) {
+        const int bx = bx0 + i % nbx, by = by0 + (i / nbx) % nby, bz = bz0 + i / (nbx * nby);
+        int sl = -1;
+        if (bx >= 0 && by >= 0 && bz >= 0 && bx < k.bdims[0] && by < k.bdims[1] &&
+            bz < k.bdims[2])
+            sl = slot_of_brick[((int64_t)bz * k.bdims[1] + by) * k.bdims[0] + bx];
+        s_bslot[i] = sl;
+    }
+    __syncthreads();
     {
-        // one wave per box row (iy, iz), lanes along x
+        // one wave per box row (iy, iz), lanes along x; four rows per pass,
+        // their loads issued together (no branch between them: an invalid
+        // node reads q[0] and keeps 0)
+        constexpr int kW = kGradThreads / 64;
         const int n0 = (int)e0, n1 = (int)e1, nrow = (int)(e1 * e2);
         const int lane = t & 63;
-        for (int rw = t >> 6; rw < nrow; rw += kGradThreads / 64) {
-            const int iy = rw % n1, iz = rw / n1;
-            const int gy = lo1 + iy, gz = lo2 + iz;
-            const bool yz_in = gy >= 0 && gz >= 0 && gy < k.ndims[1] && gz < k.ndims[2];
-            const int64_t brow = ((int64_t)(gz >> 3) * k.bdims[1] + (gy >> 3)) * k.bdims[0];
-            const int lyz = ((gz & 7) << 6) + ((gy & 7) << 3);
-            float* dst = s_q + rw * n0;
+        for (int rw0 = t >> 6; rw0 < nrow; rw0 += 4 * kW) {
             for (int ix = lane; ix < n0; ix += 64) {
                 const int gx = lo0 + ix;
-                float v = 0.f;
-                if (yz_in && gx >= 0 && gx < k.ndims[0]) {
-                    const int sl = slot_of_brick[brow + (gx >> 3)];
-                    if (sl >= 0) v = vq[(int64_t)sl * 512 + lyz + (gx & 7)];
+                const bool x_in = gx >= 0 && gx < k.ndims[0];
+                const int bxi = (gx >> 3) - bx0;
+                float v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int rw = rw0 + u * kW;
+                    const int iy = rw % n1, iz = rw / n1;
+                    const int gy = lo1 + iy, gz = lo2 + iz;
+                    const bool in = x_in && rw < nrow && gy >= 0 && gz >= 0 &&
+                                    gy < k.ndims[1] && gz < k.ndims[2];
+                    const int bi = in ? (((gz >> 3) - bz0) * nby + ((gy >> 3) - by0)) * nbx + bxi
+                                      : 0;
+                    const int sl = s_bslot[bi];
+                    const bool ok = in && sl >= 0;
+                    const int64_t g = ok ? (int64_t)sl * 512 + ((gz & 7) << 6) +
+                                               ((gy & 7) << 3) + (gx & 7)
+                                         : 0;
+                    const float q = vq[g];
+                    v[u] = ok ? q : 0.f;
                 }
-                dst[ix] = v;
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int rw = rw0 + u * kW;
+                    if (rw < nrow) s_q[rw * n0 + ix] = v[u];
+                }
             }
         }
     }
