@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(256) k_density(const double4* pos,
 // V = m / rho into vel.w when set_volume.
 // Off: the type of the candidates' byte offsets (uint32_t when the set's
 // offsets fit, so the loads take a scalar base and a 32-bit lane offset).
-// CNT (test support): only the number of terms that enter the sum (x < 1,
+// CNT (test support): only the number of terms that enter the sum (a > 0: r < h,
 // where the spline is nonzero) into members[i].
 template <bool BND, typename Off, bool CNT = false>
 __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const float4* ofs,
@@ -172,11 +172,13 @@ __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const flo
     const bool qfar = !(fabsf(qo0) <= far && fabsf(qo1) <= far && fabsf(qo2) <= far);
     // W / c0 = 2 max(1 - x, 0)^3 - max(1 - 2x, 0)^3: the two pieces of the
     // cubic spline in one expression (for x < 1/2 it expands to
-    // 6 x^2 (x - 1) + 1), zero from x = 1 on, so no support test is needed;
-    // c0 m is applied once to the sum.  One loop per row of three cells (the
-    // lanes of a wave sit in different cells: per-cell loops would run the
-    // longest cell of every lane in turn).
-    float sum = 0.f;
+    // 6 x^2 (x - 1) + 1), zero from x = 1 on, so no support test is needed.
+    // The two cubes are summed apart (two multiplies and two fmas per
+    // candidate) and combined, with c0 m, once per particle.  One loop per
+    // row of three cells (the lanes of a wave sit in different cells:
+    // per-cell loops would run the longest cell of every lane in turn).
+    const float ih = k.inv_h, ih2 = 2.f * k.inv_h;
+    float suma = 0.f, sumc = 0.f;
     int32_t nterms = 0;
     for (int z = zlo; z <= zhi; z++) {
         const float tz = (float)(q.z - cell_corner(k, 2, z));
@@ -195,13 +197,14 @@ __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const flo
                     d2 = (float)sq_dist3(pj.x, pj.y, pj.z, q.x, q.y, q.z);
                 }
                 // v_sqrt_f32 (1 ulp): W is rounded to FP32 anyway
-                const float x = __builtin_amdgcn_sqrtf(d2) * k.inv_h;
+                const float r = __builtin_amdgcn_sqrtf(d2);
                 // max(., 0) of values <= 1 as the clamp to [0, 1], which
-                // folds into the producing add and fma
-                const float a = __builtin_amdgcn_fmed3f(1.f - x, 0.f, 1.f);
-                const float c = __builtin_amdgcn_fmed3f(fmaf(-2.f, x, 1.f), 0.f, 1.f);
-                sum += fmaf(2.f * a, a * a, -(c * c * c));
-                if constexpr (CNT) nterms += x < 1.f;
+                // folds into the fma
+                const float a = __builtin_amdgcn_fmed3f(fmaf(-ih, r, 1.f), 0.f, 1.f);
+                const float c = __builtin_amdgcn_fmed3f(fmaf(-ih2, r, 1.f), 0.f, 1.f);
+                suma = fmaf(a * a, a, suma);
+                sumc = fmaf(c * c, c, sumc);
+                if constexpr (CNT) nterms += a > 0.f;
             }
         }
     }
@@ -209,7 +212,7 @@ __global__ void __launch_bounds__(256) k_density3f(const double4* pos, const flo
         members[i] = nterms;
         return;
     }
-    const float r = k.mass * k.c0h * sum;
+    const float r = k.mass * k.c0h * fmaf(2.f, suma, -sumc);
     if constexpr (BND) {
         nrm[i].w = r;
         vel[i].w = k.mass / r;

@@ -369,10 +369,13 @@ __global__ void __launch_bounds__(kGradThreads, 4) k_gradient3s(
     // spline of H (S = -W'(r) / r, q = r / H): with a = max(1 - q, 0) and
     // b = max(1 - 2 q, 0), S = c1 (a^2 - b^2) / r on every piece:
     // c1 (2 - 3 q) / H for q < 1/2, c1 (1 - q)^2 / r up to q = 1, and 0
-    // beyond (a = b = 0).  1 - q <= 1, so the clamp to [0, 1] is the max;
-    // it folds into the fma.  c1 is applied once per particle.
-    const float c1 = k.c1H, iH = k.inv_H, iH2 = 2.f * k.inv_H;
-    const float H2 = (float)k.H2;
+    // beyond (a = b = 0).  The offsets d are taken in units of H, so that
+    // with w = |d|^2 = q^2 and u = 1 / q (one rsq), a = 1 - w u and
+    // b = 2 a - 1 (both clamped to [0, 1]: a = 0 beyond q = 1 gives b = 0),
+    // and S d = c1 (a^2 - b^2) u (d / H): 9 VALU and the rsq per candidate.
+    // The clamps fold into the fmas; c1 is applied once per particle.
+    const float c1 = k.c1H;
+    const double iHd = 1.0 / k.H;
     for (int p0 = ch.x; p0 < ch.y; p0 += kGradThreads) {
         const int p = p0 + t;
         if (p >= ch.y) break;
@@ -384,18 +387,18 @@ __global__ void __launch_bounds__(kGradThreads, 4) k_gradient3s(
             const int f0 = node_floor(q.x, k.origin[0], inv);
             const int f1 = node_floor(q.y, k.origin[1], inv);
             const int f2 = node_floor(q.z, k.origin[2], inv);
-            float dxf[8], dx2[8];
+            float dxf[8], dx2[8];   // x offsets of the 8 candidates, in units of H
 #pragma unroll
             for (int c = 0; c < 8; c++) {
-                dxf[c] = (float)dsub(node_coord(k.origin[0], f0 - 3 + c, k.spacing), q.x);
+                dxf[c] = (float)(dsub(node_coord(k.origin[0], f0 - 3 + c, k.spacing), q.x) * iHd);
                 // + 1e-30: rsq stays finite at a coincident node (whose
                 // direction factors are all zero)
                 dx2[c] = dxf[c] * dxf[c] + 1e-30f;
             }
-            // y and z offsets of candidate 0; candidate c adds c s (FP32)
-            const float dy0 = (float)dsub(node_coord(k.origin[1], f1 - 3, k.spacing), q.y);
-            const float dz0 = (float)dsub(node_coord(k.origin[2], f2 - 3, k.spacing), q.z);
-            const float sf = (float)k.spacing;
+            // y and z offsets of candidate 0; candidate c adds c s (FP32, units of H)
+            const float dy0 = (float)(dsub(node_coord(k.origin[1], f1 - 3, k.spacing), q.y) * iHd);
+            const float dz0 = (float)(dsub(node_coord(k.origin[2], f2 - 3, k.spacing), q.z) * iHd);
+            const float sf = (float)(k.spacing * iHd);
             const int n0 = (int)e0, n1 = (int)e1;
             const float* base = s_q + ((f2 - 3 - lo2) * n1 + (f1 - 3 - lo1)) * n0 + (f0 - 3 - lo0);
             // Candidate c of row (cy, cz) is node f - 3 + c: its offset from x
@@ -417,18 +420,17 @@ __global__ void __launch_bounds__(kGradThreads, 4) k_gradient3s(
                     const int cut = R2 >= 11 ? 0 : (R2 >= 6 ? 1 : 2);
                     const float dyc = fmaf((float)cy, sf, dy0);
                     const float dyz = fmaf(dyc, dyc, dz2c);
-                    if (!(dyz < H2)) continue;   // this particle: none within H
+                    if (!(dyz < 1.f)) continue;   // this particle: none within H
                     const float* row = plane + cy * n0;
                     float sx = 0.f, ss = 0.f;
 #pragma unroll
                     for (int c = 0; c < 8; c++) {
                         if (c < cut || c > 7 - cut) continue;   // uniform
-                        const float d2 = dx2[c] + dyz;
-                        const float rinv = __builtin_amdgcn_rsqf(d2);
-                        const float r = d2 * rinv;
-                        const float a = __builtin_amdgcn_fmed3f(fmaf(-iH, r, 1.f), 0.f, 1.f);
-                        const float b = __builtin_amdgcn_fmed3f(fmaf(-iH2, r, 1.f), 0.f, 1.f);
-                        const float sc = row[c] * rinv * fmaf(a, a, -(b * b));
+                        const float w = dx2[c] + dyz;
+                        const float u = __builtin_amdgcn_rsqf(w);
+                        const float a = __builtin_amdgcn_fmed3f(fmaf(-w, u, 1.f), 0.f, 1.f);
+                        const float b = __builtin_amdgcn_fmed3f(fmaf(a, 2.f, -1.f), 0.f, 1.f);
+                        const float sc = row[c] * u * fmaf(a, a, -(b * b));
                         sx = fmaf(sc, dxf[c], sx);
                         ss += sc;
                         if constexpr (CNT) nterms += (a > 0.f) & (row[c] > 0.f);

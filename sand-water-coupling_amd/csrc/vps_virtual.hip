@@ -81,6 +81,9 @@ __global__ void __launch_bounds__(256) k_assign_slots(uint32_t* flag,
 
 // Real-neighbour sums of one node: w, sum W_H, sum V^2 W, sum V W (the v
 // denominator), sum v W / rho and the divergence terms.  dx = q - x_j.
+// Every sum is linear in its spline's constant (c0 of W, c1 of S = -W'/r),
+// so the terms are summed without it and finish_node applies c0h, c0H and
+// c1h once per node (one multiply fewer per term); the boundary sums alike.
 struct RealAcc {
     float w = 0.f, sH = 0.f, sV = 0.f, den = 0.f;
     float nvx = 0.f, nvy = 0.f, nvz = 0.f;   // sum v_j W / rho_j
@@ -90,8 +93,8 @@ struct RealAcc {
     __device__ __forceinline__ void add(const KConst& k, double d2, float dx,
                                         float dy, float dz, float4 vj,
                                         float inv_rhoj) {
-        const KW kh = kernel_f32(d2, k.h2, k.hf, k.inv_h, k.c0h, k.c1h);
-        const KW kH = kernel_f32(d2, k.H2, k.Hf, k.inv_H, k.c0H, k.c1H);
+        const KW kh = kernel_f32(d2, k.h2, k.hf, k.inv_h, 1.f, 1.f);
+        const KW kH = kernel_f32(d2, k.H2, k.Hf, k.inv_H, 1.f, 1.f);
         const float Vj = vj.w;
         const float wr = kh.w * inv_rhoj;
         w += kh.w;
@@ -120,12 +123,12 @@ struct RealAcc {
         const float rinv = __builtin_amdgcn_rsqf(d2f);
         const float r = d2f * rinv;
         const SplineAB sp = spline_ab(r, k.inv_h);
-        const float wh = spline_w_ab(sp, k.c0h);
-        const float sh = spline_s_ab(sp, rinv, k.c1h);
+        const float wh = spline_w_ab(sp, 1.f);
+        const float sh = spline_s_ab(sp, rinv, 1.f);
         const float Vj = vj.w;
         const float wr = wh * inv_rhoj;
         w += wh;
-        sH += spline_w_ab(spline_ab(r, k.inv_H), k.c0H);
+        sH += spline_w_ab(spline_ab(r, k.inv_H), 1.f);
         sV += Vj * Vj * wh;
         den += Vj * wh;
         nvx += vj.x * wr;
@@ -149,7 +152,7 @@ struct BAcc {
     __device__ __forceinline__ void add_exact(const KConst& k, double qx, double qy,
                                               double qz, const double4& pj, double d2,
                                               const float4& vj, float rho_b) {
-        const KW kh = kernel_f32(d2, k.h2, k.hf, k.inv_h, k.c0h, k.c1h);
+        const KW kh = kernel_f32(d2, k.h2, k.hf, k.inv_h, 1.f, 1.f);
         const float Vj = vj.w;
         const float wr = kh.w / rho_b;
         den += Vj * kh.w;
@@ -173,8 +176,8 @@ struct BAcc {
                                              float dz, const float4& vj, float inv_rho_b) {
         const float rinv = __builtin_amdgcn_rsqf(d2f);
         const SplineAB sp = spline_ab(d2f * rinv, k.inv_h);
-        const float wh = spline_w_ab(sp, k.c0h);
-        const float sh = spline_s_ab(sp, rinv, k.c1h);
+        const float wh = spline_w_ab(sp, 1.f);
+        const float sh = spline_s_ab(sp, rinv, 1.f);
         const float Vj = vj.w;
         const float wr = wh * inv_rho_b;
         den += Vj * wh;
@@ -213,8 +216,10 @@ __device__ __forceinline__ void finish_node(
     int cx, int cy, int cz, int64_t out, const double4* rpos,
     const int32_t* rcstart, const BW& bwalk, const float4* bvel,
     const float4* bnrm, float* vrho, float* vvol, float* vb, uint32_t* bits) {
-    bool virt = a.w > 1e-6f;
-    if (fabsf(a.w - 1e-6f) <= 1e-10f) {
+    // the sums carry no spline constants (see RealAcc)
+    const float aw = a.w * k.c0h;
+    bool virt = aw > 1e-6f;
+    if (fabsf(aw - 1e-6f) <= 1e-10f) {
         // exact FP64 decision near the threshold (VirtualParticle.cpp:56)
         double w64 = 0;
         for_each_candidate<D>(k, rcstart, rpos, qx, qy, qz, [&](int j, const double4& pj) {
@@ -235,17 +240,18 @@ __device__ __forceinline__ void finish_node(
     b.den = a.den;
     bwalk(k, qx, qy, qz, bvel, bnrm, b);
     // v_I = (m sum v_j W/rho_j + m_b sum v_b W/rho_b) / (sum V W + sum V_b W + 1e-6)
-    const float inv_den = 1.f / (b.den + 1e-6f);
-    const float ux = (k.mass * a.nvx + k.mass * b.bnx) * inv_den;
-    const float uy = (k.mass * a.nvy + k.mass * b.bny) * inv_den;
-    const float uz = (k.mass * a.nvz + k.mass * b.bnz) * inv_den;
-    const float rho = k.mass * a.sH;
+    const float inv_den = 1.f / fmaf(b.den, k.c0h, 1e-6f);
+    const float mc = k.mass * k.c0h;
+    const float ux = (mc * a.nvx + mc * b.bnx) * inv_den;
+    const float uy = (mc * a.nvy + mc * b.bny) * inv_den;
+    const float uz = (mc * a.nvz + mc * b.bnz) * inv_den;
+    const float rho = k.mass * k.c0H * a.sH;
     const float drho = rho - k.rho0;
-    const float D_I = a.dr - (ux * a.gx + uy * a.gy + uz * a.gz) + b.db -
-                      (ux * b.hx + uy * b.hy + uz * b.hz) -
+    const float D_I = k.c1h * (a.dr - (ux * a.gx + uy * a.gy + uz * a.gz) + b.db -
+                               (ux * b.hx + uy * b.hy + uz * b.hz)) -
                       k.kappa * (drho > 0.f ? drho : 0.f) / k.rho0;
     vrho[out] = rho;
-    vvol[out] = a.sV;
+    vvol[out] = a.sV * k.c0h;
     vb[out] = D_I;
     const int64_t lin = ((int64_t)cz * k.ndims[1] + cy) * k.ndims[0] + cx;
     atomicOr(&bits[lin >> 5], 1u << (lin & 31));
