@@ -406,8 +406,10 @@ __global__ void __launch_bounds__(kGradThreads, 4) k_gradient3s(
             // index c (3, 2, 1, 0, 0, 1, 2, 3) a row holds members only if
             // d(cy)^2 + d(cz)^2 < 16, and only at |c - 3.5| < sqrt(R2) + 0.5,
             // R2 = 16 - d(cy)^2 - d(cz)^2: all 8 for R2 >= 11, c = 1..6 for
-            // R2 = 6..8, c = 2..5 for R2 = 3 (the uniform bounds below).
-#pragma unroll 1
+            // R2 = 6..8, c = 2..5 for R2 = 3 (the uniform bounds below).  The
+            // whole 8 x 8 row nest is unrolled: the bounds are constants, and
+            // the LDS reads are scheduled ahead of the candidates that use them.
+#pragma unroll
             for (int cz = 0; cz < 8; cz++) {
                 const float dzfc = fmaf((float)cz, sf, dz0), dz2c = dzfc * dzfc;
                 const int dzm = cz <= 3 ? 3 - cz : cz - 4;
