@@ -34,21 +34,23 @@ __device__ __forceinline__ void boundary_term(const KConst& k, const double4& q,
         const double d2 = D == 3 ? sq_dist3(pj.x, pj.y, pj.z, q.x, q.y, q.z)
                                  : sq_dist2(pj.x, pj.y, q.x, q.y);
         if (d2 < k.h2) {
-            const KW kh = kernel_f32(d2, k.h2, k.hf, k.inv_h, k.c0h, k.c1h);
+            // W from the select-free spline (no division): r and 1/r from one rsq
+            const float d2f = (float)d2;
+            const float rinv = d2f > 0.f ? __builtin_amdgcn_rsqf(d2f) : 0.f;
+            const float w = spline_w_ab(spline_ab(d2f * rinv, k.inv_h), k.c0h);
             const float4 vb = bvel[j];
             float4 nb = bnrm[j];
             if (nb.x == 0.f && nb.y == 0.f && (D == 2 || nb.z == 0.f)) {
                 // a grain: the unit vector from the grain to the particle
                 // (none when they coincide)
-                const float inv = kh.r > 0.f ? 1.f / kh.r : 0.f;
-                nb.x = (float)(q.x - pj.x) * inv;
-                nb.y = (float)(q.y - pj.y) * inv;
-                nb.z = D == 3 ? (float)(q.z - pj.z) * inv : 0.f;
+                nb.x = (float)(q.x - pj.x) * rinv;
+                nb.y = (float)(q.y - pj.y) * rinv;
+                nb.z = D == 3 ? (float)(q.z - pj.z) * rinv : 0.f;
             }
             const float dn = nb.x * (vi.x - vb.x) + nb.y * (vi.y - vb.y) +
                              (D == 3 ? nb.z * (vi.z - vb.z) : 0.f);
             const float m = dn < 0.f ? dn : 0.f;
-            const float sc = vb.w * m * kh.w;
+            const float sc = vb.w * m * w;
             fx -= sc * nb.x;
             fy -= sc * nb.y;
             if (D == 3) fz -= sc * nb.z;
