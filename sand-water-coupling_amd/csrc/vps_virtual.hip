@@ -393,7 +393,7 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
     // separate arrays: x from the row's first cell for the membership test
     // (which reads 12 bytes per candidate as three 4-byte loads), and from
     // its own cell for the sums
-    __shared__ float s_px[kStageCap], s_py[kStageCap], s_pz[kStageCap], s_pw[kStageCap];
+    __shared__ float s_px[kStageCap], s_py[kStageCap], s_pz[kStageCap];
     __shared__ int s_cs[kStageRows][kStageCx + 1];  // real cell starts (staged)
     __shared__ int s_bcs[kStageRows][kStageCx + 1]; // boundary cell starts
     __shared__ int s_rg[kStageRows];    // global - staged index
@@ -493,7 +493,6 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
                     s_px[e] = xr;
                     s_py[e] = pj[u].y;
                     s_pz[e] = pj[u].z;
-                    s_pw[e] = pj[u].x;
                 }
             }
         }
@@ -527,10 +526,10 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
     const int zlo = max(cc2 - 1, 0), zhi = min(cc2 + 1, k.cdims[2] - 1);
     const int ylo = max(cc1 - 1, 0), yhi = min(cc1 + 1, k.cdims[1] - 1);
     // The (up to) three cells xlo .. xhi of a row are one contiguous staged
-    // range.  The membership test takes x relative to the same corner C_clo
-    // for all of them (one offset per node; its larger rounding stays inside
-    // the FP64 band); a member's sums use its accurate offset from its own
-    // cell corner with that cell's q - C.
+    // range.  The membership test and a member's sums take x relative to the
+    // same corner C_clo for all of them (one offset per node, rounded once
+    // from FP64: within 2e-7 h for offsets of up to three cells, inside the
+    // FP64 band for the test and far below the tolerance of the sums).
     const float tx = (float)(qx - cell_corner(k, 0, s_meta[0]));
     float tx0 = (float)(qx - cell_corner(k, 0, xlo));
     float tx1 = (float)(qx - cell_corner(k, 0, xlo + 1));
@@ -570,14 +569,12 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
             const int r = (z - s_meta[2]) * ncy + (y - s_meta[1]);
             const int rg = s_rg[r];
             const int jb = s_cs[r][i0 + (dl2 >= rem)], je = s_cs[r][i1 + 1 - (dr2 >= rem)];
-            const int b1 = s_cs[r][min(i0 + 1, i1 + 1)];
-            const int b2 = s_cs[r][min(i0 + 2, i1 + 1)];
             // Two passes per chunk of 32 candidates: the cheap FP32 test
             // sets one bit per member, then each lane visits only its own
             // members (a lane has ~15% of its candidates within h, so the
             // wave no longer runs the accumulation for every candidate).
-            // Bits enter at the bottom: candidate c0 + i sits at bit
-            // c1 - 1 - (c0 + i), so the highest bit is the next in order.
+            // Candidate c0 + i sits at bit i, so the lowest set bit is the
+            // next in order.
             for (int c0 = jb; c0 < je; c0 += 32) {
                 const int c1 = min(c0 + 32, je);
                 uint32_t mfast = 0u, mexact = 0u;
@@ -586,23 +583,20 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
                     const float d2f = dx * dx + dy * dy + dz * dz;
                     const bool fast = __float_as_uint(d2f) - lo_min - 1u < lo_span;
                     const bool exact = !(d2f >= k.h2f_hi) & !fast;   // NaN: exact
-                    mfast = mfast + mfast + (uint32_t)fast;
-                    mexact = mexact + mexact + (uint32_t)exact;
+                    mfast |= (uint32_t)fast << (j - c0);
+                    mexact |= (uint32_t)exact << (j - c0);
                 }
                 while (mfast) {
-                    const int p = 31 - __builtin_clz(mfast);
-                    mfast ^= 1u << p;
-                    const int j = c1 - 1 - p;
-                    const float txc = j < b1 ? tx0 : (j < b2 ? tx1 : tx2);
-                    const float dx = txc - s_pw[j], dy = ty - s_py[j], dz = tz - s_pz[j];
+                    const int j = c0 + __builtin_ctz(mfast);
+                    mfast &= mfast - 1u;
+                    const float dx = tx - s_px[j], dy = ty - s_py[j], dz = tz - s_pz[j];
                     const float d2f = dx * dx + dy * dy + dz * dz;
                     a.add_fast<3>(k, d2f, dx, dy, dz, rvel[j + rg], rrinv[j + rg]);
                 }
                 while (mexact) {
                     // the reference's FP64 test and the accurate spline
-                    const int p = 31 - __builtin_clz(mexact);
-                    mexact ^= 1u << p;
-                    const int j = c1 - 1 - p;
+                    const int j = c0 + __builtin_ctz(mexact);
+                    mexact &= mexact - 1u;
                     const int g = j + rg;
                     const double4 pj = rpos[g];
                     const double d2 = sq_dist3(pj.x, pj.y, pj.z, qx, qy, qz);
