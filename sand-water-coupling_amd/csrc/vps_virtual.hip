@@ -389,10 +389,10 @@ __global__ void __launch_bounds__(512) k_virtual_sums3s(
     const float4* rofs, const int32_t* rcstart, const double4* bpos,
     const float4* bofs, const float4* bvel, const float4* bnrm, const int32_t* bcstart,
     float* vrho, float* vvol, float* vb, uint32_t* bits) {
-    // x_j - C_clo (NaN: exact test only), y_j - C_j, z_j - C_j, x_j - C_j, as
-    // separate arrays: x from the row's first cell for the membership test
-    // (which reads 12 bytes per candidate as three 4-byte loads), and from
-    // its own cell for the sums
+    // x_j - C_clo (NaN: exact test only), y_j - C_j, z_j - C_j as separate
+    // arrays (the membership test and the sums read 12 bytes per candidate
+    // as three 4-byte loads); x from the row's first cell, y and z from the
+    // particle's own cell
     __shared__ float s_px[kStageCap], s_py[kStageCap], s_pz[kStageCap];
     __shared__ int s_cs[kStageRows][kStageCx + 1];  // real cell starts (staged)
     __shared__ int s_bcs[kStageRows][kStageCx + 1]; // boundary cell starts
